@@ -1,0 +1,176 @@
+/*
+ * vxfea.h -- C ABI of the MI355X (gfx950) matrix-free hex8 MG-PCG solver.
+ *
+ * Plain C types only: sizes are int64_t, arrays are raw pointers.  Unless a
+ * comment says "host", every array argument is a DEVICE pointer that the
+ * caller owns (the Python side passes torch-ROCm allocations).  The library
+ * owns only the workspace inside the opaque handle.  All work is enqueued on
+ * the handle's stream (vx_set_stream, default: the null stream) and calls
+ * that return results to the host synchronise that stream.
+ *
+ * Conventions are the reference solver's (core/fea.py:129-148):
+ *   element e = iz + nz*(iy + ny*ix)   (x slowest, z fastest)
+ *   node    n = ix + (nx+1)*(iy + (ny+1)*iz)   (x fastest)
+ *   DOF       3*n + c
+ *
+ * Every call returns VX_OK (0) or one of the negative status codes below;
+ * vx_last_error(h) gives a message for the last failure on a handle.
+ */
+#ifndef VXFEA_H
+#define VXFEA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  VX_OK = 0,
+  VX_ERR_ARG = -1,            /* bad argument / call order */
+  VX_ERR_NOMEM = -2,          /* device allocation failed */
+  VX_ERR_HIP = -3,            /* HIP runtime error */
+  VX_ERR_STALL = -4,          /* MG-PCG stall guard fired (core/multigrid.py:524-531) */
+  VX_ERR_NOT_CONVERGED = -5,  /* max_iter reached above tol (a warning: u is valid) */
+  VX_ERR_NO_DEVICE = -6
+};
+
+/* Precision of the operator / V-cycle work. */
+enum { VX_FP64 = 0, VX_MIXED = 1 };
+/* Smoother inside the V-cycle. */
+enum { VX_SMOOTH_JACOBI = 0, VX_SMOOTH_CHEBYSHEV = 1 };
+/* Coarsest-level treatment. */
+enum { VX_COARSE_SWEEPS = 0, VX_COARSE_DIRECT = 1 };
+
+typedef struct vx_solver* vx_handle;
+
+/* Solver options.  vx_default_options() fills the reference's defaults:
+ * max_levels 6, min_elems 4, n_smooth 2, omega 0.6, 30 coarsest sweeps,
+ * e_min 1e-9 (core/multigrid.py:85-87, core/fea.py:244-246). */
+typedef struct {
+  int32_t struct_size;        /* sizeof(vx_options) */
+  int32_t max_levels;
+  int32_t min_elems;
+  int32_t n_smooth;
+  double omega;
+  int32_t coarse_sweeps;
+  int32_t precision;          /* VX_FP64 | VX_MIXED */
+  int32_t smoother;           /* VX_SMOOTH_* */
+  int32_t coarse_mode;        /* VX_COARSE_* */
+  int32_t cheb_degree;        /* sweeps per Chebyshev smoothing step */
+  double cheb_ratio;          /* lower end of the Chebyshev interval / lambda_max */
+  double inner_tol;           /* mixed: relative tolerance of each FP32 inner solve */
+  int32_t max_outer;          /* mixed: max iterative-refinement steps */
+  int32_t z_chunk;            /* node planes per work-group in the tile kernels (0: auto) */
+  int32_t reserved[8];
+} vx_options;
+
+/* Per-solve statistics (mirrors MGSolver.last_* of core/multigrid.py:482-517). */
+typedef struct {
+  int32_t iters;              /* PCG iterations (mixed: total inner iterations) */
+  int32_t converged;          /* 1 if the ratio dropped below tol */
+  int32_t outer_iters;        /* mixed: iterative-refinement steps; FP64: 0 */
+  int32_t n_levels;
+  double resid_ratio;         /* last recurrence ratio ||r||/||f_free|| */
+  double true_resid_ratio;    /* ||f_free - A u|| / ||f_free|| recomputed in FP64 */
+  double solve_ms;            /* wall time of the solve on the handle's stream */
+} vx_info;
+
+/* Called synchronously on the caller's thread after every PCG iteration
+ * with (iteration, resid_ratio, tol) -- core/multigrid.py:518-519. */
+typedef void (*vx_progress_cb)(int32_t iteration, double resid_ratio, double tol,
+                               void* user);
+
+void vx_default_options(vx_options* opt);
+
+/* Number of visible devices (0 when no GPU / no driver). */
+int vx_device_count(void);
+
+/* Replaces MGSolver.__init__ (core/multigrid.py:85-174) minus the fixed DOFs:
+ * builds the level hierarchy of an nx*ny*nz grid with Poisson ratio nu. */
+int vx_create(int32_t nx, int32_t ny, int32_t nz, double nu, const vx_options* opt,
+              vx_handle* out);
+void vx_destroy(vx_handle h);
+const char* vx_last_error(vx_handle h);
+int vx_set_stream(vx_handle h, void* hip_stream);
+int vx_num_levels(vx_handle h);
+/* dims of level l (elements) */
+int vx_level_dims(vx_handle h, int32_t level, int32_t* nx, int32_t* ny, int32_t* nz);
+
+/* MGSolver._build_free (core/multigrid.py:255-278): fixed_dofs is a HOST array
+ * of original DOF ids; coarse masks are injected from the collocated fine node. */
+int vx_set_fixed(vx_handle h, const int64_t* fixed_dofs_host, int64_t n_fixed);
+
+/* MGSolver.set_density (core/multigrid.py:281-315) with the caller's void rule
+ * of VoxelFEA._solve_multigrid (core/fea.py:467-469): E_dev is the per-element
+ * stiffness scale (E*vsize) in reference element order; where active_dev (uint8,
+ * may be NULL = all active) is 0 the element gets E*e_min. */
+int vx_set_density(vx_handle h, const double* E_dev, const uint8_t* active_dev,
+                   double e_min);
+
+/* MGSolver.solve (core/multigrid.py:439-583): f_dev, u_dev, x0_dev (may be NULL)
+ * are [3*(nx+1)*(ny+1)*(nz+1)] float64.  Returns VX_ERR_STALL when the stall
+ * guard fires (u_dev then holds the last iterate), VX_ERR_NOT_CONVERGED when
+ * max_iter ran out.  hist_host (may be NULL) receives up to hist_cap
+ * per-iteration ratios. */
+int vx_solve(vx_handle h, const double* f_dev, double* u_dev, const double* x0_dev,
+             double tol, int32_t max_iter, vx_progress_cb cb, void* user,
+             double* hist_host, int32_t hist_cap, vx_info* info);
+
+/* VoxelFEA._solve_jacobi (core/fea.py:497-593): Jacobi-preconditioned CG on
+ * the active-DOF system (inactive elements dropped, no e_min, loads on nodes
+ * of no active element ignored).  E_dev/active_dev in reference element
+ * order, f_dev/u_dev full-grid float64; u is 0 on DOFs outside the system.
+ * max_iter <= 0 means 4 * (number of free DOFs). */
+int vx_solve_jacobi(vx_handle h, const double* E_dev, const uint8_t* active_dev,
+                    const double* f_dev, double* u_dev, double tol, int32_t max_iter,
+                    vx_progress_cb cb, void* user, double* hist_host, int32_t hist_cap,
+                    vx_info* info);
+
+/* Single-operator entry points for parity tests.  All float64, level-l sized.
+ * vx_matvec: MGSolver._apply  (masked K u, core/multigrid.py:318-341)
+ * vx_smooth: MGSolver._smooth (iters damped-Jacobi sweeps, :367-402)
+ * vx_restrict: MGSolver._restrict (R = P^T then coarse mask, :404-416)
+ * vx_prolong: MGSolver._prolongate (P e then fine mask, :418-424)
+ * vx_vcycle: MGSolver._vcycle(0, b) (:426-436)
+ * precision selects the FP64 or FP32 kernels (inputs/outputs stay float64). */
+int vx_matvec(vx_handle h, int32_t level, int32_t precision, const double* u, double* y);
+int vx_smooth(vx_handle h, int32_t level, int32_t precision, const double* u,
+              const double* b, int32_t iters, double* out);
+int vx_restrict(vx_handle h, int32_t level, const double* r_fine, double* r_coarse);
+int vx_prolong(vx_handle h, int32_t level, const double* e_coarse, double* e_fine);
+int vx_vcycle(vx_handle h, int32_t precision, const double* b, double* z);
+/* Inverse Jacobi diagonal of level l per DOF (MGSolver.set_density Minv). */
+int vx_inv_diag(vx_handle h, int32_t level, double* minv);
+
+/* VoxelFEA.element_von_mises_stress (core/fea.py:604-626): vm[e] for every
+ * element in reference order, 0 where active_dev is 0. */
+int vx_von_mises(vx_handle h, const double* u_dev, const double* E_dev,
+                 const uint8_t* active_dev, double* vm_dev);
+
+/* VoxelFEA.element_strain_energy (core/fea.py:595-602): u_e^T KE u_e with
+ * unit E for active elements, 0 elsewhere; reference element order. */
+int vx_strain_energy(vx_handle h, const double* u_dev, const uint8_t* active_dev,
+                     double* ce_dev);
+
+/* drop_unsupported_islands (core/fea.py:165-227) on the device: kept_dev gets
+ * the active elements 6-connected to an element touching a fixed node; if
+ * none is reached kept = active and *n_removed = 0. */
+int vx_drop_islands(vx_handle h, const uint8_t* active_dev, const int64_t* fixed_dofs_host,
+                    int64_t n_fixed, uint8_t* kept_dev, int64_t* n_removed_host);
+
+/* Timing helper: average duration (ms) of `reps` back-to-back launches of the
+ * level-l operator kernel (kind 0 = masked matvec, 1 = Jacobi sweep) in the
+ * given precision, measured with HIP events on the handle's stream. */
+int vx_time_kernel(vx_handle h, int32_t kind, int32_t level, int32_t precision,
+                   int32_t reps, double* avg_ms);
+
+/* Host-side element operator (the kernels' own code compiled for the CPU):
+ * y24 = KE(nu) * u24 for one unit element in the reference's local DOF order
+ * (core/fea.py:33-36), E = 1. */
+void vx_element_apply_host(double nu, const double* u24, double* y24);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VXFEA_H */

@@ -1,0 +1,125 @@
+"""Write golden solve traces from the reference solver itself.
+
+The per-iteration residual history, the iteration count, the stall-guard
+decision and the Jacobi fallback are properties of the whole iteration that
+no closed form pins, so they are recorded from the reference's own
+``core.fea.VoxelFEA`` / ``core.multigrid.MGSolver`` (imported as top-level
+``core.*`` with the reference directory on ``sys.path``, the way its solve
+worker does, core/solve_worker.py:76-83).  Inputs and outputs are stored as
+small ``.npz`` fixtures next to this script; the tests compare the oracle and
+the GPU solver against them.  Run from the repository root:
+
+    python tests/golden/make_golden.py /root/reference
+"""
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+def _load_workloads():
+    """The generators live in this repository's ``core`` package, whose name
+    the reference's package shares: load the module from its file."""
+    import importlib.util
+    path = os.path.join(HERE, "..", "..", "voxelsim-fea_amd", "core",
+                        "workloads.py")
+    spec = importlib.util.spec_from_file_location("vx_workloads", path)
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["vx_workloads"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+W = _load_workloads()
+
+
+def small_patch_problem(n=8):
+    """Solid n^3 cube held by a single bottom node, loaded on the top face:
+    the point-bearing case that stalls geometric MG (core/MULTIGRID_NOTES.md:19-24)."""
+    nx = ny = nz = n
+    c = n // 2
+    node = W.node_index(c, c, 0, nx, ny)
+    fixed = W.fix_nodes([node])
+    # plus two roller nodes so the continuum problem is not singular
+    fixed = np.unique(np.concatenate([
+        fixed, W.fix_nodes([W.node_index(0, 0, 0, nx, ny)], comps=(1, 2)),
+        W.fix_nodes([W.node_index(n, 0, 0, nx, ny)], comps=(2,))]))
+    top = W.face_nodes(nx, ny, nz, 2, 1)
+    ndof = 3 * (n + 1) ** 3
+    force = W.spread_load(ndof, top, (0.0, 0.0, -100.0))
+    return W.Problem("point_bearing", nx, ny, nz, 1e-3, 0.3,
+                     np.ones(n ** 3, dtype=bool), np.full(n ** 3, 200e9),
+                     fixed, force, tol=1e-6)
+
+
+def point_stall_problem(n=8):
+    """Solid n^3 cube held by ONE node: the system keeps rotational rigid-body
+    modes, MG-PCG trips its stall guard (core/multigrid.py:524-531) and the
+    solve falls back to Jacobi-PCG (core/fea.py:476-489)."""
+    nx = ny = nz = n
+    fixed = W.fix_nodes([W.node_index(0, 0, 0, nx, ny)])
+    nm = W.element_nodes_mask(np.ones((n, n, n), dtype=bool)).ravel()
+    load = np.nonzero(nm)[0][-4:]
+    force = W.spread_load(3 * (n + 1) ** 3, load, (0.0, 0.0, -10.0))
+    return W.Problem("point_stall", nx, ny, nz, 1e-3, 0.3, np.ones(n ** 3, dtype=bool),
+                     np.full(n ** 3, 200e9), fixed, force, tol=1e-6)
+
+
+def cases():
+    out = []
+    out.append(("point_stall", point_stall_problem(8), {"max_cg": 60}))
+    out.append(("cantilever", W.cantilever(), {}))
+    p = W.voids_block(n=16, n_spheres=6, seed=12)
+    out.append(("voids16", p, {}))
+    p = W.two_material(n=16, seed=15)
+    out.append(("twomat16", p, {}))
+    p = W.multi_rhs_part(n=16, nrhs=2, seed=13)
+    p.force = p.force[0]
+    out.append(("part16", p, {}))
+    out.append(("point_bearing", small_patch_problem(8), {}))
+    p = W.cantilever(8, 4, 4)
+    out.append(("cantilever_jacobi", p, {"use_multigrid": False}))
+    p = W.voids_block(n=12, n_spheres=4, seed=21)
+    out.append(("voids12_x0", p, {"x0_scale": 0.5}))
+    return out
+
+
+def run(ref_root):
+    sys.path.insert(0, ref_root)
+    from core.fea import VoxelFEA                       # reference
+    for name, p, opt in cases():
+        vf = VoxelFEA(p.nx, p.ny, p.nz, nu=p.nu, active_elems=p.active,
+                      compute_mode="CPU",
+                      use_multigrid=opt.get("use_multigrid", True))
+        vf.set_fixed(p.fixed_dofs)
+        if p.max_levels != 6:
+            from core.multigrid import MGSolver         # reference
+            vf._mg = MGSolver(p.nx, p.ny, p.nz, p.fixed_dofs, nu=p.nu,
+                              max_levels=p.max_levels, compute_mode="CPU")
+        x0 = None
+        if "x0_scale" in opt:
+            x0 = np.random.default_rng(7).standard_normal(p.ndof) * 1e-9
+        u = vf.solve(p.E_stiffness, p.force, tol=p.tol, x0=x0, max_cg=opt.get("max_cg"))
+        vm = vf.element_von_mises_stress(u, p.E_pa) / p.vsize
+        fell_back = not vf.use_multigrid and opt.get("use_multigrid", True)
+        vf.close()
+        np.savez_compressed(
+            os.path.join(HERE, f"{name}.npz"),
+            nx=p.nx, ny=p.ny, nz=p.nz, nu=p.nu, vsize=p.vsize,
+            active=p.active, E_pa=p.E_pa, fixed_dofs=p.fixed_dofs,
+            force=p.force, tol=p.tol, max_levels=p.max_levels,
+            use_multigrid=opt.get("use_multigrid", True),
+            max_cg=int(opt.get("max_cg") or 0),
+            x0=np.zeros(0) if x0 is None else x0,
+            u=u, vm=vm, last_iters=vf.last_iters,
+            last_converged=bool(vf.last_converged),
+            resid_history=np.asarray(vf.resid_history, dtype=float),
+            fell_back=bool(fell_back))
+        print(f"{name}: iters={vf.last_iters} converged={vf.last_converged} "
+              f"fell_back={fell_back} |u|max={np.abs(u).max():.3e}")
+
+
+if __name__ == "__main__":
+    run(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")

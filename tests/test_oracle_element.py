@@ -1,0 +1,190 @@
+"""Pins of the oracle's element kernels (core/fea.py:31-148) against closed
+forms and invariants, and of the HIP library's host element operator."""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle.fea_ref import hex8_KE, hex8_B_C, build_edof, node_id, von_mises
+
+NU = [0.0, 0.3, 0.33, 0.35, 0.42]
+
+
+def _mode_KE(nu):
+    """Independent derivation: energy of the trilinear field in the monomial
+    basis under 2x2x2 Gauss (gradient monomials are orthogonal), assembled
+    with a general isotropic C(G) = lam tr(G) I + mu (G + G^T)."""
+    lam = nu / ((1 + nu) * (1 - 2 * nu))
+    mu = 1 / (2 * (1 + nu))
+    signs = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1],
+                      [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]], float)
+    sx, sy, sz = signs.T
+    chi = {"x": sx, "y": sy, "z": sz, "xy": sx * sy, "yz": sy * sz, "zx": sz * sx,
+           "xyz": sx * sy * sz}
+
+    def C(G):
+        return lam * np.trace(G) * np.eye(3) + mu * (G + G.T)
+
+    KE = np.zeros((24, 24))
+    for j in range(24):
+        u = np.zeros((8, 3))
+        u[j // 3, j % 3] = 1.0
+        h = {m: chi[m] @ u for m in chi}
+        # gradient coefficient matrices (d/dx = 2 d/dxi, u = sum h_m m / 8)
+        G0 = np.stack([h["x"], h["y"], h["z"]], 1) / 4
+        Gxi = np.zeros((3, 3)); Gxi[:, 1] = h["xy"] / 4; Gxi[:, 2] = h["zx"] / 4
+        Get = np.zeros((3, 3)); Get[:, 0] = h["xy"] / 4; Get[:, 2] = h["yz"] / 4
+        Gze = np.zeros((3, 3)); Gze[:, 0] = h["zx"] / 4; Gze[:, 1] = h["yz"] / 4
+        Gyz = np.zeros((3, 3)); Gyz[:, 0] = h["xyz"] / 4
+        Gxz = np.zeros((3, 3)); Gxz[:, 1] = h["xyz"] / 4
+        Gxy = np.zeros((3, 3)); Gxy[:, 2] = h["xyz"] / 4
+        S0, Sxi, Set, Sze = C(G0), C(Gxi), C(Get), C(Gze)
+        yh = {"x": S0[:, 0] / 4, "y": S0[:, 1] / 4, "z": S0[:, 2] / 4,
+              "xy": (Sxi[:, 1] + Set[:, 0]) / 12, "yz": (Set[:, 2] + Sze[:, 1]) / 12,
+              "zx": (Sxi[:, 2] + Sze[:, 0]) / 12,
+              "xyz": (C(Gyz)[:, 0] + C(Gxz)[:, 1] + C(Gxy)[:, 2]) / 36}
+        y = sum(np.outer(chi[m], yh[m]) for m in yh)
+        KE[:, j] = y.ravel()
+    return KE
+
+
+def _rigid_modes():
+    signs = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
+                      [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], float)
+    modes = []
+    for c in range(3):
+        m = np.zeros((8, 3)); m[:, c] = 1; modes.append(m.ravel())
+    for a, b in ((0, 1), (1, 2), (2, 0)):
+        m = np.zeros((8, 3)); m[:, a] = -signs[:, b]; m[:, b] = signs[:, a]; modes.append(m.ravel())
+    return np.array(modes).T
+
+
+@pytest.mark.parametrize("nu", NU)
+def test_KE_matches_independent_mode_derivation(nu):
+    KE = hex8_KE(1.0, nu)
+    assert np.abs(KE - _mode_KE(nu)).max() < 1e-14 * np.abs(KE).max()
+
+
+@pytest.mark.parametrize("nu", NU)
+def test_KE_invariants(nu):
+    KE = hex8_KE(1.0, nu)
+    assert np.allclose(KE, KE.T, atol=1e-16)
+    w = np.linalg.eigvalsh(KE)
+    assert w.min() > -1e-13
+    assert np.sum(np.abs(w) < 1e-12) == 6
+    R = _rigid_modes()
+    assert np.abs(KE @ R).max() < 1e-14
+    assert np.allclose(hex8_KE(3.5, nu), 3.5 * KE, rtol=1e-15)
+    assert np.ptp(np.diag(KE)) < 1e-15
+
+
+@pytest.mark.parametrize("nu", [0.3, 0.42])
+def test_patch_test_uniform_strain(nu):
+    """u = G x on the corners: KE u_e = B0^T C sym(G) (unit volume)."""
+    rng = np.random.default_rng(3)
+    G = rng.standard_normal((3, 3))
+    corners = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
+                        [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], float)
+    u = (corners @ G.T).ravel()
+    B0, C0 = hex8_B_C(nu)
+    eps = np.array([G[0, 0], G[1, 1], G[2, 2], G[0, 1] + G[1, 0], G[1, 2] + G[2, 1],
+                    G[2, 0] + G[0, 2]])
+    assert np.allclose(B0 @ u, eps, atol=1e-15)
+    assert np.allclose(hex8_KE(1.0, nu) @ u, B0.T @ (C0 @ eps), atol=1e-13)
+
+
+def test_B_C_closed_form():
+    B0, C0 = hex8_B_C(0.3)
+    assert set(np.unique(np.round(np.abs(B0[B0 != 0]), 15))) == {0.25}
+    c = 1 / (1.3 * 0.4)
+    assert np.isclose(C0[0, 0], c * 0.7) and np.isclose(C0[0, 1], c * 0.3)
+    assert np.isclose(C0[3, 3], c * 0.2)
+
+
+def test_von_mises_closed_forms():
+    nu, E = 0.3, 200e9
+    corners = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0],
+                        [0, 0, 1], [1, 0, 1], [0, 1, 1], [1, 1, 1]], float)  # node order x-fastest
+    sig = 123e6
+    # uniaxial stress sigma_xx: eps = (s/E, -nu s/E, -nu s/E)
+    G = np.diag([sig / E, -nu * sig / E, -nu * sig / E])
+    u = (corners @ G.T).ravel()
+    vm = von_mises(1, 1, 1, nu, u, [E], [True])
+    assert np.isclose(vm[0], sig, rtol=1e-12)
+    # pure shear tau_xy: gamma = tau / mu
+    tau = 50e6
+    mu = E / (2 * (1 + nu))
+    G = np.zeros((3, 3)); G[0, 1] = tau / mu
+    u = (corners @ G.T).ravel()
+    vm = von_mises(1, 1, 1, nu, u, [E], [True])
+    assert np.isclose(vm[0], np.sqrt(3) * tau, rtol=1e-12)
+    assert von_mises(1, 1, 1, nu, u, [E], [False])[0] == 0.0
+
+
+def test_build_edof_brute_force():
+    nx, ny, nz = 3, 2, 4
+    edof = build_edof(nx, ny, nz)
+    offs = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+    e = 0
+    for ix in range(nx):
+        for iy in range(ny):
+            for iz in range(nz):
+                want = []
+                for ox, oy, oz in offs:
+                    n = (ix + ox) + (nx + 1) * ((iy + oy) + (ny + 1) * (iz + oz))
+                    want += [3 * n, 3 * n + 1, 3 * n + 2]
+                assert list(edof[e]) == want
+                e += 1
+    assert node_id(1, 1, 1, nx, ny) == 1 + 4 * (1 + 3)
+
+
+# ---------------------------------------------------------------------------
+# the HIP library loads on the CPU and exports its ABI; its host element
+# operator (the kernels' own code) equals KE
+# ---------------------------------------------------------------------------
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "voxelsim-fea_amd", "lib", "libvxfea.so")
+HDR = os.path.join(ROOT, "include", "vxfea.h")
+
+
+def _lib():
+    if not os.path.exists(LIB):
+        pytest.fail("libvxfea.so not built (run __graft_entry__.build())")
+    return ctypes.CDLL(LIB)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib()
+    text = open(HDR).read()
+    names = set(re.findall(r"\b(vx_[a-z_0-9]+)\s*\(", text))
+    assert len(names) >= 20
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+    from core import _native
+    assert names <= set(_native.SIGNATURES), names - set(_native.SIGNATURES)
+
+
+@pytest.mark.parametrize("nu", NU + [0.49])
+def test_library_host_element_operator(nu):
+    lib = _lib()
+    lib.vx_element_apply_host.argtypes = [ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]
+    KE = hex8_KE(1.0, nu)
+    Y = np.zeros((24, 24))
+    y = np.zeros(24)
+    for j in range(24):
+        u = np.zeros(24)
+        u[j] = 1.0
+        lib.vx_element_apply_host(nu, u.ctypes.data, y.ctypes.data)
+        Y[:, j] = y
+    assert np.abs(Y - KE).max() < 1e-15 * np.abs(KE).max() * 8
+
+
+def test_default_options_are_reference_defaults():
+    from core import _native
+    o = _native.default_options()
+    assert (o.max_levels, o.min_elems, o.n_smooth, o.coarse_sweeps) == (6, 4, 2, 30)
+    assert o.omega == 0.6 and o.precision == _native.VX_FP64
+    assert o.struct_size == ctypes.sizeof(_native.vx_options)

@@ -1,0 +1,230 @@
+"""Geometric-multigrid-preconditioned CG on the GPU (API of
+core/multigrid.py:76-583).
+
+``MGSolver`` keeps the reference's constructor and ``solve`` contract --
+full regular grid, voids carried by the caller at ``E*e_min``, damped-Jacobi
+V(n_smooth, n_smooth) cycle with a 30-sweep coarsest level, Fletcher-Reeves
+PCG with the stall guard, numpy arrays in and out -- and runs every operation
+in the HIP library (``include/vxfea.h``).  Extra keyword options select the
+MI355X forms: ``precision="mixed"`` (FP32 V-cycle and matvec inside FP64
+iterative refinement).
+
+``solve_device`` is the device-resident form: torch tensors in HBM in and
+out, no host copies (what the benchmark times).
+"""
+
+import ctypes
+
+import numpy as np
+
+from . import _native
+from . import backend
+from . import compute_plan
+
+_PRECISION = {"fp64": _native.VX_FP64, "mixed": _native.VX_MIXED}
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _dev(a, dtype=None):
+    """numpy / torch -> contiguous CUDA tensor (float64 unless dtype)."""
+    torch = _torch()
+    dt = torch.float64 if dtype is None else dtype
+    if isinstance(a, torch.Tensor):
+        return a.to(device="cuda", dtype=dt).contiguous()
+    return torch.as_tensor(np.ascontiguousarray(a), device="cuda").to(dt).contiguous()
+
+
+def _require_gpu(plan, what):
+    if not plan.on_gpu:
+        raise RuntimeError(
+            f"{what}: no usable GPU ({backend.gpu_status()}); this build solves on "
+            f"MI355X only (compute plan: {plan.label})")
+
+
+class MGSolver:
+    """Multigrid-preconditioned CG on the full voxel grid (HIP)."""
+
+    def __init__(self, nx, ny, nz, fixed_dofs, nu=0.3, xp=None, n_smooth=2, omega=0.6,
+                 min_elems=4, max_levels=6, compute_mode="AUTO", cpu_threads=0,
+                 verbose=False, precision="fp64", coarse_sweeps=30, inner_tol=1e-5,
+                 max_outer=20, z_chunk=0):
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.nelem = self.nx * self.ny * self.nz
+        self.ndof = 3 * (self.nx + 1) * (self.ny + 1) * (self.nz + 1)
+        self.n_smooth, self.omega, self.verbose = n_smooth, omega, verbose
+        self.precision = precision
+        self.plan = compute_plan.choose(self.ndof, mode=compute_mode, cpu_threads=cpu_threads,
+                                        verbose=verbose)
+        _require_gpu(self.plan, "MGSolver")
+        self.xp = self.plan.xp
+        self._lib = _native.load()
+        opt = _native.default_options(
+            max_levels=max_levels, min_elems=min_elems, n_smooth=n_smooth, omega=omega,
+            coarse_sweeps=coarse_sweeps, precision=_PRECISION[precision],
+            inner_tol=inner_tol, max_outer=max_outer, z_chunk=z_chunk)
+        h = ctypes.c_void_p()
+        _native.check(self._lib.vx_create(self.nx, self.ny, self.nz, float(nu),
+                                          ctypes.byref(opt), ctypes.byref(h)))
+        self._h = h
+        self._stream_bind()
+        self.levels = []
+        for li in range(self._lib.vx_num_levels(h)):
+            a, b, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+            self._lib.vx_level_dims(h, li, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+            d = (a.value, b.value, c.value)
+            self.levels.append({"dims": d, "ndof": 3 * (d[0] + 1) * (d[1] + 1) * (d[2] + 1),
+                                "g": float(2 ** li)})
+        fd = np.ascontiguousarray(np.asarray(fixed_dofs, dtype=np.int64).ravel())
+        _native.check(self._lib.vx_set_fixed(h, fd.ctypes.data_as(ctypes.c_void_p),
+                                             fd.size), h)
+        self.last_iters = 0
+        self.last_converged = False
+        self.last_resid_ratio = float("nan")
+        self.last_true_resid_ratio = float("nan")
+        self.last_outer_iters = 0
+        self.last_solve_ms = float("nan")
+        self.resid_history = []
+
+    # -- plumbing ----------------------------------------------------------
+    def _stream_bind(self):
+        torch = _torch()
+        self._lib.vx_set_stream(self._h, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            self._lib.vx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, allow=()):
+        return _native.check(rc, self._h, allow)
+
+    # -- density -----------------------------------------------------------
+    def set_density(self, Evec_full, active=None, e_min=1e-9):
+        """Per-level E, 2x2x2 coarsening (core/multigrid.py:281-315).  With
+        ``active`` (bool per element) the void rule E*e_min of
+        core/fea.py:467-469 is applied on the device."""
+        self._E_dev = _dev(Evec_full)
+        self._active_dev = None if active is None else _dev(active, _torch().uint8)
+        self._stream_bind()
+        self._check(self._lib.vx_set_density(self._h, _native.ptr(self._E_dev),
+                                              _native.ptr(self._active_dev), float(e_min)))
+
+    # -- solve -------------------------------------------------------------
+    def solve_device(self, f_dev, u_dev, tol=1e-4, max_cg=None, x0_dev=None,
+                     progress_cb=None, hist_cap=100000):
+        """Device-resident solve on the current density; returns the native
+        status (0 ok, VX_ERR_NOT_CONVERGED) and fills the last_* fields.
+        Raises ``_native.StallError`` when the stall guard fires."""
+        self._stream_bind()
+        hist = np.zeros(hist_cap)
+        info = _native.vx_info()
+        user_cb = progress_cb
+
+        def _cb(it, ratio, t, _user):
+            if user_cb is not None:
+                user_cb(int(it), float(ratio), float(t))
+
+        cb = _native.PROGRESS_CB(_cb) if progress_cb is not None else _native.PROGRESS_CB()
+        rc = self._lib.vx_solve(self._h, _native.ptr(f_dev), _native.ptr(u_dev),
+                                _native.ptr(x0_dev), float(tol), int(max_cg or 0), cb, None,
+                                hist.ctypes.data_as(ctypes.c_void_p), hist_cap,
+                                ctypes.byref(info))
+        self.last_iters = info.iters
+        self.last_converged = bool(info.converged)
+        self.last_resid_ratio = info.resid_ratio
+        self.last_true_resid_ratio = info.true_resid_ratio
+        self.last_outer_iters = info.outer_iters
+        self.last_solve_ms = info.solve_ms
+        self.resid_history = [float(v) for v in hist[:min(info.iters, hist_cap)]]
+        self._check(rc, allow=(_native.VX_ERR_NOT_CONVERGED,))
+        return rc
+
+    def solve(self, Evec_full, f, tol=1e-4, max_cg=None, x0=None, progress_cb=None):
+        """core/multigrid.py:439-583; returns numpy u (masked)."""
+        self.set_density(Evec_full)
+        f_dev = _dev(f)
+        u_dev = _torch().empty_like(f_dev)
+        x0_dev = None if x0 is None else _dev(x0)
+        rc = self.solve_device(f_dev, u_dev, tol=tol, max_cg=max_cg, x0_dev=x0_dev,
+                               progress_cb=progress_cb)
+        if rc == _native.VX_ERR_NOT_CONVERGED:
+            print(f"[VoxelSim FEA] WARNING: MG-PCG did not converge in "
+                  f"{max_cg or 1000} iterations (tol={tol:g}, final residual ratio="
+                  f"{self.last_resid_ratio:.3e}, plan={self.plan.label}). Returned "
+                  f"displacement may be inaccurate.")
+        if self.verbose:
+            print(f"[VoxelSim FEA] MG solve: {self.last_iters} CG iters "
+                  f"(converged={self.last_converged}), {self.last_solve_ms:.1f} ms on device")
+        return u_dev.cpu().numpy()
+
+    # -- single operators (parity tests) -----------------------------------
+    def _level_vec(self, li, a):
+        return _dev(a)
+
+    def _apply(self, li, u, precision="fp64"):
+        """Masked K u at level li (core/multigrid.py:318-341)."""
+        ud = _dev(u)
+        yd = _torch().empty_like(ud)
+        self._stream_bind()
+        self._check(self._lib.vx_matvec(self._h, li, _PRECISION[precision], _native.ptr(ud),
+                                        _native.ptr(yd)))
+        return yd.cpu().numpy()
+
+    def _smooth(self, li, u, b, iters, precision="fp64"):
+        """``iters`` damped-Jacobi sweeps (core/multigrid.py:367-402); u=None
+        starts from zero."""
+        bd = _dev(b)
+        ud = None if u is None else _dev(u)
+        od = _torch().empty_like(bd)
+        self._stream_bind()
+        self._check(self._lib.vx_smooth(self._h, li, _PRECISION[precision], _native.ptr(ud),
+                                        _native.ptr(bd), int(iters), _native.ptr(od)))
+        return od.cpu().numpy()
+
+    def _restrict(self, li, r_fine):
+        rf = _dev(r_fine)
+        rc = _torch().empty(self.levels[li + 1]["ndof"], dtype=_torch().float64, device="cuda")
+        self._stream_bind()
+        self._check(self._lib.vx_restrict(self._h, li, _native.ptr(rf), _native.ptr(rc)))
+        return rc.cpu().numpy()
+
+    def _prolongate(self, li, e_coarse):
+        ec = _dev(e_coarse)
+        ef = _torch().empty(self.levels[li]["ndof"], dtype=_torch().float64, device="cuda")
+        self._stream_bind()
+        self._check(self._lib.vx_prolong(self._h, li, _native.ptr(ec), _native.ptr(ef)))
+        return ef.cpu().numpy()
+
+    def _vcycle(self, li, b, precision="fp64"):
+        if li != 0:
+            raise ValueError("the V-cycle entry point starts at level 0")
+        bd = _dev(b)
+        zd = _torch().empty_like(bd)
+        self._stream_bind()
+        self._check(self._lib.vx_vcycle(self._h, _PRECISION[precision], _native.ptr(bd),
+                                        _native.ptr(zd)))
+        return zd.cpu().numpy()
+
+    def inv_diag(self, li):
+        md = _torch().empty(self.levels[li]["ndof"], dtype=_torch().float64, device="cuda")
+        self._stream_bind()
+        self._check(self._lib.vx_inv_diag(self._h, li, _native.ptr(md)))
+        return md.cpu().numpy()
+
+    def time_kernel(self, kind="matvec", level=0, precision="fp64", reps=20):
+        """Average device time (ms) of one level operator launch."""
+        out = ctypes.c_double()
+        self._stream_bind()
+        self._check(self._lib.vx_time_kernel(self._h, {"matvec": 0, "jacobi": 1}[kind], level,
+                                             _PRECISION[precision], reps, ctypes.byref(out)))
+        return out.value

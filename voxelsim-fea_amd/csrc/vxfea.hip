@@ -1,0 +1,1113 @@
+// vxfea.hip -- host side of the MI355X hex8 MG-PCG solver and its C ABI.
+//
+// One handle = one voxel grid with its multigrid hierarchy (the reference's
+// MGSolver, core/multigrid.py:76-583) and the workspace of one solve.  The
+// PCG scalars live in device memory; the host reads back one small struct
+// per iteration (for the convergence test, the stall guard and the progress
+// callback) while the next V-cycle is already queued.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+#include "vxfea.h"
+
+using namespace vx;
+
+namespace {
+
+constexpr int EW_BLOCKS = 1024;   // grid of the elementwise / reduction kernels
+constexpr int EW_THREADS = 256;
+
+struct Level {
+  int nx, ny, nz, Lx, Ly, Lz;
+  long long Nn, Ne;
+  uint8_t* mask = nullptr;
+  double* E64 = nullptr;
+  float* E32 = nullptr;
+  // V-cycle work vectors (levels >= 1 own b; x/t at every level)
+  double *b64 = nullptr, *x64 = nullptr, *t64 = nullptr;
+  float *b32 = nullptr, *x32 = nullptr, *t32 = nullptr;
+};
+
+template <typename T> struct Sel;
+template <> struct Sel<double> {
+  static double*& b(Level& l) { return l.b64; }
+  static double*& x(Level& l) { return l.x64; }
+  static double*& t(Level& l) { return l.t64; }
+  static double* E(Level& l) { return l.E64; }
+};
+template <> struct Sel<float> {
+  static float*& b(Level& l) { return l.b32; }
+  static float*& x(Level& l) { return l.x32; }
+  static float*& t(Level& l) { return l.t32; }
+  static float* E(Level& l) { return l.E32; }
+};
+
+}  // namespace
+
+struct vx_solver {
+  int nx, ny, nz;
+  double nu;
+  vx_options opt;
+  std::vector<Level> lv;
+  hipStream_t stream = nullptr;
+  std::string err;
+  ElemConst<double> K64;
+  ElemConst<float> K32;
+  double kd = 0;
+  bool fixed_set = false, density_set = false;
+  // level-0 PCG vectors
+  double *u64 = nullptr, *r64 = nullptr, *z64 = nullptr, *p64 = nullptr, *Ap64 = nullptr,
+         *fm64 = nullptr;
+  float *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *Ap32 = nullptr, *d32 = nullptr;
+  Level jac;                  // active-DOF system of the Jacobi-PCG fallback
+  double* jac_minv = nullptr; // per node
+  double* partial = nullptr;
+  int n_partial = 0;
+  Scalars* dscal = nullptr;
+  Scalars* hscal = nullptr;   // pinned
+  std::vector<void*> allocs;
+};
+
+namespace {
+
+int fail(vx_solver* s, int code, const std::string& msg) {
+  if (s) s->err = msg;
+  return code;
+}
+
+#define HIPCHK(call)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(s, VX_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define LAUNCHCHK()                                                                           \
+  do {                                                                                        \
+    hipError_t e_ = hipGetLastError();                                                        \
+    if (e_ != hipSuccess)                                                                     \
+      return fail(s, VX_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+#define CHK(x)             \
+  do {                     \
+    int rc_ = (x);         \
+    if (rc_ != VX_OK) return rc_; \
+  } while (0)
+
+template <typename T>
+int dalloc(vx_solver* s, T** p, size_t n) {
+  if (*p) return VX_OK;
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(s, VX_ERR_NOMEM, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed");
+  }
+  s->allocs.push_back(q);
+  *p = static_cast<T*>(q);
+  return VX_OK;
+}
+
+int ew_grid(long long n) {
+  long long g = (n + EW_THREADS - 1) / EW_THREADS;
+  return (int)std::max<long long>(1, std::min<long long>(g, EW_BLOCKS));
+}
+
+int zchunk(const vx_solver* s, const Level& L) {
+  if (s->opt.z_chunk > 0) return s->opt.z_chunk;
+  // enough work-groups to fill 256 CUs several times, long enough columns
+  // to amortise the halo layer.
+  const long long cols = (long long)((L.Lx + OWN - 1) / OWN) * ((L.Ly + OWN - 1) / OWN);
+  int zc = L.Lz;
+  while (zc > 16 && cols * ((L.Lz + zc - 1) / zc) < 2048) zc = (zc + 1) / 2;
+  return std::max(zc, 1);
+}
+
+dim3 tile_grid(const vx_solver* s, const Level& L, int zc) {
+  return dim3((L.Lx + OWN - 1) / OWN, (L.Ly + OWN - 1) / OWN, (L.Lz + zc - 1) / zc);
+}
+
+template <typename T> ElemConst<T> kconst(const vx_solver* s);
+template <> ElemConst<double> kconst<double>(const vx_solver* s) { return s->K64; }
+template <> ElemConst<float> kconst<float>(const vx_solver* s) { return s->K32; }
+
+template <typename T, int MODE, bool DOT>
+int launch_tile_L(vx_solver* s, Level& L, const T* u, const T* b, T* out, double* partial,
+                  int* n_part) {
+  TileArgs<T> a;
+  a.nx = L.nx; a.ny = L.ny; a.nz = L.nz;
+  a.Lx = L.Lx; a.Ly = L.Ly; a.Lz = L.Lz;
+  a.zc = zchunk(s, L);
+  a.u = u; a.E = Sel<T>::E(L); a.mask = L.mask; a.b = b; a.out = out;
+  a.omega = (T)s->opt.omega;
+  a.kd = (T)s->kd;
+  a.K = kconst<T>(s);
+  a.partial = partial;
+  dim3 g = tile_grid(s, L, a.zc);
+  if (n_part) *n_part = (int)(g.x * g.y * g.z);
+  if (DOT && (long long)g.x * g.y * g.z > s->n_partial)
+    return fail(s, VX_ERR_ARG, "partial buffer too small");
+  hipLaunchKernelGGL((k_tile<T, MODE, DOT>), g, dim3(NT), 0, s->stream, a);
+  LAUNCHCHK();
+  return VX_OK;
+}
+
+template <typename T, int MODE, bool DOT>
+int launch_tile(vx_solver* s, int li, const T* u, const T* b, T* out, double* partial,
+                int* n_part) {
+  return launch_tile_L<T, MODE, DOT>(s, s->lv[li], u, b, out, partial, n_part);
+}
+
+int finalize(vx_solver* s, int n, int op) {
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s->stream, s->partial, n, op, s->dscal);
+  LAUNCHCHK();
+  return VX_OK;
+}
+
+// one damped-Jacobi sweep; u == nullptr means "from zero"
+template <typename T>
+int sweep(vx_solver* s, int li, const T* u, const T* b, T* out, bool dot, int* n_part) {
+  Level& L = s->lv[li];
+  if (!u) {
+    hipLaunchKernelGGL(k_jacobi_zero<T>, dim3(ew_grid(L.Nn)), dim3(EW_THREADS), 0, s->stream,
+                       L.nx, L.ny, L.nz, Sel<T>::E(L), L.mask, b, out, (T)s->opt.omega,
+                       (T)s->kd);
+    LAUNCHCHK();
+    if (dot) {
+      hipLaunchKernelGGL(k_dot<T>, dim3(EW_BLOCKS), dim3(EW_THREADS), 0, s->stream, 3 * L.Nn,
+                         out, b, s->partial);
+      LAUNCHCHK();
+      if (n_part) *n_part = EW_BLOCKS;
+    }
+    return VX_OK;
+  }
+  if (dot) return launch_tile<T, M_JACOBI, true>(s, li, u, b, out, s->partial, n_part);
+  return launch_tile<T, M_JACOBI, false>(s, li, u, b, out, nullptr, nullptr);
+}
+
+// k sweeps starting at u_in (nullptr = zero) writing the result to `dst`,
+// using `tmp` as the other ping-pong buffer.  dot: last sweep adds partials
+// of out.b.
+template <typename T>
+int smooth_n(vx_solver* s, int li, const T* u_in, const T* b, int k, T* dst, T* tmp, bool dot,
+             int* n_part) {
+  // choose the first target so that the k-th write lands in dst
+  T* bufs[2] = {dst, tmp};
+  int cur = (k % 2 == 1) ? 0 : 1;   // first write goes to bufs[cur]
+  const T* src = u_in;
+  for (int i = 0; i < k; ++i) {
+    T* o = bufs[cur];
+    if (src == o) return fail(s, VX_ERR_ARG, "smoother aliasing");
+    CHK(sweep<T>(s, li, src, b, o, dot && i == k - 1, n_part));
+    src = o;
+    cur ^= 1;
+  }
+  return VX_OK;
+}
+
+template <typename T>
+int alloc_level_vectors(vx_solver* s) {
+  for (size_t li = 0; li < s->lv.size(); ++li) {
+    Level& L = s->lv[li];
+    const size_t n = 3 * (size_t)L.Nn;
+    if (li > 0) CHK(dalloc(s, &Sel<T>::b(L), n));
+    CHK(dalloc(s, &Sel<T>::x(L), n));
+    CHK(dalloc(s, &Sel<T>::t(L), n));
+  }
+  return VX_OK;
+}
+
+// V-cycle (core/multigrid.py:426-436): b at level 0 -> result in z0.
+// t0 and res0 are level-0 scratch vectors (distinct from b0, z0).
+// dot: partials of z0.b0 from the last level-0 write.
+template <typename T>
+int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, bool dot, int* n_part) {
+  const int nl = (int)s->lv.size();
+  const int ns = s->opt.n_smooth;
+  // descend
+  for (int li = 0; li < nl; ++li) {
+    Level& L = s->lv[li];
+    const T* b = li == 0 ? b0 : Sel<T>::b(L);
+    T* x = li == 0 ? z0 : Sel<T>::x(L);
+    T* t = li == 0 ? t0 : Sel<T>::t(L);
+    if (li == nl - 1) {
+      CHK(smooth_n<T>(s, li, nullptr, b, s->opt.coarse_sweeps, x, t, dot && li == 0, n_part));
+      break;
+    }
+    CHK(smooth_n<T>(s, li, nullptr, b, ns, x, t, false, nullptr));
+    // residual into t, then restrict into b of level li+1
+    CHK((launch_tile<T, M_RESID, false>(s, li, x, b, t, nullptr, nullptr)));
+    Level& C = s->lv[li + 1];
+    hipLaunchKernelGGL(k_restrict<T>, dim3(ew_grid(C.Nn)), dim3(EW_THREADS), 0, s->stream, C.Lx,
+                       C.Ly, C.Lz, t, C.mask, Sel<T>::b(C));
+    LAUNCHCHK();
+  }
+  // ascend
+  for (int li = nl - 2; li >= 0; --li) {
+    Level& L = s->lv[li];
+    Level& C = s->lv[li + 1];
+    const T* b = li == 0 ? b0 : Sel<T>::b(L);
+    T* x = li == 0 ? z0 : Sel<T>::x(L);
+    T* t = li == 0 ? t0 : Sel<T>::t(L);
+    hipLaunchKernelGGL(k_prolong_add<T>, dim3(ew_grid(L.Nn)), dim3(EW_THREADS), 0, s->stream, L.Lx,
+                       L.Ly, L.Lz, Sel<T>::x(C), L.mask, x, x);
+    LAUNCHCHK();
+    // post-smoothing from x, result back in x
+    if (ns % 2 == 0) {
+      CHK(smooth_n<T>(s, li, x, b, ns, x, t, dot && li == 0, n_part));
+    } else {
+      CHK(smooth_n<T>(s, li, x, b, ns, t, x, dot && li == 0, n_part));
+      HIPCHK(hipMemcpyAsync(x, t, sizeof(T) * 3 * L.Nn, hipMemcpyDeviceToDevice, s->stream));
+      if (dot && li == 0) {
+        // partials were computed on t == x content: fine
+      }
+    }
+  }
+  return VX_OK;
+}
+
+int require_ready(vx_solver* s) {
+  if (!s->fixed_set) return fail(s, VX_ERR_ARG, "call vx_set_fixed() before solving");
+  if (!s->density_set) return fail(s, VX_ERR_ARG, "call vx_set_density() before solving");
+  return VX_OK;
+}
+
+int read_scalars(vx_solver* s, hipEvent_t ev) {
+  HIPCHK(hipMemcpyAsync(s->hscal, s->dscal, sizeof(Scalars), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipEventRecord(ev, s->stream));
+  return VX_OK;
+}
+
+struct PcgResult {
+  int iters = 0;
+  bool converged = false;
+  bool stalled = false;
+  double ratio = NAN;
+};
+
+// Preconditioned CG of core/multigrid.py:474-546 on level-0 vectors of type T.
+// On entry r holds the masked initial residual and u the initial iterate;
+// the device scalar fnorm is the ratio's denominator.  The loop stops when
+// ||r||/fnorm < tol_rel; the history and the callback see that ratio times
+// ratio_scale (1 for a plain solve; the outer residual ratio for an inner
+// solve of iterative refinement) and cb_tol.
+template <typename T>
+int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* Ap, T* t0, double tol_rel, int max_iter,
+        bool stall_guard, double ratio_scale, double cb_tol, int iter_offset, vx_progress_cb cb,
+        void* user, double* hist, int hist_cap, PcgResult* res) {
+  Level& L0 = s->lv[0];
+  const long long n = 3 * L0.Nn;
+  int np = 0;
+  hipEvent_t ev;
+  HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  auto cleanup = [&]() { (void)hipEventDestroy(ev); };
+
+  int rc = vcycle<T>(s, r, z, t0, true, &np);
+  if (rc == VX_OK) rc = finalize(s, np, F_RZ0);
+  if (rc != VX_OK) { cleanup(); return rc; }
+  if (hipMemcpyAsync(p, z, sizeof(T) * n, hipMemcpyDeviceToDevice, s->stream) != hipSuccess) {
+    cleanup();
+    return fail(s, VX_ERR_HIP, "copy z->p");
+  }
+  double best = INFINITY;
+  double prev_ratio = NAN, prev2_ratio = NAN;
+  res->iters = 0;
+  res->converged = false;
+  res->ratio = NAN;
+  for (int it = 0; it < max_iter; ++it) {
+    rc = launch_tile<T, M_MATVEC, true>(s, 0, p, nullptr, Ap, s->partial, &np);
+    if (rc == VX_OK) rc = finalize(s, np, F_ALPHA);
+    if (rc != VX_OK) break;
+    hipLaunchKernelGGL(k_cg_update<T>, dim3(EW_BLOCKS), dim3(EW_THREADS), 0, s->stream, n,
+                       s->dscal, u, r, p, Ap, s->partial);
+    if (hipGetLastError() != hipSuccess) { rc = fail(s, VX_ERR_HIP, "cg_update launch"); break; }
+    rc = finalize(s, EW_BLOCKS, F_RATIO);
+    if (rc == VX_OK) rc = read_scalars(s, ev);
+    if (rc != VX_OK) break;
+    // Queue the next preconditioner application while the host waits for the
+    // ratio, unless convergence looks imminent (then waiting is cheaper than
+    // a wasted V-cycle).
+    bool predicted_done = false;
+    if (!std::isnan(prev_ratio) && !std::isnan(prev2_ratio) && prev2_ratio > 0) {
+      const double next = prev_ratio * (prev_ratio / prev2_ratio);
+      predicted_done = next < 3.0 * tol_rel;
+    }
+    auto queue_next = [&]() -> int {
+      int np2 = 0;
+      CHK(vcycle<T>(s, r, z, t0, true, &np2));
+      CHK(finalize(s, np2, F_BETA));
+      hipLaunchKernelGGL(k_p_update<T>, dim3(EW_BLOCKS), dim3(EW_THREADS), 0, s->stream, n,
+                         s->dscal, z, p);
+      LAUNCHCHK();
+      return VX_OK;
+    };
+    bool queued = false;
+    if (!predicted_done) {
+      rc = queue_next();
+      if (rc != VX_OK) break;
+      queued = true;
+    }
+    if (hipEventSynchronize(ev) != hipSuccess) { rc = fail(s, VX_ERR_HIP, "event sync"); break; }
+    const Scalars sc = *s->hscal;
+    if (sc.pAp == 0.0) {
+      res->converged = res->ratio < tol_rel;
+      break;
+    }
+    const double ratio = sc.ratio;
+    res->iters = it + 1;
+    res->ratio = ratio;
+    const int gi = iter_offset + it;
+    if (hist && gi < hist_cap) hist[gi] = ratio * ratio_scale;
+    if (cb) cb(gi + 1, ratio * ratio_scale, cb_tol, user);
+    if (ratio < tol_rel) {
+      res->converged = true;
+      break;
+    }
+    best = std::min(best, ratio);
+    if (stall_guard && it + 1 >= 10 && ratio > 5.0 * best) {
+      res->stalled = true;
+      char buf[200];
+      snprintf(buf, sizeof buf,
+               "MG-PCG stalled/diverged at iter %d (resid ratio %.3e, best seen %.3e)", it + 1,
+               ratio, best);
+      s->err = buf;
+      break;
+    }
+    prev2_ratio = prev_ratio;
+    prev_ratio = ratio;
+    if (!queued) {
+      rc = queue_next();
+      if (rc != VX_OK) break;
+    }
+  }
+  cleanup();
+  return rc;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+void vx_default_options(vx_options* o) {
+  std::memset(o, 0, sizeof(*o));
+  o->struct_size = sizeof(vx_options);
+  o->max_levels = 6;
+  o->min_elems = 4;
+  o->n_smooth = 2;
+  o->omega = 0.6;
+  o->coarse_sweeps = 30;
+  o->precision = VX_FP64;
+  o->smoother = VX_SMOOTH_JACOBI;
+  o->coarse_mode = VX_COARSE_SWEEPS;
+  o->cheb_degree = 2;
+  o->cheb_ratio = 0.3;
+  o->inner_tol = 1e-5;
+  o->max_outer = 20;
+  o->z_chunk = 0;
+}
+
+int vx_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+const char* vx_last_error(vx_handle h) { return h ? h->err.c_str() : "null handle"; }
+
+void vx_element_apply_host(double nu, const double* u24, double* y24) {
+  ElemConst<double> K = elem_const(nu);
+  double u[8][3], y[8][3];
+  for (int k = 0; k < 8; ++k)
+    for (int c = 0; c < 3; ++c) u[ref_to_bin(k)][c] = u24[3 * k + c];
+  elem_apply<double>(u, 1.0, K, y);
+  for (int k = 0; k < 8; ++k)
+    for (int c = 0; c < 3; ++c) y24[3 * k + c] = y[ref_to_bin(k)][c];
+}
+
+int vx_create(int32_t nx, int32_t ny, int32_t nz, double nu, const vx_options* opt,
+              vx_handle* out) {
+  vx_solver* s = nullptr;
+  if (!out) return VX_ERR_ARG;
+  *out = nullptr;
+  if (nx < 1 || ny < 1 || nz < 1 || !(nu > -1.0 && nu < 0.5)) return VX_ERR_ARG;
+  if (vx_device_count() < 1) return VX_ERR_NO_DEVICE;
+  s = new vx_solver();
+  s->nx = nx; s->ny = ny; s->nz = nz; s->nu = nu;
+  if (opt) {
+    if (opt->struct_size != (int32_t)sizeof(vx_options)) {
+      delete s;
+      return VX_ERR_ARG;
+    }
+    s->opt = *opt;
+  } else {
+    vx_default_options(&s->opt);
+  }
+  if (s->opt.n_smooth < 1 || s->opt.coarse_sweeps < 1 || s->opt.max_levels < 1) {
+    delete s;
+    return VX_ERR_ARG;
+  }
+  s->K64 = elem_const(nu);
+  s->K32 = {(float)s->K64.k1, (float)s->K64.k2, (float)s->K64.b1, (float)s->K64.b2,
+            (float)s->K64.t1};
+  s->kd = elem_diag(nu);
+  // hierarchy rule of core/multigrid.py:141-148
+  std::vector<std::array<int, 3>> dims;
+  dims.push_back({nx, ny, nz});
+  while ((int)dims.size() < s->opt.max_levels) {
+    auto d = dims.back();
+    const int mn = std::min(d[0], std::min(d[1], d[2]));
+    if (d[0] % 2 || d[1] % 2 || d[2] % 2) break;
+    if (!(mn / 2 >= s->opt.min_elems / 2 && mn >= 2)) break;
+    dims.push_back({d[0] / 2, d[1] / 2, d[2] / 2});
+  }
+  long long maxpart = 0;
+  for (auto& d : dims) {
+    Level L;
+    L.nx = d[0]; L.ny = d[1]; L.nz = d[2];
+    L.Lx = d[0] + 1; L.Ly = d[1] + 1; L.Lz = d[2] + 1;
+    L.Nn = (long long)L.Lx * L.Ly * L.Lz;
+    L.Ne = (long long)L.nx * L.ny * L.nz;
+    s->lv.push_back(L);
+  }
+  for (auto& L : s->lv) {
+    const int zc = zchunk(s, L);
+    dim3 g = tile_grid(s, L, zc);
+    maxpart = std::max<long long>(maxpart, (long long)g.x * g.y * g.z);
+  }
+  s->n_partial = (int)std::max<long long>(maxpart, EW_BLOCKS);
+  int rc = VX_OK;
+  for (auto& L : s->lv) {
+    if ((rc = dalloc(s, &L.mask, L.Nn)) != VX_OK) break;
+    if ((rc = dalloc(s, &L.E64, L.Ne)) != VX_OK) break;
+    if ((rc = dalloc(s, &L.E32, L.Ne)) != VX_OK) break;
+  }
+  if (rc == VX_OK) rc = dalloc(s, &s->partial, (size_t)s->n_partial);
+  if (rc == VX_OK) rc = dalloc(s, &s->dscal, 1);
+  if (rc == VX_OK && hipHostMalloc((void**)&s->hscal, sizeof(Scalars)) != hipSuccess)
+    rc = VX_ERR_NOMEM;
+  if (rc != VX_OK) {
+    vx_destroy(s);
+    return rc;
+  }
+  *out = s;
+  return VX_OK;
+}
+
+void vx_destroy(vx_handle s) {
+  if (!s) return;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  else (void)hipDeviceSynchronize();
+  for (void* p : s->allocs) (void)hipFree(p);
+  if (s->hscal) (void)hipHostFree(s->hscal);
+  delete s;
+}
+
+int vx_set_stream(vx_handle s, void* stream) {
+  if (!s) return VX_ERR_ARG;
+  s->stream = (hipStream_t)stream;
+  return VX_OK;
+}
+
+int vx_num_levels(vx_handle s) { return s ? (int)s->lv.size() : VX_ERR_ARG; }
+
+int vx_level_dims(vx_handle s, int32_t l, int32_t* nx, int32_t* ny, int32_t* nz) {
+  if (!s || l < 0 || l >= (int)s->lv.size()) return VX_ERR_ARG;
+  *nx = s->lv[l].nx; *ny = s->lv[l].ny; *nz = s->lv[l].nz;
+  return VX_OK;
+}
+
+int vx_set_fixed(vx_handle s, const int64_t* fixed, int64_t n_fixed) {
+  if (!s || (n_fixed > 0 && !fixed)) return fail(s, VX_ERR_ARG, "bad fixed-DOF arguments");
+  Level& L0 = s->lv[0];
+  std::vector<uint8_t> m(L0.Nn, 7);
+  for (int64_t i = 0; i < n_fixed; ++i) {
+    const int64_t d = fixed[i];
+    if (d < 0 || d >= 3 * L0.Nn) return fail(s, VX_ERR_ARG, "fixed DOF id out of range");
+    m[d / 3] &= (uint8_t)~(1u << (d % 3));
+  }
+  HIPCHK(hipMemcpyAsync(L0.mask, m.data(), L0.Nn, hipMemcpyHostToDevice, s->stream));
+  for (size_t li = 1; li < s->lv.size(); ++li) {
+    Level& C = s->lv[li];
+    hipLaunchKernelGGL(k_inject_mask, dim3(ew_grid(C.Nn)), dim3(EW_THREADS), 0, s->stream, C.Lx,
+                       C.Ly, C.Lz, s->lv[li - 1].mask, C.mask);
+    LAUNCHCHK();
+  }
+  HIPCHK(hipStreamSynchronize(s->stream));
+  s->fixed_set = true;
+  return VX_OK;
+}
+
+int vx_set_density(vx_handle s, const double* E, const uint8_t* active, double e_min) {
+  if (!s || !E) return fail(s, VX_ERR_ARG, "null density");
+  Level& L0 = s->lv[0];
+  hipLaunchKernelGGL(k_import_E, dim3(ew_grid(L0.Ne)), dim3(EW_THREADS), 0, s->stream, L0.nx,
+                     L0.ny, L0.nz, E, active, e_min, L0.E64, L0.E32);
+  LAUNCHCHK();
+  for (size_t li = 1; li < s->lv.size(); ++li) {
+    Level& F = s->lv[li - 1];
+    Level& C = s->lv[li];
+    hipLaunchKernelGGL(k_coarsen_E<double>, dim3(ew_grid(C.Ne)), dim3(EW_THREADS), 0, s->stream,
+                       C.nx, C.ny, C.nz, F.E64, C.E64);
+    LAUNCHCHK();
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(ew_grid(C.Ne)), dim3(EW_THREADS), 0,
+                       s->stream, C.Ne, C.E64, C.E32);
+    LAUNCHCHK();
+  }
+  s->density_set = true;
+  return VX_OK;
+}
+
+}  // extern "C"
+
+static int alloc_pcg64(vx_solver* s) {
+  const size_t n = 3 * (size_t)s->lv[0].Nn;
+  CHK(dalloc(s, &s->r64, n));
+  CHK(dalloc(s, &s->z64, n));
+  CHK(dalloc(s, &s->p64, n));
+  CHK(dalloc(s, &s->Ap64, n));
+  CHK(dalloc(s, &s->fm64, n));
+  return alloc_level_vectors<double>(s);
+}
+
+static int alloc_pcg32(vx_solver* s) {
+  const size_t n = 3 * (size_t)s->lv[0].Nn;
+  CHK(dalloc(s, &s->r32, n));
+  CHK(dalloc(s, &s->z32, n));
+  CHK(dalloc(s, &s->p32, n));
+  CHK(dalloc(s, &s->Ap32, n));
+  CHK(dalloc(s, &s->d32, n));
+  return alloc_level_vectors<float>(s);
+}
+
+extern "C" {
+
+int vx_solve(vx_handle s, const double* f, double* u, const double* x0, double tol,
+             int32_t max_iter, vx_progress_cb cb, void* user, double* hist, int32_t hist_cap,
+             vx_info* info) {
+  if (!s || !f || !u) return fail(s, VX_ERR_ARG, "null solve argument");
+  CHK(require_ready(s));
+  if (max_iter <= 0) max_iter = 1000;   // core/multigrid.py:481
+  const bool mixed = s->opt.precision == VX_MIXED;
+  CHK(alloc_pcg64(s));
+  if (mixed) CHK(alloc_pcg32(s));
+  Level& L0 = s->lv[0];
+  const long long n = 3 * L0.Nn;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, s->stream));
+
+  // f masked; fnorm = ||f_free|| (core/multigrid.py:460-471)
+  hipLaunchKernelGGL((k_mask_copy<double, double>), dim3(EW_BLOCKS), dim3(EW_THREADS), 0,
+                     s->stream, L0.Nn, L0.mask, f, s->fm64, s->partial);
+  LAUNCHCHK();
+  CHK(finalize(s, EW_BLOCKS, F_FNORM));
+  // initial iterate
+  if (x0) {
+    hipLaunchKernelGGL((k_mask_copy<double, double>), dim3(EW_BLOCKS), dim3(EW_THREADS), 0,
+                       s->stream, L0.Nn, L0.mask, x0, u, nullptr);
+    LAUNCHCHK();
+  } else {
+    HIPCHK(hipMemsetAsync(u, 0, sizeof(double) * n, s->stream));
+  }
+  HIPCHK(hipMemcpyAsync(s->hscal, s->dscal, sizeof(Scalars), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  const double fnorm = s->hscal->fnorm;
+  vx_info inf{};
+  inf.n_levels = (int)s->lv.size();
+  if (fnorm == 0.0) {
+    HIPCHK(hipMemsetAsync(u, 0, sizeof(double) * n, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    inf.converged = 1;
+    inf.resid_ratio = 0.0;
+    inf.true_resid_ratio = 0.0;
+    if (info) *info = inf;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return VX_OK;
+  }
+  int rc = VX_OK;
+  PcgResult pr;
+  int np = 0;
+  if (!mixed) {
+    // r = f - A u  (or f when u = 0)
+    if (x0) {
+      rc = launch_tile<double, M_RESID, false>(s, 0, u, s->fm64, s->r64, nullptr, nullptr);
+    } else if (hipMemcpyAsync(s->r64, s->fm64, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                              s->stream) != hipSuccess) {
+      rc = fail(s, VX_ERR_HIP, "copy f->r");
+    }
+    if (rc == VX_OK)
+      rc = pcg<double>(s, u, s->r64, s->z64, s->p64, s->Ap64, s->lv[0].t64, tol, max_iter, true,
+                       1.0, tol, 0, cb, user, hist, hist_cap, &pr);
+    inf.iters = pr.iters;
+    inf.converged = pr.converged ? 1 : 0;
+    inf.resid_ratio = pr.ratio;
+  } else {
+    // Iterative refinement: FP64 true residual, FP32 inner MG-PCG.
+    int total = 0;
+    double true_ratio = NAN;
+    for (int outer = 0; outer <= s->opt.max_outer && rc == VX_OK; ++outer) {
+      if (outer == 0 && !x0) {
+        rc = (hipMemcpyAsync(s->r64, s->fm64, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                             s->stream) == hipSuccess)
+                 ? VX_OK
+                 : fail(s, VX_ERR_HIP, "copy f->r");
+        if (rc == VX_OK) {
+          hipLaunchKernelGGL(k_dot<double>, dim3(EW_BLOCKS), dim3(EW_THREADS), 0, s->stream, n,
+                             s->r64, s->r64, s->partial);
+          np = EW_BLOCKS;
+        }
+      } else {
+        rc = launch_tile<double, M_RESID, true>(s, 0, u, s->fm64, s->r64, s->partial, &np);
+      }
+      if (rc == VX_OK) rc = finalize(s, np, F_SET_TMP);
+      if (rc != VX_OK) break;
+      if (hipMemcpyAsync(s->hscal, s->dscal, sizeof(Scalars), hipMemcpyDeviceToHost,
+                         s->stream) != hipSuccess ||
+          hipStreamSynchronize(s->stream) != hipSuccess) {
+        rc = fail(s, VX_ERR_HIP, "scalar readback");
+        break;
+      }
+      true_ratio = std::sqrt(s->hscal->tmp) / fnorm;
+      inf.outer_iters = outer;
+      if (true_ratio < tol || outer == s->opt.max_outer || total >= max_iter) break;
+      // inner solve of A d = r / ||r|| to inner_tol (tight enough for the last step)
+      hipLaunchKernelGGL(k_scale_to_f32, dim3(EW_BLOCKS), dim3(EW_THREADS), 0, s->stream, n,
+                         s->dscal, s->r64, s->r32);
+      LAUNCHCHK();
+      HIPCHK(hipMemsetAsync(s->d32, 0, sizeof(float) * n, s->stream));
+      const double want = std::max(s->opt.inner_tol, 0.5 * tol / true_ratio);
+      // the inner right-hand side has unit norm
+      hipLaunchKernelGGL(k_set_fnorm, dim3(1), dim3(1), 0, s->stream, s->dscal, 1.0);
+      LAUNCHCHK();
+      rc = pcg<float>(s, s->d32, s->r32, s->z32, s->p32, s->Ap32, s->lv[0].t32, want,
+                      max_iter - total, false, true_ratio, tol, total, cb, user, hist, hist_cap,
+                      &pr);
+      total += pr.iters;
+      if (rc != VX_OK) break;
+      hipLaunchKernelGGL(k_add_scaled, dim3(EW_BLOCKS), dim3(EW_THREADS), 0, s->stream, n,
+                         s->dscal, s->d32, u);
+      LAUNCHCHK();
+      if (pr.iters == 0) break;
+    }
+    inf.iters = total;
+    inf.resid_ratio = true_ratio;
+    inf.true_resid_ratio = true_ratio;
+    inf.converged = (true_ratio < tol) ? 1 : 0;
+  }
+  // final mask of u (core/multigrid.py:547) and the true FP64 residual
+  if (rc == VX_OK || rc == VX_ERR_STALL) {
+    hipLaunchKernelGGL((k_mask_copy<double, double>), dim3(EW_BLOCKS), dim3(EW_THREADS), 0,
+                       s->stream, L0.Nn, L0.mask, u, u, nullptr);
+    HIPCHK(hipEventRecord(e1, s->stream));
+    if (!mixed) {
+      int rc2 = launch_tile<double, M_RESID, true>(s, 0, u, s->fm64, s->z64, s->partial, &np);
+      if (rc2 == VX_OK) rc2 = finalize(s, np, F_SET_TMP);
+      if (rc2 == VX_OK) {
+        HIPCHK(hipMemcpyAsync(s->hscal, s->dscal, sizeof(Scalars), hipMemcpyDeviceToHost,
+                              s->stream));
+      }
+    }
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (!mixed) inf.true_resid_ratio = std::sqrt(s->hscal->tmp) / fnorm;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    inf.solve_ms = ms;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (info) *info = inf;
+  if (rc != VX_OK) return rc;
+  if (pr.stalled) return VX_ERR_STALL;
+  if (!inf.converged) return VX_ERR_NOT_CONVERGED;
+  return VX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Jacobi-PCG fallback on the active-DOF system (core/fea.py:497-593).
+//
+// The reference compresses the system to the DOFs of active elements and
+// drops void elements entirely (no e_min).  The same system is solved here
+// on the full grid: inactive elements get E = 0 and DOFs that no active
+// element touches are masked like fixed DOFs, so every dot product and
+// update runs over exactly the reference's reduced DOF set.
+// ---------------------------------------------------------------------------
+int vx_solve_jacobi(vx_handle s, const double* E, const uint8_t* active, const double* f,
+                    double* u, double tol, int32_t max_iter, vx_progress_cb cb, void* user,
+                    double* hist, int32_t hist_cap, vx_info* info) {
+  if (!s || !E || !f || !u) return fail(s, VX_ERR_ARG, "null Jacobi argument");
+  if (!s->fixed_set) return fail(s, VX_ERR_ARG, "call vx_set_fixed() before solving");
+  CHK(alloc_pcg64(s));
+  Level& L0 = s->lv[0];
+  Level& J = s->jac;
+  if (!J.mask) {
+    J = L0;
+    J.mask = nullptr; J.E64 = nullptr; J.E32 = nullptr;
+    J.b64 = J.x64 = J.t64 = nullptr;
+    J.b32 = J.x32 = J.t32 = nullptr;
+    CHK(dalloc(s, &J.mask, J.Nn));
+    CHK(dalloc(s, &J.E64, J.Ne));
+    CHK(dalloc(s, &J.E32, J.Ne));
+    CHK(dalloc(s, &s->jac_minv, J.Nn));
+  }
+  const long long n = 3 * J.Nn;
+  hipEvent_t e0, e1, ev;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  HIPCHK(hipEventRecord(e0, s->stream));
+  hipLaunchKernelGGL(k_import_E, dim3(ew_grid(J.Ne)), dim3(EW_THREADS), 0, s->stream, J.nx, J.ny,
+                     J.nz, E, active, 0.0, J.E64, J.E32);
+  LAUNCHCHK();
+  hipLaunchKernelGGL(k_jacobi_setup, dim3(ew_grid(J.Nn)), dim3(EW_THREADS), 0, s->stream, J.nx,
+                     J.ny, J.nz, active, L0.mask, J.E64, s->kd, J.mask, s->jac_minv);
+  LAUNCHCHK();
+  // f masked to the reduced free set; fnorm; count of free DOFs
+  hipLaunchKernelGGL((k_mask_copy<double, double>), dim3(EW_BLOCKS), dim3(EW_THREADS), 0,
+                     s->stream, J.Nn, J.mask, f, s->fm64, s->partial);
+  LAUNCHCHK();
+  CHK(finalize(s, EW_BLOCKS, F_FNORM));
+  hipLaunchKernelGGL(k_count_free, dim3(EW_BLOCKS), dim3(EW_THREADS), 0, s->stream, J.Nn, J.mask,
+                     s->partial);
+  LAUNCHCHK();
+  CHK(finalize(s, EW_BLOCKS, F_SET_TMP));
+  HIPCHK(hipMemsetAsync(u, 0, sizeof(double) * n, s->stream));
+  HIPCHK(hipMemcpyAsync(s->hscal, s->dscal, sizeof(Scalars), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  const double fnorm = s->hscal->fnorm;
+  const long long n_free = (long long)llround(s->hscal->tmp);
+  vx_info inf{};
+  inf.n_levels = 1;
+  int rc = VX_OK;
+  PcgResult res;
+  if (fnorm == 0.0) {
+    inf.converged = 1;
+  } else {
+    if (max_iter <= 0) max_iter = (int)std::min<long long>(4 * n_free, 2000000000LL);
+    double* r = s->r64;
+    double* z = s->z64;
+    double* p = s->p64;
+    double* Ap = s->Ap64;
+    int np = 0;
+    HIPCHK(hipMemcpyAsync(r, s->fm64, sizeof(double) * n, hipMemcpyDeviceToDevice, s->stream));
+    hipLaunchKernelGGL(k_diag_precond, dim3(EW_BLOCKS), dim3(EW_THREADS), 0, s->stream, J.Nn,
+                       J.mask, s->jac_minv, r, z, s->partial);
+    LAUNCHCHK();
+    CHK(finalize(s, EW_BLOCKS, F_RZ0));
+    HIPCHK(hipMemcpyAsync(p, z, sizeof(double) * n, hipMemcpyDeviceToDevice, s->stream));
+    for (int it = 0; it < max_iter; ++it) {
+      rc = launch_tile_L<double, M_MATVEC, true>(s, J, p, nullptr, Ap, s->partial, &np);
+      if (rc == VX_OK) rc = finalize(s, np, F_ALPHA);
+      if (rc != VX_OK) break;
+      hipLaunchKernelGGL(k_cg_update<double>, dim3(EW_BLOCKS), dim3(EW_THREADS), 0, s->stream, n,
+                         s->dscal, u, r, p, Ap, s->partial);
+      rc = finalize(s, EW_BLOCKS, F_RATIO);
+      if (rc == VX_OK) rc = read_scalars(s, ev);
+      if (rc != VX_OK) break;
+      hipLaunchKernelGGL(k_diag_precond, dim3(EW_BLOCKS), dim3(EW_THREADS), 0, s->stream, J.Nn,
+                         J.mask, s->jac_minv, r, z, s->partial);
+      rc = finalize(s, EW_BLOCKS, F_BETA);
+      if (rc != VX_OK) break;
+      hipLaunchKernelGGL(k_p_update<double>, dim3(EW_BLOCKS), dim3(EW_THREADS), 0, s->stream, n,
+                         s->dscal, z, p);
+      if (hipEventSynchronize(ev) != hipSuccess) { rc = fail(s, VX_ERR_HIP, "event sync"); break; }
+      const Scalars sc = *s->hscal;
+      if (sc.pAp == 0.0) {
+        res.converged = res.ratio < tol;
+        break;
+      }
+      res.iters = it + 1;
+      res.ratio = sc.ratio;
+      if (hist && it < hist_cap) hist[it] = sc.ratio;
+      if (cb) cb(it + 1, sc.ratio, tol, user);
+      if (sc.ratio < tol) {
+        res.converged = true;
+        break;
+      }
+    }
+    inf.iters = res.iters;
+    inf.converged = res.converged ? 1 : 0;
+    inf.resid_ratio = res.ratio;
+  }
+  if (rc == VX_OK) {
+    hipLaunchKernelGGL((k_mask_copy<double, double>), dim3(EW_BLOCKS), dim3(EW_THREADS), 0,
+                       s->stream, J.Nn, J.mask, u, u, nullptr);
+    HIPCHK(hipEventRecord(e1, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    inf.solve_ms = ms;
+    inf.true_resid_ratio = NAN;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipEventDestroy(ev);
+  if (info) *info = inf;
+  if (rc != VX_OK) return rc;
+  return inf.converged ? VX_OK : VX_ERR_NOT_CONVERGED;
+}
+
+// element_strain_energy (core/fea.py:595-602): u_e^T KE u_e with unit E.
+int vx_strain_energy(vx_handle s, const double* u, const uint8_t* active, double* ce) {
+  if (!s || !u || !ce) return fail(s, VX_ERR_ARG, "null strain-energy argument");
+  Level& L = s->lv[0];
+  hipLaunchKernelGGL(k_strain_energy, dim3(ew_grid(L.Ne)), dim3(EW_THREADS), 0, s->stream, L.nx,
+                     L.ny, L.nz, s->K64, u, active, ce);
+  LAUNCHCHK();
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return VX_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// single-operator entry points (float64 in/out)
+// ---------------------------------------------------------------------------
+static int check_level(vx_solver* s, int l) {
+  if (!s) return VX_ERR_ARG;
+  if (l < 0 || l >= (int)s->lv.size()) return fail(s, VX_ERR_ARG, "level out of range");
+  return require_ready(s);
+}
+
+template <typename T>
+static int to_T(vx_solver* s, const double* a, T* b, long long n) {
+  hipLaunchKernelGGL((k_convert<double, T>), dim3(ew_grid(n)), dim3(EW_THREADS), 0, s->stream, n,
+                     a, b);
+  LAUNCHCHK();
+  return VX_OK;
+}
+template <typename T>
+static int from_T(vx_solver* s, const T* a, double* b, long long n) {
+  hipLaunchKernelGGL((k_convert<T, double>), dim3(ew_grid(n)), dim3(EW_THREADS), 0, s->stream, n,
+                     a, b);
+  LAUNCHCHK();
+  return VX_OK;
+}
+
+extern "C" {
+
+int vx_matvec(vx_handle s, int32_t l, int32_t prec, const double* u, double* y) {
+  CHK(check_level(s, l));
+  Level& L = s->lv[l];
+  const long long n = 3 * L.Nn;
+  if (prec == VX_FP64) {
+    CHK((launch_tile<double, M_MATVEC, false>(s, l, u, nullptr, y, nullptr, nullptr)));
+  } else {
+    CHK(alloc_level_vectors<float>(s));
+    CHK(to_T<float>(s, u, L.x32, n));
+    CHK((launch_tile<float, M_MATVEC, false>(s, l, L.x32, nullptr, L.t32, nullptr, nullptr)));
+    CHK(from_T<float>(s, L.t32, y, n));
+  }
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return VX_OK;
+}
+
+int vx_smooth(vx_handle s, int32_t l, int32_t prec, const double* u, const double* b,
+              int32_t iters, double* out) {
+  CHK(check_level(s, l));
+  if (iters < 1) return fail(s, VX_ERR_ARG, "iters < 1");
+  Level& L = s->lv[l];
+  const long long n = 3 * L.Nn;
+  if (prec == VX_FP64) {
+    CHK(alloc_level_vectors<double>(s));
+    CHK(smooth_n<double>(s, l, u, b, iters, L.x64, L.t64, false, nullptr));
+    HIPCHK(hipMemcpyAsync(out, L.x64, sizeof(double) * n, hipMemcpyDeviceToDevice, s->stream));
+  } else {
+    CHK(alloc_pcg32(s));
+    float* bb = s->d32;
+    float* uu = s->r32;
+    CHK(to_T<float>(s, b, bb, n));
+    if (u) CHK(to_T<float>(s, u, uu, n));
+    CHK(smooth_n<float>(s, l, u ? uu : nullptr, bb, iters, L.x32, L.t32, false, nullptr));
+    CHK(from_T<float>(s, L.x32, out, n));
+  }
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return VX_OK;
+}
+
+int vx_restrict(vx_handle s, int32_t l, const double* rf, double* rc) {
+  CHK(check_level(s, l));
+  if (l + 1 >= (int)s->lv.size()) return fail(s, VX_ERR_ARG, "no coarser level");
+  Level& C = s->lv[l + 1];
+  hipLaunchKernelGGL(k_restrict<double>, dim3(ew_grid(C.Nn)), dim3(EW_THREADS), 0, s->stream,
+                     C.Lx, C.Ly, C.Lz, rf, C.mask, rc);
+  LAUNCHCHK();
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return VX_OK;
+}
+
+int vx_prolong(vx_handle s, int32_t l, const double* ec, double* ef) {
+  CHK(check_level(s, l));
+  if (l + 1 >= (int)s->lv.size()) return fail(s, VX_ERR_ARG, "no coarser level");
+  Level& L = s->lv[l];
+  HIPCHK(hipMemsetAsync(ef, 0, sizeof(double) * 3 * L.Nn, s->stream));
+  hipLaunchKernelGGL(k_prolong_add<double>, dim3(ew_grid(L.Nn)), dim3(EW_THREADS), 0, s->stream,
+                     L.Lx, L.Ly, L.Lz, ec, L.mask, ef, ef);
+  LAUNCHCHK();
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return VX_OK;
+}
+
+int vx_vcycle(vx_handle s, int32_t prec, const double* b, double* z) {
+  CHK(check_level(s, 0));
+  const long long n = 3 * s->lv[0].Nn;
+  if (prec == VX_FP64) {
+    CHK(alloc_pcg64(s));
+    HIPCHK(hipMemcpyAsync(s->r64, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s->stream));
+    CHK(vcycle<double>(s, s->r64, s->z64, s->lv[0].t64, false, nullptr));
+    HIPCHK(hipMemcpyAsync(z, s->z64, sizeof(double) * n, hipMemcpyDeviceToDevice, s->stream));
+  } else {
+    CHK(alloc_pcg32(s));
+    CHK(to_T<float>(s, b, s->r32, n));
+    CHK(vcycle<float>(s, s->r32, s->z32, s->lv[0].t32, false, nullptr));
+    CHK(from_T<float>(s, s->z32, z, n));
+  }
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return VX_OK;
+}
+
+int vx_inv_diag(vx_handle s, int32_t l, double* minv) {
+  CHK(check_level(s, l));
+  Level& L = s->lv[l];
+  hipLaunchKernelGGL(k_inv_diag<double>, dim3(ew_grid(L.Nn)), dim3(EW_THREADS), 0, s->stream,
+                     L.nx, L.ny, L.nz, L.E64, s->kd, minv);
+  LAUNCHCHK();
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return VX_OK;
+}
+
+int vx_von_mises(vx_handle s, const double* u, const double* E, const uint8_t* active,
+                 double* vm) {
+  if (!s || !u || !E || !vm) return fail(s, VX_ERR_ARG, "null von Mises argument");
+  Level& L = s->lv[0];
+  hipLaunchKernelGGL(k_von_mises, dim3(ew_grid(L.Ne)), dim3(EW_THREADS), 0, s->stream, L.nx, L.ny,
+                     L.nz, s->nu, u, E, active, vm);
+  LAUNCHCHK();
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return VX_OK;
+}
+
+int vx_drop_islands(vx_handle s, const uint8_t* active, const int64_t* fixed, int64_t n_fixed,
+                    uint8_t* kept, int64_t* n_removed) {
+  if (!s || !active || !kept || (n_fixed > 0 && !fixed))
+    return fail(s, VX_ERR_ARG, "bad island arguments");
+  Level& L = s->lv[0];
+  std::vector<uint8_t> nf(L.Nn, 0);
+  for (int64_t i = 0; i < n_fixed; ++i) {
+    if (fixed[i] < 0 || fixed[i] >= 3 * L.Nn) return fail(s, VX_ERR_ARG, "fixed DOF out of range");
+    nf[fixed[i] / 3] = 1;
+  }
+  uint8_t* dnf = nullptr;
+  int* dchg = nullptr;
+  unsigned long long* dcnt = nullptr;
+  HIPCHK(hipMalloc(&dnf, L.Nn));
+  HIPCHK(hipMalloc(&dchg, sizeof(int)));
+  HIPCHK(hipMalloc(&dcnt, 2 * sizeof(unsigned long long)));
+  auto done = [&]() {
+    (void)hipFree(dnf);
+    (void)hipFree(dchg);
+    (void)hipFree(dcnt);
+  };
+  HIPCHK(hipMemcpyAsync(dnf, nf.data(), L.Nn, hipMemcpyHostToDevice, s->stream));
+  hipLaunchKernelGGL(k_island_seed, dim3(ew_grid(L.Ne)), dim3(EW_THREADS), 0, s->stream, L.nx,
+                     L.ny, L.nz, active, dnf, kept);
+  int hchg = 1;
+  long long passes = 0;
+  while (hchg) {
+    HIPCHK(hipMemsetAsync(dchg, 0, sizeof(int), s->stream));
+    hipLaunchKernelGGL(k_island_grow, dim3(ew_grid((long long)L.nx * L.ny)), dim3(EW_THREADS), 0,
+                       s->stream, L.nx, L.ny, L.nz, active, kept, dchg);
+    HIPCHK(hipMemcpyAsync(&hchg, dchg, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (++passes > 4 * (L.Ne + 1)) {
+      done();
+      return fail(s, VX_ERR_HIP, "island flood fill did not terminate");
+    }
+  }
+  HIPCHK(hipMemsetAsync(dcnt, 0, 2 * sizeof(unsigned long long), s->stream));
+  hipLaunchKernelGGL(k_count_u8, dim3(ew_grid(L.Ne)), dim3(EW_THREADS), 0, s->stream, L.Ne, active,
+                     dcnt);
+  hipLaunchKernelGGL(k_count_u8, dim3(ew_grid(L.Ne)), dim3(EW_THREADS), 0, s->stream, L.Ne, kept,
+                     dcnt + 1);
+  unsigned long long hc[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync(hc, dcnt, sizeof hc, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  if (hc[1] == 0) {
+    // nothing reaches a support: leave the part untouched (core/fea.py:220-225)
+    HIPCHK(hipMemcpyAsync(kept, active, L.Ne, hipMemcpyDeviceToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (n_removed) *n_removed = 0;
+  } else if (n_removed) {
+    *n_removed = (int64_t)(hc[0] - hc[1]);
+  }
+  done();
+  return VX_OK;
+}
+
+int vx_time_kernel(vx_handle s, int32_t kind, int32_t l, int32_t prec, int32_t reps,
+                   double* avg_ms) {
+  CHK(check_level(s, l));
+  if (reps < 1 || !avg_ms) return fail(s, VX_ERR_ARG, "bad timing arguments");
+  Level& L = s->lv[l];
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  int rc = VX_OK;
+  if (prec == VX_FP64) {
+    CHK(alloc_pcg64(s));
+    double* x = l == 0 ? s->p64 : L.x64;
+    double* y = l == 0 ? s->Ap64 : L.t64;
+    double* bb = l == 0 ? s->r64 : L.b64;
+    HIPCHK(hipMemsetAsync(x, 0, sizeof(double) * 3 * L.Nn, s->stream));
+    HIPCHK(hipMemsetAsync(bb, 0, sizeof(double) * 3 * L.Nn, s->stream));
+    for (int w = 0; w < 2 && rc == VX_OK; ++w)
+      rc = kind == 0 ? launch_tile<double, M_MATVEC, false>(s, l, x, nullptr, y, nullptr, nullptr)
+                     : launch_tile<double, M_JACOBI, false>(s, l, x, bb, y, nullptr, nullptr);
+    HIPCHK(hipEventRecord(a, s->stream));
+    for (int i = 0; i < reps && rc == VX_OK; ++i)
+      rc = kind == 0 ? launch_tile<double, M_MATVEC, false>(s, l, x, nullptr, y, nullptr, nullptr)
+                     : launch_tile<double, M_JACOBI, false>(s, l, x, bb, y, nullptr, nullptr);
+    HIPCHK(hipEventRecord(b, s->stream));
+  } else {
+    CHK(alloc_pcg32(s));
+    float* x = l == 0 ? s->p32 : L.x32;
+    float* y = l == 0 ? s->Ap32 : L.t32;
+    float* bb = l == 0 ? s->r32 : L.b32;
+    HIPCHK(hipMemsetAsync(x, 0, sizeof(float) * 3 * L.Nn, s->stream));
+    HIPCHK(hipMemsetAsync(bb, 0, sizeof(float) * 3 * L.Nn, s->stream));
+    for (int w = 0; w < 2 && rc == VX_OK; ++w)
+      rc = kind == 0 ? launch_tile<float, M_MATVEC, false>(s, l, x, nullptr, y, nullptr, nullptr)
+                     : launch_tile<float, M_JACOBI, false>(s, l, x, bb, y, nullptr, nullptr);
+    HIPCHK(hipEventRecord(a, s->stream));
+    for (int i = 0; i < reps && rc == VX_OK; ++i)
+      rc = kind == 0 ? launch_tile<float, M_MATVEC, false>(s, l, x, nullptr, y, nullptr, nullptr)
+                     : launch_tile<float, M_JACOBI, false>(s, l, x, bb, y, nullptr, nullptr);
+    HIPCHK(hipEventRecord(b, s->stream));
+  }
+  HIPCHK(hipEventSynchronize(b));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, a, b));
+  *avg_ms = ms / reps;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return rc;
+}
+
+}  // extern "C"
