@@ -62,7 +62,9 @@ typedef struct {
   double inner_tol;           /* mixed: relative tolerance of each FP32 inner solve */
   int32_t max_outer;          /* mixed: max iterative-refinement steps */
   int32_t z_chunk;            /* node planes per work-group in the tile kernels (0: auto) */
-  int32_t reserved[8];
+  int32_t e_coarsen;          /* coarse-level E: 0 arithmetic mean of the 8 children
+                                 (core/multigrid.py:287-290), 1 geometric, 2 harmonic */
+  int32_t reserved[7];
 } vx_options;
 
 /* Per-solve statistics (mirrors MGSolver.last_* of core/multigrid.py:482-517). */
@@ -160,8 +162,9 @@ int vx_drop_islands(vx_handle h, const uint8_t* active_dev, const int64_t* fixed
                     int64_t n_fixed, uint8_t* kept_dev, int64_t* n_removed_host);
 
 /* Timing helper: average duration (ms) of `reps` back-to-back launches of the
- * level-l operator kernel (kind 0 = masked matvec, 1 = Jacobi sweep) in the
- * given precision, measured with HIP events on the handle's stream. */
+ * level-l operator kernel (kind 0 = masked matvec, 1 = damped-Jacobi sweep,
+ * 2 = Chebyshev smoothing step) in the given precision, measured with HIP
+ * events on the handle's stream. */
 int vx_time_kernel(vx_handle h, int32_t kind, int32_t level, int32_t precision,
                    int32_t reps, double* avg_ms);
 

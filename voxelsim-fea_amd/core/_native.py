@@ -49,7 +49,8 @@ class vx_options(ctypes.Structure):
         ("inner_tol", ctypes.c_double),
         ("max_outer", ctypes.c_int32),
         ("z_chunk", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 8),
+        ("e_coarsen", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 7),
     ]
 
 

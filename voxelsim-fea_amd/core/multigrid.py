@@ -22,6 +22,7 @@ from . import backend
 from . import compute_plan
 
 _PRECISION = {"fp64": _native.VX_FP64, "mixed": _native.VX_MIXED}
+_SMOOTHER = {"jacobi": _native.VX_SMOOTH_JACOBI, "chebyshev": _native.VX_SMOOTH_CHEBYSHEV}
 
 
 def _torch():
@@ -51,12 +52,16 @@ class MGSolver:
     def __init__(self, nx, ny, nz, fixed_dofs, nu=0.3, xp=None, n_smooth=2, omega=0.6,
                  min_elems=4, max_levels=6, compute_mode="AUTO", cpu_threads=0,
                  verbose=False, precision="fp64", coarse_sweeps=30, inner_tol=1e-5,
-                 max_outer=20, z_chunk=0):
+                 max_outer=20, z_chunk=0, smoother="jacobi", cheb_ratio=0.3,
+                 system="full", e_coarsen="mean"):
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
         self.nelem = self.nx * self.ny * self.nz
         self.ndof = 3 * (self.nx + 1) * (self.ny + 1) * (self.nz + 1)
         self.n_smooth, self.omega, self.verbose = n_smooth, omega, verbose
         self.precision = precision
+        if system not in ("full", "active"):
+            raise ValueError("system must be 'full' (voids at E*e_min) or 'active'")
+        self.system = system
         self.plan = compute_plan.choose(self.ndof, mode=compute_mode, cpu_threads=cpu_threads,
                                         verbose=verbose)
         _require_gpu(self.plan, "MGSolver")
@@ -65,7 +70,9 @@ class MGSolver:
         opt = _native.default_options(
             max_levels=max_levels, min_elems=min_elems, n_smooth=n_smooth, omega=omega,
             coarse_sweeps=coarse_sweeps, precision=_PRECISION[precision],
-            inner_tol=inner_tol, max_outer=max_outer, z_chunk=z_chunk)
+            inner_tol=inner_tol, max_outer=max_outer, z_chunk=z_chunk,
+            smoother=_SMOOTHER[smoother], cheb_ratio=cheb_ratio,
+            e_coarsen={"mean": 0, "geometric": 1, "harmonic": 2}[e_coarsen])
         h = ctypes.c_void_p()
         _native.check(self._lib.vx_create(self.nx, self.ny, self.nz, float(nu),
                                           ctypes.byref(opt), ctypes.byref(h)))
@@ -112,12 +119,16 @@ class MGSolver:
     def set_density(self, Evec_full, active=None, e_min=1e-9):
         """Per-level E, 2x2x2 coarsening (core/multigrid.py:281-315).  With
         ``active`` (bool per element) the void rule E*e_min of
-        core/fea.py:467-469 is applied on the device."""
+        core/fea.py:467-469 is applied on the device; with system="active"
+        voids carry no stiffness and DOFs of no active element leave the
+        system (the system of the reference's Jacobi-PCG path,
+        core/fea.py:497-593)."""
         self._E_dev = _dev(Evec_full)
         self._active_dev = None if active is None else _dev(active, _torch().uint8)
         self._stream_bind()
+        em = -1.0 if self.system == "active" else float(e_min)
         self._check(self._lib.vx_set_density(self._h, _native.ptr(self._E_dev),
-                                              _native.ptr(self._active_dev), float(e_min)))
+                                              _native.ptr(self._active_dev), em))
 
     # -- solve -------------------------------------------------------------
     def solve_device(self, f_dev, u_dev, tol=1e-4, max_cg=None, x0_dev=None,
@@ -144,6 +155,32 @@ class MGSolver:
         self.last_resid_ratio = info.resid_ratio
         self.last_true_resid_ratio = info.true_resid_ratio
         self.last_outer_iters = info.outer_iters
+        self.last_solve_ms = info.solve_ms
+        self.resid_history = [float(v) for v in hist[:min(info.iters, hist_cap)]]
+        self._check(rc, allow=(_native.VX_ERR_NOT_CONVERGED,))
+        return rc
+
+    def solve_jacobi_device(self, E_dev, active_dev, f_dev, u_dev, tol=1e-4, max_cg=None,
+                            progress_cb=None, hist_cap=100000):
+        """Jacobi-PCG on the active-DOF system on the device
+        (core/fea.py:497-593); same fields as solve_device."""
+        self._stream_bind()
+        hist = np.zeros(hist_cap)
+        info = _native.vx_info()
+
+        def _cb(it, ratio, t, _user):
+            progress_cb(int(it), float(ratio), float(t))
+
+        cb = _native.PROGRESS_CB(_cb) if progress_cb is not None else _native.PROGRESS_CB()
+        rc = self._lib.vx_solve_jacobi(self._h, _native.ptr(E_dev), _native.ptr(active_dev),
+                                       _native.ptr(f_dev), _native.ptr(u_dev), float(tol),
+                                       int(max_cg or 0), cb, None,
+                                       hist.ctypes.data_as(ctypes.c_void_p), hist_cap,
+                                       ctypes.byref(info))
+        self.last_iters = info.iters
+        self.last_converged = bool(info.converged)
+        self.last_resid_ratio = info.resid_ratio
+        self.last_true_resid_ratio = info.true_resid_ratio
         self.last_solve_ms = info.solve_ms
         self.resid_history = [float(v) for v in hist[:min(info.iters, hist_cap)]]
         self._check(rc, allow=(_native.VX_ERR_NOT_CONVERGED,))
@@ -225,6 +262,6 @@ class MGSolver:
         """Average device time (ms) of one level operator launch."""
         out = ctypes.c_double()
         self._stream_bind()
-        self._check(self._lib.vx_time_kernel(self._h, {"matvec": 0, "jacobi": 1}[kind], level,
+        self._check(self._lib.vx_time_kernel(self._h, {"matvec": 0, "jacobi": 1, "chebyshev": 2}[kind], level,
                                              _PRECISION[precision], reps, ctypes.byref(out)))
         return out.value

@@ -165,6 +165,35 @@ inline double elem_diag(double nu) {
   return y[0][0];
 }
 
+// Largest eigenvalue of the unit-E KE by power iteration on the element
+// operator.  Divided by the diagonal kd it bounds lambda_max(D^-1 A) for any
+// assembly of scaled copies of KE (Rayleigh-quotient mediant over elements),
+// which is the upper end of the Chebyshev smoothing interval.
+inline double elem_lambda_max(double nu) {
+  ElemConst<double> K = elem_const(nu);
+  double u[8][3], y[8][3];
+  for (int q = 0; q < 8; ++q)
+    for (int c = 0; c < 3; ++c) u[q][c] = 1.0 + 0.37 * q - 0.21 * c + 0.05 * q * c * (q & 1 ? -1 : 1);
+  double lam = 0.0;
+  for (int it = 0; it < 500; ++it) {
+    elem_apply<double>(u, 1.0, K, y);
+    double nrm = 0.0, ray = 0.0;
+    for (int q = 0; q < 8; ++q)
+      for (int c = 0; c < 3; ++c) {
+        nrm += y[q][c] * y[q][c];
+        ray += u[q][c] * y[q][c];
+      }
+    double un = 0.0;
+    for (int q = 0; q < 8; ++q)
+      for (int c = 0; c < 3; ++c) un += u[q][c] * u[q][c];
+    lam = ray / un;
+    nrm = std::sqrt(nrm);
+    for (int q = 0; q < 8; ++q)
+      for (int c = 0; c < 3; ++c) u[q][c] = y[q][c] / nrm;
+  }
+  return lam;
+}
+
 // Reference local node k (core/fea.py:33-36) -> binary corner index.
 VX_HD inline int ref_to_bin(int k) {
   const int tab[8] = {0, 1, 3, 2, 4, 5, 7, 6};

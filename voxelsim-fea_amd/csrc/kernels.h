@@ -25,7 +25,7 @@ constexpr int NT = TILE * TILE;       // threads per tile work-group
 constexpr int FOOTVALS = FOOT * FOOT * 3;
 constexpr int PRE = (FOOTVALS + NT - 1) / NT;
 
-enum { M_MATVEC = 0, M_RESID = 1, M_JACOBI = 2 };
+enum { M_MATVEC = 0, M_RESID = 1, M_JACOBI = 2, M_CHEB = 3 };
 
 template <typename T>
 struct TileArgs {
@@ -37,7 +37,9 @@ struct TileArgs {
   const uint8_t* __restrict__ mask;
   const T* __restrict__ b;
   T* __restrict__ out;
-  T omega;
+  T omega;               // M_JACOBI weight
+  const T* __restrict__ up;   // M_CHEB: previous iterate (nullptr = zero)
+  T cu, cr;              // M_CHEB: out = u + cu (u - up) + cr D^-1 (b - K u)
   T kd;                  // unit-E diagonal of KE
   ElemConst<T> K;
   double* __restrict__ partial;   // one double per work-group (DOT variants)
@@ -81,6 +83,8 @@ __device__ inline double block_sum(double v, double* sred) {
 //       M_RESID : out = b - (mask ? K u : 0)          (:431)
 //       M_JACOBI: out = u + omega*(mask ? (b-Ku)/d : 0), d = kd*sum E
 //                                                     (:367-402, :300-304)
+//       M_CHEB  : out = u + mask ? cu (u - up) + cr (b-Ku)/d : 0, one step of
+//                 the Chebyshev iteration on D^-1 A (three-term recurrence)
 // DOT   adds the work-group partial of  sum out*u (MATVEC) or sum out*b
 //       (JACOBI) to partial[block].
 // ---------------------------------------------------------------------------
@@ -210,13 +214,24 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs<T> a) {
           a.out[3 * n + c] = o;
           if (DOT) dacc += (double)o * (double)o;
         }
-      } else {
+      } else if (MODE == M_JACOBI) {
         const T d = a.kd * (EX + sQ[3][ty - 1][tx]);
         const T minv = T(1) / (d == T(0) ? T(1) : d);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           const T bb = a.b[3 * n + c];
           const T o = ub0[c] + a.omega * (((mk >> c) & 1u) ? minv * (bb - y[c]) : T(0));
+          a.out[3 * n + c] = o;
+          if (DOT) dacc += (double)o * (double)bb;
+        }
+      } else {
+        const T d = a.kd * (EX + sQ[3][ty - 1][tx]);
+        const T minv = T(1) / (d == T(0) ? T(1) : d);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const T bb = a.b[3 * n + c];
+          const T du = a.up ? ub0[c] - a.up[3 * n + c] : ub0[c];
+          const T o = ub0[c] + (((mk >> c) & 1u) ? a.cu * du + a.cr * (minv * (bb - y[c])) : T(0));
           a.out[3 * n + c] = o;
           if (DOT) dacc += (double)o * (double)bb;
         }
@@ -364,8 +379,12 @@ __global__ void k_prolong_add(int Lfx, int Lfy, int Lfz, const T* __restrict__ e
 
 // E of level l+1 = 2 * mean of the 8 children (scaled E carries 2^l;
 // core/multigrid.py:287-292).  x-fastest layout on both levels.
+// mode 0: arithmetic mean (the reference rule); 1: geometric mean; 2:
+// harmonic mean (0 when any child is 0).  The factor 2 of the level scale is
+// applied to all three.
 template <typename T>
-__global__ void k_coarsen_E(int cnx, int cny, int cnz, const T* __restrict__ Ef, T* __restrict__ Ec) {
+__global__ void k_coarsen_E(int cnx, int cny, int cnz, const T* __restrict__ Ef, T* __restrict__ Ec,
+                            int mode) {
   const int fnx = 2 * cnx, fny = 2 * cny;
   const long long Nc = (long long)cnx * cny * cnz;
   for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < Nc;
@@ -373,13 +392,29 @@ __global__ void k_coarsen_E(int cnx, int cny, int cnz, const T* __restrict__ Ef,
     const int x = (int)(e % cnx);
     const int y = (int)((e / cnx) % cny);
     const int z = (int)(e / ((long long)cnx * cny));
-    T s = T(0);
-    for (int dz = 0; dz < 2; ++dz)
-      for (int dy = 0; dy < 2; ++dy) {
-        const long long r = ((long long)(2 * z + dz) * fny + (2 * y + dy)) * fnx + 2 * x;
-        s += Ef[r] + Ef[r + 1];
-      }
-    Ec[e] = s * T(0.25);
+    if (mode == 0) {
+      T s = T(0);
+      for (int dz = 0; dz < 2; ++dz)
+        for (int dy = 0; dy < 2; ++dy) {
+          const long long r = ((long long)(2 * z + dz) * fny + (2 * y + dy)) * fnx + 2 * x;
+          s += Ef[r] + Ef[r + 1];
+        }
+      Ec[e] = s * T(0.25);
+    } else {
+      double acc = 0.0;
+      bool zero = false;
+      for (int dz = 0; dz < 2; ++dz)
+        for (int dy = 0; dy < 2; ++dy)
+          for (int dx = 0; dx < 2; ++dx) {
+            const double v = (double)Ef[((long long)(2 * z + dz) * fny + (2 * y + dy)) * fnx + 2 * x + dx];
+            if (v <= 0.0) zero = true;
+            else acc += (mode == 1) ? log(v) : 1.0 / v;
+          }
+      double m = 0.0;
+      if (mode == 1) m = zero ? 0.0 : exp(acc / 8.0);
+      else m = zero ? 0.0 : 8.0 / acc;
+      Ec[e] = (T)(2.0 * m);
+    }
   }
 }
 
@@ -407,6 +442,35 @@ __global__ void k_convert(long long n, const Tin* __restrict__ a, Tout* __restri
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
     b[i] = (Tout)a[i];
+}
+
+// Effective free mask of a level: the fixed-DOF mask, and in the active-DOF
+// system also 0 where no adjacent element carries stiffness.
+__global__ void k_effective_mask(int nx, int ny, int nz, const double* __restrict__ E,
+                                 const uint8_t* __restrict__ mfixed, int drop_unloaded,
+                                 uint8_t* __restrict__ mask) {
+  const int Lx = nx + 1, Ly = ny + 1, Lz = nz + 1;
+  const long long Nn = (long long)Lx * Ly * Lz;
+  for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < Nn;
+       n += (long long)gridDim.x * blockDim.x) {
+    uint8_t m = mfixed[n];
+    if (drop_unloaded) {
+      const int ix = (int)(n % Lx);
+      const int iy = (int)((n / Lx) % Ly);
+      const int iz = (int)(n / ((long long)Lx * Ly));
+      bool any = false;
+      for (int dz = -1; dz <= 0; ++dz)
+        for (int dy = -1; dy <= 0; ++dy)
+          for (int dx = -1; dx <= 0; ++dx) {
+            const int x = ix + dx, y = iy + dy, z = iz + dz;
+            if (x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz &&
+                E[((long long)z * ny + y) * nx + x] > 0.0)
+              any = true;
+          }
+      if (!any) m = 0;
+    }
+    mask[n] = m;
+  }
 }
 
 // Coarse free mask by injection from the collocated fine node

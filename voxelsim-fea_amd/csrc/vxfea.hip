@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "kernels.h"
@@ -33,8 +34,9 @@ struct Level {
   double* E64 = nullptr;
   float* E32 = nullptr;
   // V-cycle work vectors (levels >= 1 own b; x/t at every level)
-  double *b64 = nullptr, *x64 = nullptr, *t64 = nullptr;
-  float *b32 = nullptr, *x32 = nullptr, *t32 = nullptr;
+  double *b64 = nullptr, *x64 = nullptr, *t64 = nullptr, *w64 = nullptr;
+  float *b32 = nullptr, *x32 = nullptr, *t32 = nullptr, *w32 = nullptr;
+  uint8_t* mask_fixed = nullptr;   // injected fixed-DOF mask (set_fixed)
 };
 
 template <typename T> struct Sel;
@@ -42,12 +44,14 @@ template <> struct Sel<double> {
   static double*& b(Level& l) { return l.b64; }
   static double*& x(Level& l) { return l.x64; }
   static double*& t(Level& l) { return l.t64; }
+  static double*& w(Level& l) { return l.w64; }
   static double* E(Level& l) { return l.E64; }
 };
 template <> struct Sel<float> {
   static float*& b(Level& l) { return l.b32; }
   static float*& x(Level& l) { return l.x32; }
   static float*& t(Level& l) { return l.t32; }
+  static float*& w(Level& l) { return l.w32; }
   static float* E(Level& l) { return l.E32; }
 };
 
@@ -63,7 +67,9 @@ struct vx_solver {
   ElemConst<double> K64;
   ElemConst<float> K32;
   double kd = 0;
+  double lam_max = 0;        // max eigenvalue of D_e^-1 KE: bounds lambda_max(D^-1 A)
   bool fixed_set = false, density_set = false;
+  bool active_system = false;
   // level-0 PCG vectors
   double *u64 = nullptr, *r64 = nullptr, *z64 = nullptr, *p64 = nullptr, *Ap64 = nullptr,
          *fm64 = nullptr;
@@ -141,15 +147,24 @@ template <typename T> ElemConst<T> kconst(const vx_solver* s);
 template <> ElemConst<double> kconst<double>(const vx_solver* s) { return s->K64; }
 template <> ElemConst<float> kconst<float>(const vx_solver* s) { return s->K32; }
 
+struct SmoothCoef {
+  double omega = 0;
+  const void* up = nullptr;
+  double cu = 0, cr = 0;
+};
+
 template <typename T, int MODE, bool DOT>
 int launch_tile_L(vx_solver* s, Level& L, const T* u, const T* b, T* out, double* partial,
-                  int* n_part) {
+                  int* n_part, const SmoothCoef* sc = nullptr) {
   TileArgs<T> a;
   a.nx = L.nx; a.ny = L.ny; a.nz = L.nz;
   a.Lx = L.Lx; a.Ly = L.Ly; a.Lz = L.Lz;
   a.zc = zchunk(s, L);
   a.u = u; a.E = Sel<T>::E(L); a.mask = L.mask; a.b = b; a.out = out;
-  a.omega = (T)s->opt.omega;
+  a.omega = (T)(sc ? sc->omega : s->opt.omega);
+  a.up = sc ? static_cast<const T*>(sc->up) : nullptr;
+  a.cu = (T)(sc ? sc->cu : 0.0);
+  a.cr = (T)(sc ? sc->cr : 0.0);
   a.kd = (T)s->kd;
   a.K = kconst<T>(s);
   a.partial = partial;
@@ -164,8 +179,8 @@ int launch_tile_L(vx_solver* s, Level& L, const T* u, const T* b, T* out, double
 
 template <typename T, int MODE, bool DOT>
 int launch_tile(vx_solver* s, int li, const T* u, const T* b, T* out, double* partial,
-                int* n_part) {
-  return launch_tile_L<T, MODE, DOT>(s, s->lv[li], u, b, out, partial, n_part);
+                int* n_part, const SmoothCoef* sc = nullptr) {
+  return launch_tile_L<T, MODE, DOT>(s, s->lv[li], u, b, out, partial, n_part, sc);
 }
 
 int finalize(vx_solver* s, int n, int op) {
@@ -174,13 +189,14 @@ int finalize(vx_solver* s, int n, int op) {
   return VX_OK;
 }
 
-// one damped-Jacobi sweep; u == nullptr means "from zero"
+// one damped-Jacobi sweep with weight omega; u == nullptr means "from zero"
 template <typename T>
-int sweep(vx_solver* s, int li, const T* u, const T* b, T* out, bool dot, int* n_part) {
+int sweep(vx_solver* s, int li, const T* u, const T* b, T* out, bool dot, int* n_part,
+          double omega) {
   Level& L = s->lv[li];
   if (!u) {
     hipLaunchKernelGGL(k_jacobi_zero<T>, dim3(ew_grid(L.Nn)), dim3(EW_THREADS), 0, s->stream,
-                       L.nx, L.ny, L.nz, Sel<T>::E(L), L.mask, b, out, (T)s->opt.omega,
+                       L.nx, L.ny, L.nz, Sel<T>::E(L), L.mask, b, out, (T)omega,
                        (T)s->kd);
     LAUNCHCHK();
     if (dot) {
@@ -191,8 +207,53 @@ int sweep(vx_solver* s, int li, const T* u, const T* b, T* out, bool dot, int* n
     }
     return VX_OK;
   }
-  if (dot) return launch_tile<T, M_JACOBI, true>(s, li, u, b, out, s->partial, n_part);
-  return launch_tile<T, M_JACOBI, false>(s, li, u, b, out, nullptr, nullptr);
+  SmoothCoef sc;
+  sc.omega = omega;
+  if (dot) return launch_tile<T, M_JACOBI, true>(s, li, u, b, out, s->partial, n_part, &sc);
+  return launch_tile<T, M_JACOBI, false>(s, li, u, b, out, nullptr, nullptr, &sc);
+}
+
+// Chebyshev smoothing (degree k) on D^-1 A over [ratio*lmax, lmax]:
+// x1 = x0 + D^-1 r0 / theta, x_{j+1} = x_j + rho_j rho_{j-1} (x_j - x_{j-1})
+// + (2 rho_j / delta) D^-1 r_j.  u_in == nullptr starts from zero.  Buffers
+// dst/tmp/tmp2 rotate; the k-th iterate lands in dst.
+template <typename T>
+int cheb_n(vx_solver* s, int li, const T* u_in, const T* b, int k, T* dst, T* tmp, T* tmp2,
+           bool dot, int* n_part) {
+  const double beta = s->lam_max, alpha = s->opt.cheb_ratio * beta;
+  const double theta = 0.5 * (beta + alpha), delta = 0.5 * (beta - alpha);
+  const double sigma = theta / delta;
+  // assign targets so that write j (0-based) goes to bufs[(k-1-j) % 3]'s rotation ending at dst
+  T* ring[3] = {dst, tmp, tmp2};
+  auto target = [&](int j) { return ring[((k - 1 - j) % 3 + 3) % 3]; };
+  const T* xprev = u_in;   // x_{j-1}
+  const T* xcur = u_in;    // x_j
+  double rho_prev = 1.0 / sigma, rho = rho_prev;
+  for (int j = 0; j < k; ++j) {
+    T* o = target(j);
+    const bool last = (j == k - 1);
+    if (j == 0) {
+      if (o == xcur) return fail(s, VX_ERR_ARG, "chebyshev aliasing");
+      CHK(sweep<T>(s, li, xcur, b, o, dot && last, n_part, 1.0 / theta));
+    } else {
+      rho = 1.0 / (2.0 * sigma - rho_prev);
+      SmoothCoef sc;
+      sc.up = (j == 1 && !u_in) ? nullptr : (const void*)xprev;
+      sc.cu = rho * rho_prev;
+      sc.cr = 2.0 * rho / delta;
+      // out may overwrite up (read only at the owner's own node, before its
+      // write) but never u, whose halo other work-groups read
+      if (o == xcur) return fail(s, VX_ERR_ARG, "chebyshev aliasing");
+      if (dot && last)
+        CHK((launch_tile<T, M_CHEB, true>(s, li, xcur, b, o, s->partial, n_part, &sc)));
+      else
+        CHK((launch_tile<T, M_CHEB, false>(s, li, xcur, b, o, nullptr, nullptr, &sc)));
+      rho_prev = rho;
+    }
+    xprev = xcur;
+    xcur = o;
+  }
+  return VX_OK;
 }
 
 // k sweeps starting at u_in (nullptr = zero) writing the result to `dst`,
@@ -208,7 +269,7 @@ int smooth_n(vx_solver* s, int li, const T* u_in, const T* b, int k, T* dst, T* 
   for (int i = 0; i < k; ++i) {
     T* o = bufs[cur];
     if (src == o) return fail(s, VX_ERR_ARG, "smoother aliasing");
-    CHK(sweep<T>(s, li, src, b, o, dot && i == k - 1, n_part));
+    CHK(sweep<T>(s, li, src, b, o, dot && i == k - 1, n_part, s->opt.omega));
     src = o;
     cur ^= 1;
   }
@@ -223,15 +284,41 @@ int alloc_level_vectors(vx_solver* s) {
     if (li > 0) CHK(dalloc(s, &Sel<T>::b(L), n));
     CHK(dalloc(s, &Sel<T>::x(L), n));
     CHK(dalloc(s, &Sel<T>::t(L), n));
+    CHK(dalloc(s, &Sel<T>::w(L), n));
   }
   return VX_OK;
 }
 
+// smoothing step of the configured kind; result in dst (tmp, tmp2 scratch)
+template <typename T>
+int smooth_step(vx_solver* s, int li, const T* u_in, const T* b, int k, T* dst, T* tmp, T* tmp2,
+                bool dot, int* n_part) {
+  if (s->opt.smoother == VX_SMOOTH_CHEBYSHEV) {
+    if (u_in == dst && k % 3 == 1) {
+      // the first write would land on the start vector: go through tmp
+      CHK(cheb_n<T>(s, li, u_in, b, k, tmp, dst, tmp2, dot, n_part));
+      HIPCHK(hipMemcpyAsync(dst, tmp, sizeof(T) * 3 * s->lv[li].Nn, hipMemcpyDeviceToDevice,
+                            s->stream));
+      return VX_OK;
+    }
+    return cheb_n<T>(s, li, u_in, b, k, dst, tmp, tmp2, dot, n_part);
+  }
+  if (u_in == dst) {
+    // Jacobi from dst: ping-pong through tmp, copy back for odd counts
+    if (k % 2 == 0) return smooth_n<T>(s, li, u_in, b, k, dst, tmp, dot, n_part);
+    CHK(smooth_n<T>(s, li, u_in, b, k, tmp, dst, dot, n_part));
+    HIPCHK(hipMemcpyAsync(dst, tmp, sizeof(T) * 3 * s->lv[li].Nn, hipMemcpyDeviceToDevice,
+                          s->stream));
+    return VX_OK;
+  }
+  return smooth_n<T>(s, li, u_in, b, k, dst, tmp, dot, n_part);
+}
+
 // V-cycle (core/multigrid.py:426-436): b at level 0 -> result in z0.
-// t0 and res0 are level-0 scratch vectors (distinct from b0, z0).
+// t0 and w0 are level-0 scratch vectors (distinct from b0, z0).
 // dot: partials of z0.b0 from the last level-0 write.
 template <typename T>
-int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, bool dot, int* n_part) {
+int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, T* w0, bool dot, int* n_part) {
   const int nl = (int)s->lv.size();
   const int ns = s->opt.n_smooth;
   // descend
@@ -240,11 +327,13 @@ int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, bool dot, int* n_part) {
     const T* b = li == 0 ? b0 : Sel<T>::b(L);
     T* x = li == 0 ? z0 : Sel<T>::x(L);
     T* t = li == 0 ? t0 : Sel<T>::t(L);
+    T* w = li == 0 ? w0 : Sel<T>::w(L);
     if (li == nl - 1) {
-      CHK(smooth_n<T>(s, li, nullptr, b, s->opt.coarse_sweeps, x, t, dot && li == 0, n_part));
+      const int k = s->opt.coarse_sweeps;
+      CHK(smooth_step<T>(s, li, nullptr, b, k, x, t, w, dot && li == 0, n_part));
       break;
     }
-    CHK(smooth_n<T>(s, li, nullptr, b, ns, x, t, false, nullptr));
+    CHK(smooth_step<T>(s, li, nullptr, b, ns, x, t, w, false, nullptr));
     // residual into t, then restrict into b of level li+1
     CHK((launch_tile<T, M_RESID, false>(s, li, x, b, t, nullptr, nullptr)));
     Level& C = s->lv[li + 1];
@@ -259,19 +348,11 @@ int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, bool dot, int* n_part) {
     const T* b = li == 0 ? b0 : Sel<T>::b(L);
     T* x = li == 0 ? z0 : Sel<T>::x(L);
     T* t = li == 0 ? t0 : Sel<T>::t(L);
+    T* w = li == 0 ? w0 : Sel<T>::w(L);
     hipLaunchKernelGGL(k_prolong_add<T>, dim3(ew_grid(L.Nn)), dim3(EW_THREADS), 0, s->stream, L.Lx,
                        L.Ly, L.Lz, Sel<T>::x(C), L.mask, x, x);
     LAUNCHCHK();
-    // post-smoothing from x, result back in x
-    if (ns % 2 == 0) {
-      CHK(smooth_n<T>(s, li, x, b, ns, x, t, dot && li == 0, n_part));
-    } else {
-      CHK(smooth_n<T>(s, li, x, b, ns, t, x, dot && li == 0, n_part));
-      HIPCHK(hipMemcpyAsync(x, t, sizeof(T) * 3 * L.Nn, hipMemcpyDeviceToDevice, s->stream));
-      if (dot && li == 0) {
-        // partials were computed on t == x content: fine
-      }
-    }
+    CHK(smooth_step<T>(s, li, x, b, ns, x, t, w, dot && li == 0, n_part));
   }
   return VX_OK;
 }
@@ -312,7 +393,7 @@ int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* Ap, T* t0, double tol_rel, int 
   HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
   auto cleanup = [&]() { (void)hipEventDestroy(ev); };
 
-  int rc = vcycle<T>(s, r, z, t0, true, &np);
+  int rc = vcycle<T>(s, r, z, t0, Sel<T>::w(s->lv[0]), true, &np);
   if (rc == VX_OK) rc = finalize(s, np, F_RZ0);
   if (rc != VX_OK) { cleanup(); return rc; }
   if (hipMemcpyAsync(p, z, sizeof(T) * n, hipMemcpyDeviceToDevice, s->stream) != hipSuccess) {
@@ -344,7 +425,7 @@ int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* Ap, T* t0, double tol_rel, int 
     }
     auto queue_next = [&]() -> int {
       int np2 = 0;
-      CHK(vcycle<T>(s, r, z, t0, true, &np2));
+      CHK(vcycle<T>(s, r, z, t0, Sel<T>::w(s->lv[0]), true, &np2));
       CHK(finalize(s, np2, F_BETA));
       hipLaunchKernelGGL(k_p_update<T>, dim3(EW_BLOCKS), dim3(EW_THREADS), 0, s->stream, n,
                          s->dscal, z, p);
@@ -466,6 +547,7 @@ int vx_create(int32_t nx, int32_t ny, int32_t nz, double nu, const vx_options* o
   s->K32 = {(float)s->K64.k1, (float)s->K64.k2, (float)s->K64.b1, (float)s->K64.b2,
             (float)s->K64.t1};
   s->kd = elem_diag(nu);
+  s->lam_max = elem_lambda_max(nu) / s->kd;
   // hierarchy rule of core/multigrid.py:141-148
   std::vector<std::array<int, 3>> dims;
   dims.push_back({nx, ny, nz});
@@ -494,6 +576,7 @@ int vx_create(int32_t nx, int32_t ny, int32_t nz, double nu, const vx_options* o
   int rc = VX_OK;
   for (auto& L : s->lv) {
     if ((rc = dalloc(s, &L.mask, L.Nn)) != VX_OK) break;
+    if ((rc = dalloc(s, &L.mask_fixed, L.Nn)) != VX_OK) break;
     if ((rc = dalloc(s, &L.E64, L.Ne)) != VX_OK) break;
     if ((rc = dalloc(s, &L.E32, L.Ne)) != VX_OK) break;
   }
@@ -541,13 +624,16 @@ int vx_set_fixed(vx_handle s, const int64_t* fixed, int64_t n_fixed) {
     if (d < 0 || d >= 3 * L0.Nn) return fail(s, VX_ERR_ARG, "fixed DOF id out of range");
     m[d / 3] &= (uint8_t)~(1u << (d % 3));
   }
-  HIPCHK(hipMemcpyAsync(L0.mask, m.data(), L0.Nn, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipMemcpyAsync(L0.mask_fixed, m.data(), L0.Nn, hipMemcpyHostToDevice, s->stream));
   for (size_t li = 1; li < s->lv.size(); ++li) {
     Level& C = s->lv[li];
     hipLaunchKernelGGL(k_inject_mask, dim3(ew_grid(C.Nn)), dim3(EW_THREADS), 0, s->stream, C.Lx,
-                       C.Ly, C.Lz, s->lv[li - 1].mask, C.mask);
+                       C.Ly, C.Lz, s->lv[li - 1].mask_fixed, C.mask_fixed);
     LAUNCHCHK();
   }
+  for (auto& L : s->lv)
+    HIPCHK(hipMemcpyAsync(L.mask, L.mask_fixed, L.Nn, hipMemcpyDeviceToDevice, s->stream));
+  s->density_set = false;   // the effective masks are rebuilt by set_density
   HIPCHK(hipStreamSynchronize(s->stream));
   s->fixed_set = true;
   return VX_OK;
@@ -555,18 +641,28 @@ int vx_set_fixed(vx_handle s, const int64_t* fixed, int64_t n_fixed) {
 
 int vx_set_density(vx_handle s, const double* E, const uint8_t* active, double e_min) {
   if (!s || !E) return fail(s, VX_ERR_ARG, "null density");
+  if (!s->fixed_set) return fail(s, VX_ERR_ARG, "call vx_set_fixed() before vx_set_density()");
   Level& L0 = s->lv[0];
+  // e_min < 0 selects the active-DOF system: voids carry no stiffness and
+  // DOFs that no element with E > 0 touches leave the system.
+  const bool active_sys = e_min < 0.0;
+  s->active_system = active_sys;
   hipLaunchKernelGGL(k_import_E, dim3(ew_grid(L0.Ne)), dim3(EW_THREADS), 0, s->stream, L0.nx,
-                     L0.ny, L0.nz, E, active, e_min, L0.E64, L0.E32);
+                     L0.ny, L0.nz, E, active, active_sys ? 0.0 : e_min, L0.E64, L0.E32);
   LAUNCHCHK();
   for (size_t li = 1; li < s->lv.size(); ++li) {
     Level& F = s->lv[li - 1];
     Level& C = s->lv[li];
     hipLaunchKernelGGL(k_coarsen_E<double>, dim3(ew_grid(C.Ne)), dim3(EW_THREADS), 0, s->stream,
-                       C.nx, C.ny, C.nz, F.E64, C.E64);
+                       C.nx, C.ny, C.nz, F.E64, C.E64, s->opt.e_coarsen);
     LAUNCHCHK();
     hipLaunchKernelGGL((k_convert<double, float>), dim3(ew_grid(C.Ne)), dim3(EW_THREADS), 0,
                        s->stream, C.Ne, C.E64, C.E32);
+    LAUNCHCHK();
+  }
+  for (auto& L : s->lv) {
+    hipLaunchKernelGGL(k_effective_mask, dim3(ew_grid(L.Nn)), dim3(EW_THREADS), 0, s->stream,
+                       L.nx, L.ny, L.nz, L.E64, L.mask_fixed, active_sys ? 1 : 0, L.mask);
     LAUNCHCHK();
   }
   s->density_set = true;
@@ -777,7 +873,7 @@ int vx_solve_jacobi(vx_handle s, const double* E, const uint8_t* active, const d
                      J.nz, E, active, 0.0, J.E64, J.E32);
   LAUNCHCHK();
   hipLaunchKernelGGL(k_jacobi_setup, dim3(ew_grid(J.Nn)), dim3(EW_THREADS), 0, s->stream, J.nx,
-                     J.ny, J.nz, active, L0.mask, J.E64, s->kd, J.mask, s->jac_minv);
+                     J.ny, J.nz, active, L0.mask_fixed, J.E64, s->kd, J.mask, s->jac_minv);
   LAUNCHCHK();
   // f masked to the reduced free set; fnorm; count of free DOFs
   hipLaunchKernelGGL((k_mask_copy<double, double>), dim3(EW_BLOCKS), dim3(EW_THREADS), 0,
@@ -971,12 +1067,12 @@ int vx_vcycle(vx_handle s, int32_t prec, const double* b, double* z) {
   if (prec == VX_FP64) {
     CHK(alloc_pcg64(s));
     HIPCHK(hipMemcpyAsync(s->r64, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s->stream));
-    CHK(vcycle<double>(s, s->r64, s->z64, s->lv[0].t64, false, nullptr));
+    CHK(vcycle<double>(s, s->r64, s->z64, s->lv[0].t64, s->lv[0].w64, false, nullptr));
     HIPCHK(hipMemcpyAsync(z, s->z64, sizeof(double) * n, hipMemcpyDeviceToDevice, s->stream));
   } else {
     CHK(alloc_pcg32(s));
     CHK(to_T<float>(s, b, s->r32, n));
-    CHK(vcycle<float>(s, s->r32, s->z32, s->lv[0].t32, false, nullptr));
+    CHK(vcycle<float>(s, s->r32, s->z32, s->lv[0].t32, s->lv[0].w32, false, nullptr));
     CHK(from_T<float>(s, s->z32, z, n));
   }
   HIPCHK(hipStreamSynchronize(s->stream));
@@ -1064,41 +1160,49 @@ int vx_drop_islands(vx_handle s, const uint8_t* active, const int64_t* fixed, in
 int vx_time_kernel(vx_handle s, int32_t kind, int32_t l, int32_t prec, int32_t reps,
                    double* avg_ms) {
   CHK(check_level(s, l));
-  if (reps < 1 || !avg_ms) return fail(s, VX_ERR_ARG, "bad timing arguments");
+  if (reps < 1 || !avg_ms || kind < 0 || kind > 2)
+    return fail(s, VX_ERR_ARG, "bad timing arguments");
   Level& L = s->lv[l];
   hipEvent_t a, b;
   HIPCHK(hipEventCreate(&a));
   HIPCHK(hipEventCreate(&b));
   int rc = VX_OK;
+  auto run = [&](auto* x, auto* y, auto* bb, auto* up) -> int {
+    using T = std::remove_pointer_t<decltype(x)>;
+    SmoothCoef sc;
+    sc.omega = s->opt.omega;
+    sc.up = up;
+    sc.cu = 0.5;
+    sc.cr = 0.1;
+    if (kind == 0) return launch_tile<T, M_MATVEC, false>(s, l, x, nullptr, y, nullptr, nullptr);
+    if (kind == 1) return launch_tile<T, M_JACOBI, false>(s, l, x, bb, y, nullptr, nullptr, &sc);
+    return launch_tile<T, M_CHEB, false>(s, l, x, bb, y, nullptr, nullptr, &sc);
+  };
   if (prec == VX_FP64) {
     CHK(alloc_pcg64(s));
     double* x = l == 0 ? s->p64 : L.x64;
     double* y = l == 0 ? s->Ap64 : L.t64;
     double* bb = l == 0 ? s->r64 : L.b64;
+    double* up = L.w64;
     HIPCHK(hipMemsetAsync(x, 0, sizeof(double) * 3 * L.Nn, s->stream));
     HIPCHK(hipMemsetAsync(bb, 0, sizeof(double) * 3 * L.Nn, s->stream));
-    for (int w = 0; w < 2 && rc == VX_OK; ++w)
-      rc = kind == 0 ? launch_tile<double, M_MATVEC, false>(s, l, x, nullptr, y, nullptr, nullptr)
-                     : launch_tile<double, M_JACOBI, false>(s, l, x, bb, y, nullptr, nullptr);
+    HIPCHK(hipMemsetAsync(up, 0, sizeof(double) * 3 * L.Nn, s->stream));
+    for (int w = 0; w < 2 && rc == VX_OK; ++w) rc = run(x, y, bb, up);
     HIPCHK(hipEventRecord(a, s->stream));
-    for (int i = 0; i < reps && rc == VX_OK; ++i)
-      rc = kind == 0 ? launch_tile<double, M_MATVEC, false>(s, l, x, nullptr, y, nullptr, nullptr)
-                     : launch_tile<double, M_JACOBI, false>(s, l, x, bb, y, nullptr, nullptr);
+    for (int i = 0; i < reps && rc == VX_OK; ++i) rc = run(x, y, bb, up);
     HIPCHK(hipEventRecord(b, s->stream));
   } else {
     CHK(alloc_pcg32(s));
     float* x = l == 0 ? s->p32 : L.x32;
     float* y = l == 0 ? s->Ap32 : L.t32;
     float* bb = l == 0 ? s->r32 : L.b32;
+    float* up = L.w32;
     HIPCHK(hipMemsetAsync(x, 0, sizeof(float) * 3 * L.Nn, s->stream));
     HIPCHK(hipMemsetAsync(bb, 0, sizeof(float) * 3 * L.Nn, s->stream));
-    for (int w = 0; w < 2 && rc == VX_OK; ++w)
-      rc = kind == 0 ? launch_tile<float, M_MATVEC, false>(s, l, x, nullptr, y, nullptr, nullptr)
-                     : launch_tile<float, M_JACOBI, false>(s, l, x, bb, y, nullptr, nullptr);
+    HIPCHK(hipMemsetAsync(up, 0, sizeof(float) * 3 * L.Nn, s->stream));
+    for (int w = 0; w < 2 && rc == VX_OK; ++w) rc = run(x, y, bb, up);
     HIPCHK(hipEventRecord(a, s->stream));
-    for (int i = 0; i < reps && rc == VX_OK; ++i)
-      rc = kind == 0 ? launch_tile<float, M_MATVEC, false>(s, l, x, nullptr, y, nullptr, nullptr)
-                     : launch_tile<float, M_JACOBI, false>(s, l, x, bb, y, nullptr, nullptr);
+    for (int i = 0; i < reps && rc == VX_OK; ++i) rc = run(x, y, bb, up);
     HIPCHK(hipEventRecord(b, s->stream));
   }
   HIPCHK(hipEventSynchronize(b));
