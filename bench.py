@@ -33,7 +33,7 @@ HBM_PEAK_GBS = 8000.0
 
 # the MI355X solver configuration measured as `value`
 OPT = dict(precision="mixed", smoother="chebyshev", cheb_ratio=0.1, system="active",
-           inner_tol=2e-7)
+           inner_tol=1e-6)
 
 
 def log(msg):

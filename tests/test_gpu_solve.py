@@ -51,14 +51,14 @@ def test_fp64_matches_reference_trace(name):
     assert h.shape == ref.shape
     # 1e-4 relative plus an absolute floor of 1e-9 (ratios start near 1):
     # late CG iterations amplify summation-order rounding, most on e_min
-    # voids.  Jacobi-PCG amplifies it far more (its late ratios move by up to
-    # ~1e-2 relative under a different summation order); its first 20
-    # iterations are held to 1e-8.
+    # voids.  Jacobi-PCG amplifies it far more (its last ratios move by tens
+    # of percent under a different summation order): its first 20 iterations
+    # are held to 1e-8 and the next 15 to 1e-6; the count and u must match.
     if bool(g["use_multigrid"]):
         assert np.all(np.abs(h - ref) <= 1e-4 * ref + 1e-9)
     else:
         assert np.all(np.abs(h[:20] - ref[:20]) <= 1e-8 * ref[:20])
-        assert np.all(np.abs(h - ref) <= 3e-2 * ref)
+        assert np.all(np.abs(h[:35] - ref[:35]) <= 1e-6 * ref[:35])
     assert _rel(u, g["u"]) < 1e-6
     vm = vf.element_von_mises_stress(u, g["E_pa"]) / float(g["vsize"])
     assert _rel(vm, g["vm"]) < 1e-6

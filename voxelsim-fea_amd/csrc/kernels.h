@@ -32,7 +32,11 @@ struct TileArgs {
   int nx, ny, nz;        // elements
   int Lx, Ly, Lz;        // nodes
   int zc;                // owned node planes per work-group
+  int gx, gy, gz;        // tile grid (the launch is 1D, XCD-ordered)
   const T* __restrict__ u;
+  const T* __restrict__ pold;   // PUPD: previous direction; the kernel applies
+  T* __restrict__ pnew;         //   K to p = u + beta * pold and stores p
+  const double* __restrict__ beta;
   const T* __restrict__ E;
   const uint8_t* __restrict__ mask;
   const T* __restrict__ b;
@@ -45,9 +49,25 @@ struct TileArgs {
   double* __restrict__ partial;   // one double per work-group (DOT variants)
 };
 
-template <typename T>
-__device__ inline T shfl_up1(T v) {
-  return __shfl_up(v, 1, 64);
+// lane i <- lane i-1 inside each row of 16 lanes (DPP row_shr:1); the first
+// lane of a row receives 0.  One tile row of element columns is one DPP row.
+__device__ __forceinline__ float dpp_shr1(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x111, 0xf, 0xf, true));
+}
+__device__ __forceinline__ double dpp_shr1(double v) {
+  const long long w = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(w & 0xffffffffLL), 0x111, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(w >> 32), 0x111, 0xf, 0xf, true);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Inverse Jacobi diagonal 1/d (1 where d == 0, core/multigrid.py:303-304):
+// exact division in FP64 (the reference's arithmetic); the hardware
+// reciprocal (1 ulp) in FP32, where the smoother runs inside refinement.
+__device__ __forceinline__ double inv_diag(double d) { return 1.0 / (d == 0.0 ? 1.0 : d); }
+__device__ __forceinline__ float inv_diag(float d) {
+  return d == 0.0f ? 1.0f : __builtin_amdgcn_rcpf(d);
 }
 
 // Deterministic work-group sum of one double; thread 0 gets the result.
@@ -71,13 +91,17 @@ __device__ inline double block_sum(double v, double* sred) {
 // A work-group owns a 15x15 column of nodes and a chunk of zc node planes.
 // Its 256 threads map to the 16x16 element columns that touch those nodes
 // (one halo column on the low side).  The kernel marches up z: each step
-// stages one node plane of the 17x17 footprint in LDS (coalesced rows, the
-// next plane prefetched in registers), every thread applies its element of
-// the current layer in the Hadamard basis, the 2D modes of the forces of the
-// bottom plane are completed with the top-plane half kept from the previous
-// layer, and the node sums over the four element columns are formed by a
-// lane shuffle in x and one LDS exchange in y.  No atomics: every node's sum
-// is formed in a fixed order, so the result is deterministic.
+// stages one node plane of the 17x17 footprint in LDS (coalesced rows; the
+// plane after next is prefetched into registers), every thread applies its
+// element of the current layer in the Hadamard basis, the 2D modes of the
+// forces of the bottom plane are completed with the top-plane half kept from
+// the previous layer, and the node sums over the four element columns are
+// formed by a DPP row shift in x and one LDS exchange in y (double-buffered,
+// so one barrier per layer).  No atomics: every node's sum is formed in a
+// fixed order, so the result is deterministic.  Work-groups are numbered so
+// that each XCD (work-groups are dealt to the 8 XCDs round-robin) receives a
+// contiguous run of x-neighbouring tiles, whose shared halo then stays in
+// that XCD's L2.
 //
 // MODE  M_MATVEC: out = mask ? K u : 0                (core/multigrid.py:318-341)
 //       M_RESID : out = b - (mask ? K u : 0)          (:431)
@@ -86,95 +110,153 @@ __device__ inline double block_sum(double v, double* sred) {
 //       M_CHEB  : out = u + mask ? cu (u - up) + cr (b-Ku)/d : 0, one step of
 //                 the Chebyshev iteration on D^-1 A (three-term recurrence)
 // DOT   adds the work-group partial of  sum out*u (MATVEC) or sum out*b
-//       (JACOBI) to partial[block].
+//       (the smoothers) to partial[block]; RESID: sum out*out.
+// PUPD  (M_MATVEC) the input is p = u + beta*pold, formed while staging,
+//       and stored to pnew by the owners.
 // ---------------------------------------------------------------------------
-template <typename T, int MODE, bool DOT>
+template <typename T, int MODE, bool DOT, bool PUPD>
 __global__ __launch_bounds__(NT) void k_tile(TileArgs<T> a) {
-  __shared__ T sP[2][3][FOOT][FOOT];
-  __shared__ T sQ[4][TILE][TILE + 1];
+  __shared__ T sP[2][3 * FOOT * FOOT];
+  __shared__ T sQ[2][4][TILE][TILE + 1];
   __shared__ double sred[NT / 64];
+
+  // XCD-aware tile numbering
+  const int G = a.gx * a.gy * a.gz;
+  const int per = (G + 7) >> 3;
+  const int v = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+  if (v >= G) return;
+  const int bx = v % a.gx;
+  const int byz = v / a.gx;
+  const int by = byz % a.gy;
+  const int bz = byz / a.gy;
 
   const int tid = threadIdx.x;
   const int tx = tid & (TILE - 1), ty = tid / TILE;
-  const int X0 = blockIdx.x * OWN, Y0 = blockIdx.y * OWN;
-  const int Z0 = blockIdx.z * a.zc;
+  const int X0 = bx * OWN, Y0 = by * OWN;
+  const int Z0 = bz * a.zc;
   const int Z1 = min(Z0 + a.zc, a.Lz);
   const int ex = X0 - 1 + tx, ey = Y0 - 1 + ty;
   const bool col_in = ex >= 0 && ex < a.nx && ey >= 0 && ey < a.ny;
+  const int e0 = col_in ? ey * a.nx + ex : 0;
+  const int nxy = a.nx * a.ny;
   const bool owner = tx >= 1 && ty >= 1 && ex < a.Lx && ey < a.Ly;
-  const long long plane = (long long)a.Lx * a.Ly;
+  const int n0 = ey * a.Lx + ex;
+  const int plane = a.Lx * a.Ly;
+  const int plane3 = 3 * plane;
+
+  // staging descriptors: global offset inside a plane (-1: outside the grid)
+  // and LDS slot of the (up to) PRE values this thread moves per plane
+  int goff[PRE], loff[PRE];
+#pragma unroll
+  for (int k = 0; k < PRE; ++k) {
+    const int i = tid + k * NT;
+    goff[k] = -1;
+    loff[k] = -1;
+    if (i < FOOTVALS) {
+      const int row = i / (FOOT * 3);
+      const int col = i - row * (FOOT * 3);
+      const int xn = col / 3;
+      const int c = col - xn * 3;
+      const int x = X0 - 1 + xn, y = Y0 - 1 + row;
+      loff[k] = (c * FOOT + row) * FOOT + xn;
+      if (x >= 0 && x < a.Lx && y >= 0 && y < a.Ly) goff[k] = (y * a.Lx + x) * 3 + c;
+    }
+  }
+  const T beta = PUPD ? (T)(*a.beta) : T(0);
 
   T pre[PRE];
   auto fetch = [&](int pz) {
+    const bool zin = pz >= 0 && pz < a.Lz;
+    const int base = pz * plane3;
 #pragma unroll
     for (int k = 0; k < PRE; ++k) {
-      const int i = tid + k * NT;
-      T v = T(0);
-      if (i < FOOTVALS) {
-        const int row = i / (FOOT * 3);
-        const int col = i - row * (FOOT * 3);
-        const int x = X0 - 1 + col / 3;
-        const int y = Y0 - 1 + row;
-        if (pz >= 0 && pz < a.Lz && x >= 0 && x < a.Lx && y >= 0 && y < a.Ly)
-          v = a.u[((long long)pz * plane + (long long)y * a.Lx + x) * 3 + (col - (col / 3) * 3)];
+      T val = T(0);
+      if (zin && goff[k] >= 0) {
+        val = a.u[base + goff[k]];
+        if (PUPD) val += beta * a.pold[base + goff[k]];
       }
-      pre[k] = v;
+      pre[k] = val;
     }
   };
   auto store = [&](int buf) {
 #pragma unroll
-    for (int k = 0; k < PRE; ++k) {
-      const int i = tid + k * NT;
-      if (i < FOOTVALS) {
-        const int row = i / (FOOT * 3);
-        const int col = i - row * (FOOT * 3);
-        sP[buf][col - (col / 3) * 3][row][col / 3] = pre[k];
-      }
-    }
+    for (int k = 0; k < PRE; ++k)
+      if (loff[k] >= 0) sP[buf][loff[k]] = pre[k];
+  };
+  auto corners = [&](int buf, int c, T& u00, T& u10, T& u01, T& u11) {
+    const T* P = &sP[buf][(c * FOOT + ty) * FOOT + tx];
+    u00 = P[0];
+    u10 = P[1];
+    u01 = P[FOOT];
+    u11 = P[FOOT + 1];
   };
 
   T Pb[3][4], acc[3][4];
-  T ub0[3], ut0[3];
+  T ub[3];
   T Eprev = T(0);
   double dacc = 0.0;
 
-  // prologue: bottom plane of the first layer (Z0-1)
+  // prologue: plane Z0-1 (bottom of the first layer) and plane Z0
   fetch(Z0 - 1);
   store(0);
+  fetch(Z0);
   __syncthreads();
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    had2(sP[0][c][ty][tx], sP[0][c][ty][tx + 1], sP[0][c][ty + 1][tx], sP[0][c][ty + 1][tx + 1],
-         Pb[c]);
-    ut0[c] = sP[0][c][ty][tx];
+    T u00, u10, u01, u11;
+    corners(0, c, u00, u10, u01, u11);
+    had2(u00, u10, u01, u11, Pb[c]);
+    ub[c] = u00;
 #pragma unroll
     for (int m = 0; m < 4; ++m) acc[c][m] = T(0);
   }
-  fetch(Z0);
-  int buf = 1;
+  store(1);
+  fetch(Z0 + 1);
+  __syncthreads();
 
-  for (int iz = Z0 - 1; iz < Z1; ++iz) {
-    store(buf);
-    __syncthreads();
-    if (iz + 2 <= Z1) fetch(iz + 2);
-
-    T Pt[3][4];
+  const int J = Z1 - Z0 + 1;   // layers Z0-1 .. Z1-1
+  // software pipeline: E of the next layer and the owner's b / up / mask of
+  // the next output plane are loaded one step ahead
+  constexpr bool NEEDB = MODE != M_MATVEC;
+  constexpr bool NEEDUP = MODE == M_CHEB;
+  const bool has_up = NEEDUP && a.up != nullptr;
+  auto loadE = [&](int iz) -> T {
+    return (col_in && iz >= 0 && iz < a.nz) ? a.E[iz * nxy + e0] : T(0);
+  };
+  unsigned mk_n = 0;
+  T b_n[3] = {T(0), T(0), T(0)}, up_n[3] = {T(0), T(0), T(0)};
+  auto loadEpi = [&](int pz) {
+    if (owner && pz >= Z0 && pz < Z1) {
+      const int n = pz * plane + n0;
+      mk_n = a.mask[n];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        if (NEEDB) b_n[c] = a.b[3 * n + c];
+        if (NEEDUP) up_n[c] = has_up ? a.up[3 * n + c] : T(0);
+      }
+    }
+  };
+  T E_n = loadE(Z0 - 1);
+  loadEpi(Z0);
+  for (int j = 0; j < J; ++j) {
+    const int iz = Z0 - 1 + j;
+    const int top = (j + 1) & 1;
+    const T Ee = E_n;
+    E_n = loadE(iz + 1);
+    T Pt[3][4], ut[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      ub0[c] = ut0[c];
-      const T u00 = sP[buf][c][ty][tx];
-      had2(u00, sP[buf][c][ty][tx + 1], sP[buf][c][ty + 1][tx], sP[buf][c][ty + 1][tx + 1], Pt[c]);
-      ut0[c] = u00;
+      T u00, u10, u01, u11;
+      corners(top, c, u00, u10, u01, u11);
+      had2(u00, u10, u01, u11, Pt[c]);
+      ut[c] = u00;
     }
-    const T Ee = (col_in && iz >= 0 && iz < a.nz)
-                     ? a.E[((long long)iz * a.ny + ey) * a.nx + ex]
-                     : T(0);
     T Yb[3][4], Yt[3][4];
     elem_modes(Pb, Pt, Ee, a.K, Yb, Yt);
     const T Es = Eprev + Ee;
     Eprev = Ee;
-
-    T A0[3], A1[3];
+    T A0[3];
+    const int q = j & 1;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       T Y[4];
@@ -186,195 +268,288 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs<T> a) {
       }
       T y00, y10, y01, y11;
       ihad2(Y, y00, y10, y01, y11);
-      A0[c] = y00 + shfl_up1(y10);
-      A1[c] = y01 + shfl_up1(y11);
-      sQ[c][ty][tx] = A1[c];
+      A0[c] = y00 + dpp_shr1(y10);
+      sQ[q][c][ty][tx] = y01 + dpp_shr1(y11);
     }
-    const T EX = Es + shfl_up1(Es);
-    sQ[3][ty][tx] = EX;
+    const T EX = Es + dpp_shr1(Es);
+    sQ[q][3][ty][tx] = EX;
+    // stage the plane after the current top; prefetch the next one
+    store(j & 1);
+    fetch(iz + 3);
+    // this plane's epilogue operands (loaded one step ago)
+    const unsigned mk = mk_n;
+    T bb[3], upv[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      bb[c] = b_n[c];
+      upv[c] = up_n[c];
+    }
+    if (iz >= Z0) loadEpi(iz + 1);
     __syncthreads();
 
     if (iz >= Z0 && owner) {
-      const long long n = ((long long)iz * a.Ly + ey) * a.Lx + ex;
-      const unsigned mk = a.mask[n];
+      const int n = iz * plane + n0;
       T y[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) y[c] = A0[c] + sQ[c][ty - 1][tx];
+      for (int c = 0; c < 3; ++c) y[c] = A0[c] + sQ[q][c][ty - 1][tx];
+      if (PUPD) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a.pnew[3 * n + c] = ub[c];
+      }
       if (MODE == M_MATVEC) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           const T o = ((mk >> c) & 1u) ? y[c] : T(0);
           a.out[3 * n + c] = o;
-          if (DOT) dacc += (double)o * (double)ub0[c];
+          if (DOT) dacc += (double)o * (double)ub[c];
         }
       } else if (MODE == M_RESID) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          const T o = a.b[3 * n + c] - (((mk >> c) & 1u) ? y[c] : T(0));
+          const T o = bb[c] - (((mk >> c) & 1u) ? y[c] : T(0));
           a.out[3 * n + c] = o;
           if (DOT) dacc += (double)o * (double)o;
         }
-      } else if (MODE == M_JACOBI) {
-        const T d = a.kd * (EX + sQ[3][ty - 1][tx]);
-        const T minv = T(1) / (d == T(0) ? T(1) : d);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const T bb = a.b[3 * n + c];
-          const T o = ub0[c] + a.omega * (((mk >> c) & 1u) ? minv * (bb - y[c]) : T(0));
-          a.out[3 * n + c] = o;
-          if (DOT) dacc += (double)o * (double)bb;
-        }
       } else {
-        const T d = a.kd * (EX + sQ[3][ty - 1][tx]);
-        const T minv = T(1) / (d == T(0) ? T(1) : d);
+        const T d = a.kd * (EX + sQ[q][3][ty - 1][tx]);
+        const T minv = inv_diag(d);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          const T bb = a.b[3 * n + c];
-          const T du = a.up ? ub0[c] - a.up[3 * n + c] : ub0[c];
-          const T o = ub0[c] + (((mk >> c) & 1u) ? a.cu * du + a.cr * (minv * (bb - y[c])) : T(0));
+          T o;
+          if (MODE == M_JACOBI) {
+            o = ub[c] + a.omega * (((mk >> c) & 1u) ? minv * (bb[c] - y[c]) : T(0));
+          } else {
+            const T du = ub[c] - upv[c];
+            o = ub[c] + (((mk >> c) & 1u) ? a.cu * du + a.cr * (minv * (bb[c] - y[c])) : T(0));
+          }
           a.out[3 * n + c] = o;
-          if (DOT) dacc += (double)o * (double)bb;
+          if (DOT) dacc += (double)o * (double)bb[c];
         }
       }
     }
-    buf ^= 1;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) ub[c] = ut[c];
   }
 
   if (DOT) {
     const double s = block_sum(dacc, sred);
-    if (tid == 0)
-      a.partial[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = s;
+    if (tid == 0) a.partial[v] = s;
   }
 }
 
-// First damped-Jacobi sweep from a zero initial guess (core/multigrid.py:430
-// starts from zeros_like): A*0 = 0, so out = omega * mask * b / d.
+// Inverse Jacobi diagonal per node of a level: 1/(kd * sum of the <= 8
+// adjacent E), 1 where that is 0 (core/multigrid.py:300-304; the diagonal of
+// KE is the same for all 24 DOFs).  One thread per node, 3D launch.
 template <typename T>
-__global__ void k_jacobi_zero(int nx, int ny, int nz, const T* __restrict__ E,
-                              const uint8_t* __restrict__ mask, const T* __restrict__ b,
-                              T* __restrict__ out, T omega, T kd) {
-  const int Lx = nx + 1, Ly = ny + 1, Lz = nz + 1;
-  const long long Nn = (long long)Lx * Ly * Lz;
+__global__ void k_node_minv(int nx, int ny, int nz, const T* __restrict__ E, T kd,
+                            T* __restrict__ minv) {
+  const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+  const int iy = blockIdx.y, iz = blockIdx.z;
+  const int Lx = nx + 1, Ly = ny + 1;
+  if (ix >= Lx) return;
+  T es = T(0);
+#pragma unroll
+  for (int dz = -1; dz <= 0; ++dz)
+#pragma unroll
+    for (int dy = -1; dy <= 0; ++dy)
+#pragma unroll
+      for (int dx = -1; dx <= 0; ++dx) {
+        const int x = ix + dx, y = iy + dy, z = iz + dz;
+        if (x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz)
+          es += E[((long long)z * ny + y) * nx + x];
+      }
+  minv[((long long)iz * Ly + iy) * Lx + ix] = inv_diag(kd * es);
+}
+
+// ---------------------------------------------------------------------------
+// Coarsest-level smoother in one work-group: k smoothing steps from zero
+// (core/multigrid.py:428-429) on a grid small enough to keep x, the previous
+// iterate, b and every element's 24 forces in LDS.  Per step, one thread per
+// element applies its operator into its LDS slot, then one thread per DOF
+// sums the <= 8 slots of its node in a fixed order (no atomics):
+//   j = 0          : x1 = w0 * D^-1 b                      (A * 0 = 0)
+//   Jacobi, j > 0  : x  = x + w0 * D^-1 (b - K x)
+//   Chebyshev j > 0: x  = x + cu_j (x - x_prev) + cr_j D^-1 (b - K x)
+// (masked DOFs stay 0).
+// ---------------------------------------------------------------------------
+template <typename T>
+constexpr int coarse_max_elems() { return sizeof(T) == 8 ? 512 : 1024; }
+template <typename T>
+constexpr int coarse_max_nodes() { return sizeof(T) == 8 ? 800 : 1400; }
+constexpr int COARSE_MAX_STEPS = 64;
+
+struct CoarseCoef {
+  int steps;
+  int cheb;
+  double w0;
+  double cu[COARSE_MAX_STEPS];
+  double cr[COARSE_MAX_STEPS];
+};
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_coarse_smooth(int nx, int ny, int nz,
+                                                        const T* __restrict__ E,
+                                                        const uint8_t* __restrict__ mask,
+                                                        const T* __restrict__ minv,
+                                                        const T* __restrict__ b,
+                                                        T* __restrict__ out, ElemConst<T> K,
+                                                        CoarseCoef cc) {
+  constexpr int MAXN = coarse_max_nodes<T>(), MAXE = coarse_max_elems<T>();
+  __shared__ T sye[MAXE * 24];
+  __shared__ T sx[3 * MAXN], sxp[3 * MAXN], sb[3 * MAXN];
+  __shared__ T smi[MAXN];
+  __shared__ uint8_t smk[MAXN];
+  const int Lx = nx + 1, Ly = ny + 1;
+  const int Nn = Lx * Ly * (nz + 1);
+  const int Ne = nx * ny * nz;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int n = tid; n < Nn; n += nt) {
+    smk[n] = mask[n];
+    smi[n] = minv[n];
+  }
+  for (int i = tid; i < 3 * Nn; i += nt) sb[i] = b[i];
+  __syncthreads();
+  for (int i = tid; i < 3 * Nn; i += nt) {
+    const int n = i / 3, c = i - 3 * n;
+    sx[i] = ((smk[n] >> c) & 1u) ? (T)cc.w0 * smi[n] * sb[i] : T(0);
+    sxp[i] = T(0);
+  }
+  __syncthreads();
+  for (int st = 1; st < cc.steps; ++st) {
+    for (int e = tid; e < Ne; e += nt) {
+      const int ex = e % nx, ey = (e / nx) % ny, ez = e / (nx * ny);
+      T u[8][3], y[8][3];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int n = ((ez + (q >> 2)) * Ly + (ey + ((q >> 1) & 1))) * Lx + (ex + (q & 1));
+#pragma unroll
+        for (int c = 0; c < 3; ++c) u[q][c] = sx[3 * n + c];
+      }
+      elem_apply<T>(u, E[e], K, y);
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) sye[e * 24 + 3 * q + c] = y[q][c];
+    }
+    __syncthreads();
+    const T w0 = (T)cc.w0, cu = (T)cc.cu[st], cr = (T)cc.cr[st];
+    for (int i = tid; i < 3 * Nn; i += nt) {
+      const int n = i / 3, c = i - 3 * n;
+      const int ix = n % Lx, iy = (n / Lx) % Ly, iz = n / (Lx * Ly);
+      T kx = T(0);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {   // element at (node - corner offset q), corner q
+        const int ex = ix - (q & 1), ey = iy - ((q >> 1) & 1), ez = iz - (q >> 2);
+        if (ex >= 0 && ex < nx && ey >= 0 && ey < ny && ez >= 0 && ez < nz)
+          kx += sye[((ez * ny + ey) * nx + ex) * 24 + 3 * q + c];
+      }
+      const T x = sx[i];
+      T xn = x;
+      if ((smk[n] >> c) & 1u) {
+        const T r = smi[n] * (sb[i] - kx);
+        xn = cc.cheb ? x + cu * (x - sxp[i]) + cr * r : x + w0 * r;
+      }
+      sxp[i] = x;
+      sx[i] = xn;
+    }
+    __syncthreads();
+  }
+  for (int i = tid; i < 3 * Nn; i += nt) out[i] = sx[i];
+}
+
+// per-node values -> per-DOF (3 copies)
+__global__ void k_expand3(long long Nn, const double* __restrict__ a, double* __restrict__ b) {
   for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < Nn;
        n += (long long)gridDim.x * blockDim.x) {
-    const int ix = (int)(n % Lx);
-    const int iy = (int)((n / Lx) % Ly);
-    const int iz = (int)(n / ((long long)Lx * Ly));
-    T es = T(0);
-#pragma unroll
-    for (int dz = -1; dz <= 0; ++dz)
-#pragma unroll
-      for (int dy = -1; dy <= 0; ++dy)
-#pragma unroll
-        for (int dx = -1; dx <= 0; ++dx) {
-          const int x = ix + dx, y = iy + dy, z = iz + dz;
-          if (x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz)
-            es += E[((long long)z * ny + y) * nx + x];
-        }
-    const T d = kd * es;
-    const T minv = T(1) / (d == T(0) ? T(1) : d);
+    b[3 * n] = a[n];
+    b[3 * n + 1] = a[n];
+    b[3 * n + 2] = a[n];
+  }
+}
+
+// Smoothing step from a zero start: out = c * mask * minv * b (A*0 = 0).
+template <typename T>
+__global__ void k_zero_step(long long Nn, const uint8_t* __restrict__ mask,
+                            const T* __restrict__ minv, const T* __restrict__ b,
+                            T* __restrict__ out, T c) {
+  for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < Nn;
+       n += (long long)gridDim.x * blockDim.x) {
     const unsigned mk = mask[n];
+    const T w = c * minv[n];
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
-      out[3 * n + c] = omega * (((mk >> c) & 1u) ? minv * b[3 * n + c] : T(0));
-  }
-}
-
-// Inverse Jacobi diagonal per DOF (diagnostic / parity only).
-template <typename T>
-__global__ void k_inv_diag(int nx, int ny, int nz, const T* __restrict__ E, T kd,
-                           double* __restrict__ minv_out) {
-  const int Lx = nx + 1, Ly = ny + 1, Lz = nz + 1;
-  const long long Nn = (long long)Lx * Ly * Lz;
-  for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < Nn;
-       n += (long long)gridDim.x * blockDim.x) {
-    const int ix = (int)(n % Lx);
-    const int iy = (int)((n / Lx) % Ly);
-    const int iz = (int)(n / ((long long)Lx * Ly));
-    T es = T(0);
-    for (int dz = -1; dz <= 0; ++dz)
-      for (int dy = -1; dy <= 0; ++dy)
-        for (int dx = -1; dx <= 0; ++dx) {
-          const int x = ix + dx, y = iy + dy, z = iz + dz;
-          if (x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz)
-            es += E[((long long)z * ny + y) * nx + x];
-        }
-    const T d = kd * es;
-    const double mi = 1.0 / (double)(d == T(0) ? T(1) : d);
-    for (int c = 0; c < 3; ++c) minv_out[3 * n + c] = mi;
+    for (int k = 0; k < 3; ++k) out[3 * n + k] = ((mk >> k) & 1u) ? w * b[3 * n + k] : T(0);
   }
 }
 
 // R = P^T (27-point full weighting, unscaled) followed by the coarse mask
-// (core/multigrid.py:404-416).  One thread per coarse node.
+// (core/multigrid.py:404-416).  3D launch: one thread per coarse node of an
+// x-row, block rows (y, z) from the grid.
 template <typename T>
-__global__ void k_restrict(int Lcx, int Lcy, int Lcz, const T* __restrict__ rf,
-                           const uint8_t* __restrict__ cmask, T* __restrict__ rc) {
+__global__ void k_restrict3(int Lcx, int Lcy, int Lcz, const T* __restrict__ rf,
+                            const uint8_t* __restrict__ cmask, T* __restrict__ rc) {
+  const int jx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int jy = blockIdx.y, jz = blockIdx.z;
+  if (jx >= Lcx) return;
   const int Lfx = 2 * Lcx - 1, Lfy = 2 * Lcy - 1, Lfz = 2 * Lcz - 1;
-  const long long Nc = (long long)Lcx * Lcy * Lcz;
-  for (long long J = blockIdx.x * (long long)blockDim.x + threadIdx.x; J < Nc;
-       J += (long long)gridDim.x * blockDim.x) {
-    const int jx = (int)(J % Lcx);
-    const int jy = (int)((J / Lcx) % Lcy);
-    const int jz = (int)(J / ((long long)Lcx * Lcy));
-    T s[3] = {T(0), T(0), T(0)};
-    for (int dz = -1; dz <= 1; ++dz) {
-      const int z = 2 * jz + dz;
-      if (z < 0 || z >= Lfz) continue;
-      const T wz = dz == 0 ? T(1) : T(0.5);
-      for (int dy = -1; dy <= 1; ++dy) {
-        const int y = 2 * jy + dy;
-        if (y < 0 || y >= Lfy) continue;
-        const T wzy = wz * (dy == 0 ? T(1) : T(0.5));
-        const long long rowb = ((long long)z * Lfy + y) * Lfx;
+  T s0 = T(0), s1 = T(0), s2 = T(0);
 #pragma unroll
-        for (int dx = -1; dx <= 1; ++dx) {
-          const int x = 2 * jx + dx;
-          if (x < 0 || x >= Lfx) continue;
-          const T w = wzy * (dx == 0 ? T(1) : T(0.5));
-          const long long i = (rowb + x) * 3;
-          s[0] += w * rf[i];
-          s[1] += w * rf[i + 1];
-          s[2] += w * rf[i + 2];
-        }
+  for (int dz = -1; dz <= 1; ++dz) {
+    const int z = 2 * jz + dz;
+    if (z < 0 || z >= Lfz) continue;
+    const T wz = dz == 0 ? T(1) : T(0.5);
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int y = 2 * jy + dy;
+      if (y < 0 || y >= Lfy) continue;
+      const T wzy = wz * (dy == 0 ? T(1) : T(0.5));
+      const T* row = rf + ((long long)z * Lfy + y) * Lfx * 3;
+#pragma unroll
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int x = 2 * jx + dx;
+        if (x < 0 || x >= Lfx) continue;
+        const T w = wzy * (dx == 0 ? T(1) : T(0.5));
+        s0 += w * row[3 * x];
+        s1 += w * row[3 * x + 1];
+        s2 += w * row[3 * x + 2];
       }
     }
-    const unsigned mk = cmask[J];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) rc[3 * J + c] = ((mk >> c) & 1u) ? s[c] : T(0);
   }
+  const long long J = ((long long)jz * Lcy + jy) * Lcx + jx;
+  const unsigned mk = cmask[J];
+  rc[3 * J] = (mk & 1u) ? s0 : T(0);
+  rc[3 * J + 1] = (mk & 2u) ? s1 : T(0);
+  rc[3 * J + 2] = (mk & 4u) ? s2 : T(0);
 }
 
-// out = x + mask * (P e) with trilinear P (core/multigrid.py:418-424, 434).
-// x may alias out.
+// x += mask * (P e) with trilinear P (core/multigrid.py:418-424, 434).
+// 3D launch: one thread per fine node of an x-row.
 template <typename T>
-__global__ void k_prolong_add(int Lfx, int Lfy, int Lfz, const T* __restrict__ ec,
-                              const uint8_t* __restrict__ fmask, const T* x, T* out) {
+__global__ void k_prolong_add3(int Lfx, int Lfy, int Lfz, const T* __restrict__ ec,
+                               const uint8_t* __restrict__ fmask, T* __restrict__ x) {
+  const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+  const int iy = blockIdx.y, iz = blockIdx.z;
+  if (ix >= Lfx) return;
   const int Lcx = (Lfx + 1) / 2, Lcy = (Lfy + 1) / 2;
-  const long long Nf = (long long)Lfx * Lfy * Lfz;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < Nf;
-       i += (long long)gridDim.x * blockDim.x) {
-    const int ix = (int)(i % Lfx);
-    const int iy = (int)((i / Lfx) % Lfy);
-    const int iz = (int)(i / ((long long)Lfx * Lfy));
-    const int cx = ix >> 1, cy = iy >> 1, cz = iz >> 1;
-    const int nxs = (ix & 1) ? 2 : 1, nys = (iy & 1) ? 2 : 1, nzs = (iz & 1) ? 2 : 1;
-    const T wx = (ix & 1) ? T(0.5) : T(1), wy = (iy & 1) ? T(0.5) : T(1),
-            wz = (iz & 1) ? T(0.5) : T(1);
-    T s[3] = {T(0), T(0), T(0)};
-    for (int kz = 0; kz < nzs; ++kz)
-      for (int ky = 0; ky < nys; ++ky)
-        for (int kx = 0; kx < nxs; ++kx) {
-          const long long J = ((long long)(cz + kz) * Lcy + (cy + ky)) * Lcx + (cx + kx);
-          const T w = wx * wy * wz;
-          s[0] += w * ec[3 * J];
-          s[1] += w * ec[3 * J + 1];
-          s[2] += w * ec[3 * J + 2];
-        }
-    const unsigned mk = fmask[i];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) out[3 * i + c] = x[3 * i + c] + (((mk >> c) & 1u) ? s[c] : T(0));
-  }
+  const int cx = ix >> 1, cy = iy >> 1, cz = iz >> 1;
+  const int ox = ix & 1, oy = iy & 1, oz = iz & 1;
+  const T wx = ox ? T(0.5) : T(1), wy = oy ? T(0.5) : T(1), wz = oz ? T(0.5) : T(1);
+  T s0 = T(0), s1 = T(0), s2 = T(0);
+  for (int kz = 0; kz <= oz; ++kz)
+    for (int ky = 0; ky <= oy; ++ky) {
+      const T* row = ec + ((long long)(cz + kz) * Lcy + (cy + ky)) * Lcx * 3;
+      for (int kx = 0; kx <= ox; ++kx) {
+        const T w = wx * wy * wz;
+        const T* q = row + 3 * (cx + kx);
+        s0 += w * q[0];
+        s1 += w * q[1];
+        s2 += w * q[2];
+      }
+    }
+  const long long i = ((long long)iz * Lfy + iy) * Lfx + ix;
+  const unsigned mk = fmask[i];
+  if (mk & 1u) x[3 * i] += s0;
+  if (mk & 2u) x[3 * i + 1] += s1;
+  if (mk & 4u) x[3 * i + 2] += s2;
 }
 
 // E of level l+1 = 2 * mean of the 8 children (scaled E carries 2^l;
@@ -523,7 +698,10 @@ __global__ void k_finalize(const double* __restrict__ partial, int n, int op, Sc
     switch (op) {
       case F_SET_TMP: s->tmp = v; break;
       case F_FNORM: s->fnorm = sqrt(v); break;
-      case F_RZ0: s->rz = v; break;
+      case F_RZ0:
+        s->rz = v;
+        s->beta = 0.0;
+        break;
       case F_ALPHA:
         s->pAp = v;
         s->alpha = (v == 0.0) ? 0.0 : s->rz / v;
@@ -543,17 +721,51 @@ __global__ void k_finalize(const double* __restrict__ partial, int n, int op, Sc
 
 __global__ void k_set_fnorm(Scalars* s, double v) { s->fnorm = v; }
 
-// u += alpha p ; r -= alpha Ap ; partial sum r.r
+// 4 consecutive values through one (float4) or two (double2) vector accesses
+__device__ __forceinline__ void ld4(const float* p, float v[4]) {
+  const float4 q = *reinterpret_cast<const float4*>(p);
+  v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+}
+__device__ __forceinline__ void st4(float* p, const float v[4]) {
+  *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+}
+__device__ __forceinline__ void ld4(const double* p, double v[4]) {
+  const double2 a = *reinterpret_cast<const double2*>(p);
+  const double2 b = *reinterpret_cast<const double2*>(p + 2);
+  v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+}
+__device__ __forceinline__ void st4(double* p, const double v[4]) {
+  *reinterpret_cast<double2*>(p) = make_double2(v[0], v[1]);
+  *reinterpret_cast<double2*>(p + 2) = make_double2(v[2], v[3]);
+}
+
+// u += alpha p ; r -= alpha Ap ; partial sum r.r  (arrays 16-byte aligned)
 template <typename T>
 __global__ void k_cg_update(long long n, const Scalars* __restrict__ s, T* __restrict__ u,
                             T* __restrict__ r, const T* __restrict__ p, const T* __restrict__ Ap,
                             double* __restrict__ partial) {
   __shared__ double sred[4];
-  const double alpha = s->alpha;
-  const T a = (T)alpha;
+  const T a = (T)s->alpha;
   double acc = 0.0;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+  const long long n4 = n >> 2;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n4;
        i += (long long)gridDim.x * blockDim.x) {
+    T uu[4], rr[4], pp[4], aa[4];
+    ld4(u + 4 * i, uu);
+    ld4(r + 4 * i, rr);
+    ld4(p + 4 * i, pp);
+    ld4(Ap + 4 * i, aa);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      uu[k] = uu[k] + a * pp[k];
+      rr[k] = rr[k] - a * aa[k];
+      acc += (double)rr[k] * (double)rr[k];
+    }
+    st4(u + 4 * i, uu);
+    st4(r + 4 * i, rr);
+  }
+  if (blockIdx.x == 0 && threadIdx.x < (n & 3)) {
+    const long long i = 4 * n4 + threadIdx.x;
     u[i] = u[i] + a * p[i];
     const T rn = r[i] - a * Ap[i];
     r[i] = rn;

@@ -37,6 +37,8 @@ struct Level {
   double *b64 = nullptr, *x64 = nullptr, *t64 = nullptr, *w64 = nullptr;
   float *b32 = nullptr, *x32 = nullptr, *t32 = nullptr, *w32 = nullptr;
   uint8_t* mask_fixed = nullptr;   // injected fixed-DOF mask (set_fixed)
+  double* mi64 = nullptr;          // inverse Jacobi diagonal per node
+  float* mi32 = nullptr;
 };
 
 template <typename T> struct Sel;
@@ -46,6 +48,7 @@ template <> struct Sel<double> {
   static double*& t(Level& l) { return l.t64; }
   static double*& w(Level& l) { return l.w64; }
   static double* E(Level& l) { return l.E64; }
+  static double* mi(Level& l) { return l.mi64; }
 };
 template <> struct Sel<float> {
   static float*& b(Level& l) { return l.b32; }
@@ -53,6 +56,7 @@ template <> struct Sel<float> {
   static float*& t(Level& l) { return l.t32; }
   static float*& w(Level& l) { return l.w32; }
   static float* E(Level& l) { return l.E32; }
+  static float* mi(Level& l) { return l.mi32; }
 };
 
 }  // namespace
@@ -71,9 +75,10 @@ struct vx_solver {
   bool fixed_set = false, density_set = false;
   bool active_system = false;
   // level-0 PCG vectors
-  double *u64 = nullptr, *r64 = nullptr, *z64 = nullptr, *p64 = nullptr, *Ap64 = nullptr,
-         *fm64 = nullptr;
-  float *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *Ap32 = nullptr, *d32 = nullptr;
+  double *u64 = nullptr, *r64 = nullptr, *z64 = nullptr, *p64 = nullptr, *q64 = nullptr,
+         *Ap64 = nullptr, *fm64 = nullptr;
+  float *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *q32 = nullptr, *Ap32 = nullptr,
+        *d32 = nullptr;
   Level jac;                  // active-DOF system of the Jacobi-PCG fallback
   double* jac_minv = nullptr; // per node
   double* partial = nullptr;
@@ -153,14 +158,23 @@ struct SmoothCoef {
   double cu = 0, cr = 0;
 };
 
-template <typename T, int MODE, bool DOT>
+struct PUpd {
+  const void* pold = nullptr;
+  void* pnew = nullptr;
+  const double* beta = nullptr;
+};
+
+template <typename T, int MODE, bool DOT, bool PUPD = false>
 int launch_tile_L(vx_solver* s, Level& L, const T* u, const T* b, T* out, double* partial,
-                  int* n_part, const SmoothCoef* sc = nullptr) {
+                  int* n_part, const SmoothCoef* sc = nullptr, const PUpd* pu = nullptr) {
   TileArgs<T> a;
   a.nx = L.nx; a.ny = L.ny; a.nz = L.nz;
   a.Lx = L.Lx; a.Ly = L.Ly; a.Lz = L.Lz;
   a.zc = zchunk(s, L);
   a.u = u; a.E = Sel<T>::E(L); a.mask = L.mask; a.b = b; a.out = out;
+  a.pold = pu ? static_cast<const T*>(pu->pold) : nullptr;
+  a.pnew = pu ? static_cast<T*>(pu->pnew) : nullptr;
+  a.beta = pu ? pu->beta : nullptr;
   a.omega = (T)(sc ? sc->omega : s->opt.omega);
   a.up = sc ? static_cast<const T*>(sc->up) : nullptr;
   a.cu = (T)(sc ? sc->cu : 0.0);
@@ -169,10 +183,13 @@ int launch_tile_L(vx_solver* s, Level& L, const T* u, const T* b, T* out, double
   a.K = kconst<T>(s);
   a.partial = partial;
   dim3 g = tile_grid(s, L, a.zc);
-  if (n_part) *n_part = (int)(g.x * g.y * g.z);
-  if (DOT && (long long)g.x * g.y * g.z > s->n_partial)
-    return fail(s, VX_ERR_ARG, "partial buffer too small");
-  hipLaunchKernelGGL((k_tile<T, MODE, DOT>), g, dim3(NT), 0, s->stream, a);
+  a.gx = (int)g.x; a.gy = (int)g.y; a.gz = (int)g.z;
+  const long long G = (long long)g.x * g.y * g.z;
+  if (n_part) *n_part = (int)G;
+  if (DOT && G > s->n_partial) return fail(s, VX_ERR_ARG, "partial buffer too small");
+  const long long launch = ((G + 7) / 8) * 8;   // XCD-ordered 1D grid
+  hipLaunchKernelGGL((k_tile<T, MODE, DOT, PUPD>), dim3((unsigned)launch), dim3(NT), 0, s->stream,
+                     a);
   LAUNCHCHK();
   return VX_OK;
 }
@@ -195,9 +212,8 @@ int sweep(vx_solver* s, int li, const T* u, const T* b, T* out, bool dot, int* n
           double omega) {
   Level& L = s->lv[li];
   if (!u) {
-    hipLaunchKernelGGL(k_jacobi_zero<T>, dim3(ew_grid(L.Nn)), dim3(EW_THREADS), 0, s->stream,
-                       L.nx, L.ny, L.nz, Sel<T>::E(L), L.mask, b, out, (T)omega,
-                       (T)s->kd);
+    hipLaunchKernelGGL(k_zero_step<T>, dim3(ew_grid(L.Nn)), dim3(EW_THREADS), 0, s->stream,
+                       L.Nn, L.mask, Sel<T>::mi(L), b, out, (T)omega);
     LAUNCHCHK();
     if (dot) {
       hipLaunchKernelGGL(k_dot<T>, dim3(EW_BLOCKS), dim3(EW_THREADS), 0, s->stream, 3 * L.Nn,
@@ -314,6 +330,36 @@ int smooth_step(vx_solver* s, int li, const T* u_in, const T* b, int k, T* dst, 
   return smooth_n<T>(s, li, u_in, b, k, dst, tmp, dot, n_part);
 }
 
+// All coarsest-level smoothing steps in one single-work-group launch
+// (same step sequence as smooth_step from zero).
+template <typename T>
+int coarse_fused(vx_solver* s, Level& L, const T* b, T* x) {
+  CoarseCoef cc{};
+  const int k = s->opt.coarse_sweeps;
+  cc.steps = k;
+  if (s->opt.smoother == VX_SMOOTH_CHEBYSHEV) {
+    const double beta = s->lam_max, alpha = s->opt.cheb_ratio * beta;
+    const double theta = 0.5 * (beta + alpha), delta = 0.5 * (beta - alpha);
+    const double sigma = theta / delta;
+    cc.cheb = 1;
+    cc.w0 = 1.0 / theta;
+    double rho_prev = 1.0 / sigma;
+    for (int j = 1; j < k; ++j) {
+      const double rho = 1.0 / (2.0 * sigma - rho_prev);
+      cc.cu[j] = rho * rho_prev;
+      cc.cr[j] = 2.0 * rho / delta;
+      rho_prev = rho;
+    }
+  } else {
+    cc.cheb = 0;
+    cc.w0 = s->opt.omega;
+  }
+  hipLaunchKernelGGL(k_coarse_smooth<T>, dim3(1), dim3(1024), 0, s->stream, L.nx, L.ny, L.nz,
+                     Sel<T>::E(L), L.mask, Sel<T>::mi(L), b, x, kconst<T>(s), cc);
+  LAUNCHCHK();
+  return VX_OK;
+}
+
 // V-cycle (core/multigrid.py:426-436): b at level 0 -> result in z0.
 // t0 and w0 are level-0 scratch vectors (distinct from b0, z0).
 // dot: partials of z0.b0 from the last level-0 write.
@@ -330,15 +376,20 @@ int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, T* w0, bool dot, int* n_part
     T* w = li == 0 ? w0 : Sel<T>::w(L);
     if (li == nl - 1) {
       const int k = s->opt.coarse_sweeps;
-      CHK(smooth_step<T>(s, li, nullptr, b, k, x, t, w, dot && li == 0, n_part));
+      if (li > 0 && L.Nn <= coarse_max_nodes<T>() && L.Ne <= coarse_max_elems<T>() &&
+          k <= COARSE_MAX_STEPS) {
+        CHK(coarse_fused<T>(s, L, b, x));
+      } else {
+        CHK(smooth_step<T>(s, li, nullptr, b, k, x, t, w, dot && li == 0, n_part));
+      }
       break;
     }
     CHK(smooth_step<T>(s, li, nullptr, b, ns, x, t, w, false, nullptr));
     // residual into t, then restrict into b of level li+1
     CHK((launch_tile<T, M_RESID, false>(s, li, x, b, t, nullptr, nullptr)));
     Level& C = s->lv[li + 1];
-    hipLaunchKernelGGL(k_restrict<T>, dim3(ew_grid(C.Nn)), dim3(EW_THREADS), 0, s->stream, C.Lx,
-                       C.Ly, C.Lz, t, C.mask, Sel<T>::b(C));
+    hipLaunchKernelGGL(k_restrict3<T>, dim3((C.Lx + 63) / 64, C.Ly, C.Lz), dim3(64), 0,
+                       s->stream, C.Lx, C.Ly, C.Lz, t, C.mask, Sel<T>::b(C));
     LAUNCHCHK();
   }
   // ascend
@@ -349,8 +400,8 @@ int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, T* w0, bool dot, int* n_part
     T* x = li == 0 ? z0 : Sel<T>::x(L);
     T* t = li == 0 ? t0 : Sel<T>::t(L);
     T* w = li == 0 ? w0 : Sel<T>::w(L);
-    hipLaunchKernelGGL(k_prolong_add<T>, dim3(ew_grid(L.Nn)), dim3(EW_THREADS), 0, s->stream, L.Lx,
-                       L.Ly, L.Lz, Sel<T>::x(C), L.mask, x, x);
+    hipLaunchKernelGGL(k_prolong_add3<T>, dim3((L.Lx + 63) / 64, L.Ly, L.Lz), dim3(64), 0,
+                       s->stream, L.Lx, L.Ly, L.Lz, Sel<T>::x(C), L.mask, x);
     LAUNCHCHK();
     CHK(smooth_step<T>(s, li, x, b, ns, x, t, w, dot && li == 0, n_part));
   }
@@ -383,7 +434,7 @@ struct PcgResult {
 // ratio_scale (1 for a plain solve; the outer residual ratio for an inner
 // solve of iterative refinement) and cb_tol.
 template <typename T>
-int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* Ap, T* t0, double tol_rel, int max_iter,
+int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* p2, T* Ap, T* t0, double tol_rel, int max_iter,
         bool stall_guard, double ratio_scale, double cb_tol, int iter_offset, vx_progress_cb cb,
         void* user, double* hist, int hist_cap, PcgResult* res) {
   Level& L0 = s->lv[0];
@@ -393,24 +444,35 @@ int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* Ap, T* t0, double tol_rel, int 
   HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
   auto cleanup = [&]() { (void)hipEventDestroy(ev); };
 
-  int rc = vcycle<T>(s, r, z, t0, Sel<T>::w(s->lv[0]), true, &np);
-  if (rc == VX_OK) rc = finalize(s, np, F_RZ0);
+  int rc = vcycle<T>(s, r, z, t0, Sel<T>::w(L0), true, &np);
+  if (rc == VX_OK) rc = finalize(s, np, F_RZ0);   // also beta = 0
   if (rc != VX_OK) { cleanup(); return rc; }
-  if (hipMemcpyAsync(p, z, sizeof(T) * n, hipMemcpyDeviceToDevice, s->stream) != hipSuccess) {
+  // both direction buffers start at 0 so that p = z + 0 * p_prev is exact
+  if (hipMemsetAsync(p, 0, sizeof(T) * n, s->stream) != hipSuccess ||
+      hipMemsetAsync(p2, 0, sizeof(T) * n, s->stream) != hipSuccess) {
     cleanup();
-    return fail(s, VX_ERR_HIP, "copy z->p");
+    return fail(s, VX_ERR_HIP, "memset p");
   }
+  T* pb[2] = {p, p2};
   double best = INFINITY;
   double prev_ratio = NAN, prev2_ratio = NAN;
   res->iters = 0;
   res->converged = false;
   res->ratio = NAN;
   for (int it = 0; it < max_iter; ++it) {
-    rc = launch_tile<T, M_MATVEC, true>(s, 0, p, nullptr, Ap, s->partial, &np);
+    T* pcur = pb[it & 1];
+    T* pprev = pb[(it + 1) & 1];
+    // p = z + beta p_prev formed inside the operator kernel; Ap; p.Ap partials
+    PUpd pu;
+    pu.pold = pprev;
+    pu.pnew = pcur;
+    pu.beta = &s->dscal->beta;
+    rc = launch_tile_L<T, M_MATVEC, true, true>(s, L0, z, nullptr, Ap, s->partial, &np, nullptr,
+                                                 &pu);
     if (rc == VX_OK) rc = finalize(s, np, F_ALPHA);
     if (rc != VX_OK) break;
     hipLaunchKernelGGL(k_cg_update<T>, dim3(EW_BLOCKS), dim3(EW_THREADS), 0, s->stream, n,
-                       s->dscal, u, r, p, Ap, s->partial);
+                       s->dscal, u, r, pcur, Ap, s->partial);
     if (hipGetLastError() != hipSuccess) { rc = fail(s, VX_ERR_HIP, "cg_update launch"); break; }
     rc = finalize(s, EW_BLOCKS, F_RATIO);
     if (rc == VX_OK) rc = read_scalars(s, ev);
@@ -425,11 +487,8 @@ int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* Ap, T* t0, double tol_rel, int 
     }
     auto queue_next = [&]() -> int {
       int np2 = 0;
-      CHK(vcycle<T>(s, r, z, t0, Sel<T>::w(s->lv[0]), true, &np2));
+      CHK(vcycle<T>(s, r, z, t0, Sel<T>::w(L0), true, &np2));
       CHK(finalize(s, np2, F_BETA));
-      hipLaunchKernelGGL(k_p_update<T>, dim3(EW_BLOCKS), dim3(EW_THREADS), 0, s->stream, n,
-                         s->dscal, z, p);
-      LAUNCHCHK();
       return VX_OK;
     };
     bool queued = false;
@@ -577,6 +636,8 @@ int vx_create(int32_t nx, int32_t ny, int32_t nz, double nu, const vx_options* o
   for (auto& L : s->lv) {
     if ((rc = dalloc(s, &L.mask, L.Nn)) != VX_OK) break;
     if ((rc = dalloc(s, &L.mask_fixed, L.Nn)) != VX_OK) break;
+    if ((rc = dalloc(s, &L.mi64, L.Nn)) != VX_OK) break;
+    if ((rc = dalloc(s, &L.mi32, L.Nn)) != VX_OK) break;
     if ((rc = dalloc(s, &L.E64, L.Ne)) != VX_OK) break;
     if ((rc = dalloc(s, &L.E32, L.Ne)) != VX_OK) break;
   }
@@ -664,6 +725,13 @@ int vx_set_density(vx_handle s, const double* E, const uint8_t* active, double e
     hipLaunchKernelGGL(k_effective_mask, dim3(ew_grid(L.Nn)), dim3(EW_THREADS), 0, s->stream,
                        L.nx, L.ny, L.nz, L.E64, L.mask_fixed, active_sys ? 1 : 0, L.mask);
     LAUNCHCHK();
+    const dim3 g3((L.Lx + 63) / 64, L.Ly, L.Lz);
+    hipLaunchKernelGGL(k_node_minv<double>, g3, dim3(64), 0, s->stream, L.nx, L.ny, L.nz, L.E64,
+                       s->kd, L.mi64);
+    LAUNCHCHK();
+    hipLaunchKernelGGL(k_node_minv<float>, g3, dim3(64), 0, s->stream, L.nx, L.ny, L.nz, L.E32,
+                       (float)s->kd, L.mi32);
+    LAUNCHCHK();
   }
   s->density_set = true;
   return VX_OK;
@@ -676,6 +744,7 @@ static int alloc_pcg64(vx_solver* s) {
   CHK(dalloc(s, &s->r64, n));
   CHK(dalloc(s, &s->z64, n));
   CHK(dalloc(s, &s->p64, n));
+  CHK(dalloc(s, &s->q64, n));
   CHK(dalloc(s, &s->Ap64, n));
   CHK(dalloc(s, &s->fm64, n));
   return alloc_level_vectors<double>(s);
@@ -686,6 +755,7 @@ static int alloc_pcg32(vx_solver* s) {
   CHK(dalloc(s, &s->r32, n));
   CHK(dalloc(s, &s->z32, n));
   CHK(dalloc(s, &s->p32, n));
+  CHK(dalloc(s, &s->q32, n));
   CHK(dalloc(s, &s->Ap32, n));
   CHK(dalloc(s, &s->d32, n));
   return alloc_level_vectors<float>(s);
@@ -750,8 +820,8 @@ int vx_solve(vx_handle s, const double* f, double* u, const double* x0, double t
       rc = fail(s, VX_ERR_HIP, "copy f->r");
     }
     if (rc == VX_OK)
-      rc = pcg<double>(s, u, s->r64, s->z64, s->p64, s->Ap64, s->lv[0].t64, tol, max_iter, true,
-                       1.0, tol, 0, cb, user, hist, hist_cap, &pr);
+      rc = pcg<double>(s, u, s->r64, s->z64, s->p64, s->q64, s->Ap64, s->lv[0].t64, tol,
+                       max_iter, true, 1.0, tol, 0, cb, user, hist, hist_cap, &pr);
     inf.iters = pr.iters;
     inf.converged = pr.converged ? 1 : 0;
     inf.resid_ratio = pr.ratio;
@@ -793,7 +863,7 @@ int vx_solve(vx_handle s, const double* f, double* u, const double* x0, double t
       // the inner right-hand side has unit norm
       hipLaunchKernelGGL(k_set_fnorm, dim3(1), dim3(1), 0, s->stream, s->dscal, 1.0);
       LAUNCHCHK();
-      rc = pcg<float>(s, s->d32, s->r32, s->z32, s->p32, s->Ap32, s->lv[0].t32, want,
+      rc = pcg<float>(s, s->d32, s->r32, s->z32, s->p32, s->q32, s->Ap32, s->lv[0].t32, want,
                       max_iter - total, false, true_ratio, tol, total, cb, user, hist, hist_cap,
                       &pr);
       total += pr.iters;
@@ -1042,8 +1112,8 @@ int vx_restrict(vx_handle s, int32_t l, const double* rf, double* rc) {
   CHK(check_level(s, l));
   if (l + 1 >= (int)s->lv.size()) return fail(s, VX_ERR_ARG, "no coarser level");
   Level& C = s->lv[l + 1];
-  hipLaunchKernelGGL(k_restrict<double>, dim3(ew_grid(C.Nn)), dim3(EW_THREADS), 0, s->stream,
-                     C.Lx, C.Ly, C.Lz, rf, C.mask, rc);
+  hipLaunchKernelGGL(k_restrict3<double>, dim3((C.Lx + 63) / 64, C.Ly, C.Lz), dim3(64), 0,
+                     s->stream, C.Lx, C.Ly, C.Lz, rf, C.mask, rc);
   LAUNCHCHK();
   HIPCHK(hipStreamSynchronize(s->stream));
   return VX_OK;
@@ -1054,8 +1124,8 @@ int vx_prolong(vx_handle s, int32_t l, const double* ec, double* ef) {
   if (l + 1 >= (int)s->lv.size()) return fail(s, VX_ERR_ARG, "no coarser level");
   Level& L = s->lv[l];
   HIPCHK(hipMemsetAsync(ef, 0, sizeof(double) * 3 * L.Nn, s->stream));
-  hipLaunchKernelGGL(k_prolong_add<double>, dim3(ew_grid(L.Nn)), dim3(EW_THREADS), 0, s->stream,
-                     L.Lx, L.Ly, L.Lz, ec, L.mask, ef, ef);
+  hipLaunchKernelGGL(k_prolong_add3<double>, dim3((L.Lx + 63) / 64, L.Ly, L.Lz), dim3(64), 0,
+                     s->stream, L.Lx, L.Ly, L.Lz, ec, L.mask, ef);
   LAUNCHCHK();
   HIPCHK(hipStreamSynchronize(s->stream));
   return VX_OK;
@@ -1082,8 +1152,8 @@ int vx_vcycle(vx_handle s, int32_t prec, const double* b, double* z) {
 int vx_inv_diag(vx_handle s, int32_t l, double* minv) {
   CHK(check_level(s, l));
   Level& L = s->lv[l];
-  hipLaunchKernelGGL(k_inv_diag<double>, dim3(ew_grid(L.Nn)), dim3(EW_THREADS), 0, s->stream,
-                     L.nx, L.ny, L.nz, L.E64, s->kd, minv);
+  hipLaunchKernelGGL(k_expand3, dim3(ew_grid(L.Nn)), dim3(EW_THREADS), 0, s->stream, L.Nn,
+                     L.mi64, minv);
   LAUNCHCHK();
   HIPCHK(hipStreamSynchronize(s->stream));
   return VX_OK;
