@@ -79,6 +79,54 @@ def cpu_baseline(p, iters, budget_s=30.0):
                        f"iterations x 6.7 applications")}
 
 
+def measured_traffic():
+    """HBM bytes per launch of the dominant kernel, from the committed
+    counter summary (two rocprofv3 --pmc passes, FETCH_SIZE and WRITE_SIZE,
+    scaled by a 1 GiB copy); None when the summary is absent."""
+    import re
+    path = os.path.join(ROOT, "profiles", "chebyshev_fp32_256_traffic.md")
+    try:
+        m = re.search(r"= \*\*([0-9.]+) MB\*\*", open(path).read())
+        return float(m.group(1)) * 1e6 if m else None
+    except OSError:
+        return None
+
+
+def copy_bandwidth(nbytes=2 << 30, reps=10):
+    """Achievable HBM bandwidth: torch's device copy of a buffer of known size
+    (read + write bytes / time)."""
+    import torch
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    return 2 * nbytes / dt / 1e9
+
+
+def kernel_only(p, E_dev, act_dev, MGSolver):
+    """The level-0 FP32 Chebyshev tile kernel 20 times, then a plain 1 GiB
+    FP32 copy 5 times (the reference dispatch for counter scaling)."""
+    import torch
+    opt = MGSolver(p.nx, p.ny, p.nz, p.fixed_dofs, nu=p.nu, compute_mode="GPU", **OPT)
+    opt.set_density(E_dev, active=act_dev)
+    ms = opt.time_kernel("chebyshev", 0, "mixed", 20)
+    a = torch.ones((1 << 30) // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    nn = (p.nx + 1) * (p.ny + 1) * (p.nz + 1)
+    print(json.dumps({"kernel": "chebyshev_fp32_level0", "ms": ms,
+                      "algorithmic_bytes": 48 * nn + nn + 4 * p.nelem,
+                      "copy_bytes_read": 1 << 30, "copy_bytes_written": 1 << 30}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +136,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ref-iters", type=int, default=3000,
                     help="Jacobi-PCG fallback iterations of the reference comparator")
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="run only the dominant kernel (and a plain copy of known size) for "
+                         "hardware-counter passes")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the solve runs whole-domain on one MI355X "
@@ -109,6 +160,10 @@ def main():
     f_dev = torch.tensor(p.force, device="cuda")
     u_dev = torch.empty_like(f_dev)
     tol = 1e-6
+
+    if args.kernel_only:
+        kernel_only(p, E_dev, act_dev, MGSolver)
+        return
 
     def timed(fn):
         torch.cuda.synchronize()
@@ -199,6 +254,9 @@ def main():
     gbs = bytes_sweep / (ms_jac32 * 1e-3) / 1e9
     opt.close()
 
+    copy_gbs = copy_bandwidth()
+    log(f"plain device copy: {copy_gbs:.0f} GB/s")
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -230,9 +288,12 @@ def main():
                   "true_resid": opt.last_true_resid_ratio, "converged": opt.last_converged,
                   "rel_disp_diff_vs_fp64": rel_err},
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs / HBM_PEAK_GBS, "traffic": None,
+                     "frac": gbs / HBM_PEAK_GBS,
+                     "traffic": measured_traffic() if args.n == 256 else None,
+                     "traffic_source": "profiles/chebyshev_fp32_256_traffic.md",
                      "kernel": "k_tile<float, M_CHEB> (Chebyshev smoothing step), level 0",
-                     "kernel_ms": ms_jac32, "algorithmic_bytes": bytes_sweep},
+                     "kernel_ms": ms_jac32, "algorithmic_bytes": bytes_sweep,
+                     "copy_GBs": copy_gbs, "frac_of_copy": gbs / copy_gbs},
         "kernels": {
             "chebyshev_fp32": {"ms": ms_jac32, "GBs": gbs},
             "matvec_fp32": {"ms": ms_mv32, "GBs": bytes_mv32 / (ms_mv32 * 1e-3) / 1e9},
