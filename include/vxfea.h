@@ -64,7 +64,8 @@ typedef struct {
   int32_t z_chunk;            /* node planes per work-group in the tile kernels (0: auto) */
   int32_t e_coarsen;          /* coarse-level E: 0 arithmetic mean of the 8 children
                                  (core/multigrid.py:287-290), 1 geometric, 2 harmonic */
-  int32_t reserved[7];
+  int32_t max_batch;          /* right-hand sides per vx_solve_batch call (workspace size) */
+  int32_t reserved[6];
 } vx_options;
 
 /* Per-solve statistics (mirrors MGSolver.last_* of core/multigrid.py:482-517). */
@@ -119,6 +120,17 @@ int vx_solve(vx_handle h, const double* f_dev, double* u_dev, const double* x0_d
              double tol, int32_t max_iter, vx_progress_cb cb, void* user,
              double* hist_host, int32_t hist_cap, vx_info* info);
 
+/* Batched solve: nrhs (<= max_batch) load cases of the same grid, density and
+ * fixed DOFs in one pass -- every operator, smoother, transfer and reduction
+ * launch covers all load cases (vectors [nrhs][ndof], contiguous).  Each load
+ * case follows exactly the recurrence of vx_solve and stops at its own
+ * convergence (its updates are frozen from then on).  hist_host (may be NULL)
+ * is [nrhs][hist_cap]; infos (may be NULL) receives one vx_info per load case.
+ * The progress callback reports the largest ratio among the unconverged cases. */
+int vx_solve_batch(vx_handle h, int32_t nrhs, const double* F_dev, double* U_dev, double tol,
+                   int32_t max_iter, vx_progress_cb cb, void* user, double* hist_host,
+                   int32_t hist_cap, vx_info* infos);
+
 /* VoxelFEA._solve_jacobi (core/fea.py:497-593): Jacobi-preconditioned CG on
  * the active-DOF system (inactive elements dropped, no e_min, loads on nodes
  * of no active element ignored).  E_dev/active_dev in reference element
@@ -149,6 +161,19 @@ int vx_inv_diag(vx_handle h, int32_t level, double* minv);
  * element in reference order, 0 where active_dev is 0. */
 int vx_von_mises(vx_handle h, const double* u_dev, const double* E_dev,
                  const uint8_t* active_dev, double* vm_dev);
+
+/* Post-solve fields of the reference's solve worker (core/solve_worker.py:358-431)
+ * from u (full grid) and the per-element modulus E in Pa (reference order):
+ * stress3d = von Mises / vsize, safety3d = clip(yield / max(vm, eps), 0, 10) on
+ * active elements, disp3d = mean |u| of the 8 corner nodes (all float32, per
+ * element).  scalars5_host receives max_stress_pa, stress_vmax (99th percentile
+ * of active stress, >= 1e-6), max_displacement_m (all nodes),
+ * displacement_vmax_m (active elements, >= 1e-9), min_safety_factor (NaN
+ * when yield <= 0). */
+int vx_post_fields(vx_handle h, const double* u_dev, const double* E_dev,
+                   const uint8_t* active_dev, double vsize, double yield_pa,
+                   float* stress3d_dev, float* safety3d_dev, float* disp3d_dev,
+                   double* scalars5_host);
 
 /* VoxelFEA.element_strain_energy (core/fea.py:595-602): u_e^T KE u_e with
  * unit E for active elements, 0 elsewhere; reference element order. */

@@ -175,3 +175,71 @@ def test_islands_vs_oracle(rng):
     act[2:4, 2:4, 3:5] = True
     k, n = drop_unsupported_islands(act, np.arange(3 * 49))
     assert n == 0 and np.array_equal(k, act)
+
+
+@pytest.mark.parametrize("name", ["part16", "voids16", "cantilever"])
+def test_post_fields_vs_oracle(name):
+    """Result fields of core/solve_worker.py:358-431 from the GPU u."""
+    print("case post fields", name, flush=True)
+    from core.postsolve import final_fields
+    from oracle.postsolve_ref import post_fields
+    g = load_golden(name)
+    vf = _fea(g)
+    u = vf.solve(g["E_pa"] * float(g["vsize"]), g["force"], tol=float(g["tol"]))
+    got = final_fields(vf, u, g["E_pa"], float(g["vsize"]), 250e6, vf.last_converged)
+    ref = post_fields(int(g["nx"]), int(g["ny"]), int(g["nz"]), float(g["nu"]), float(g["vsize"]),
+                      g["u"], g["E_pa"], g["active"], 250e6, bool(g["last_converged"]))
+    for k in ("stress_vmax", "max_stress_pa", "max_displacement_m", "displacement_vmax_m",
+              "min_safety_factor"):
+        assert abs(got[k] - ref[k]) <= 1e-6 * abs(ref[k]), k
+    for k in ("stress3d", "safety3d", "displacement3d"):
+        assert np.abs(got[k] - ref[k]).max() <= 1e-5 * np.abs(ref[k]).max(), k
+    assert got["reliable"] == ref["reliable"]
+    vf.close()
+
+
+def _batch_problem(n=16, nrhs=4):
+    from core import workloads as W
+    return W.multi_rhs_part(n=n, nrhs=nrhs, seed=13)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batched_solve_equals_single_solves(precision):
+    """Batched load cases (one launch per operation for all cases) give each
+    case exactly its single solve: bitwise in FP64, and in mixed precision
+    within 1e-5 of the FP64 single solve with the true residual below tol."""
+    print("case batch", precision, flush=True)
+    import torch
+    from core.multigrid import MGSolver
+    p = _batch_problem()
+    nrhs = p.force.shape[0]
+    E = torch.tensor(p.E_stiffness, device="cuda")
+    A = torch.tensor(p.active.astype(np.uint8), device="cuda")
+    kw = dict(precision=precision, smoother="chebyshev", cheb_ratio=0.1, system="active")
+    single = MGSolver(p.nx, p.ny, p.nz, p.fixed_dofs, nu=p.nu, compute_mode="GPU",
+                      **dict(kw, precision="fp64"))
+    single.set_density(E, active=A)
+    ref = []
+    for q in range(nrhs):
+        f = torch.tensor(p.force[q], device="cuda")
+        u = torch.empty_like(f)
+        single.solve_device(f, u, tol=1e-8)
+        ref.append((u.cpu().numpy(), single.last_iters))
+    batch = MGSolver(p.nx, p.ny, p.nz, p.fixed_dofs, nu=p.nu, compute_mode="GPU",
+                     max_batch=nrhs, **kw)
+    batch.set_density(E, active=A)
+    F = torch.tensor(p.force, device="cuda")
+    U = torch.empty_like(F)
+    batch.solve_batch_device(F, U, tol=1e-8)
+    Ub = U.cpu().numpy()
+    for q in range(nrhs):
+        info = batch.last_batch[q]
+        assert info["converged"]
+        if precision == "fp64":
+            assert info["iters"] == ref[q][1]
+            assert np.array_equal(Ub[q], ref[q][0])
+        else:
+            assert info["true_resid_ratio"] < 1e-8
+            assert _rel(Ub[q], ref[q][0]) < 1e-5
+    single.close()
+    batch.close()

@@ -50,7 +50,8 @@ class vx_options(ctypes.Structure):
         ("max_outer", ctypes.c_int32),
         ("z_chunk", ctypes.c_int32),
         ("e_coarsen", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 7),
+        ("max_batch", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 6),
     ]
 
 
@@ -90,6 +91,8 @@ SIGNATURES = {
     "vx_set_density": (ctypes.c_int, [_P, _P, _P, _D]),
     "vx_solve": (ctypes.c_int, [_P, _P, _P, _P, _D, _I32, PROGRESS_CB, _P, _P, _I32,
                                 ctypes.POINTER(vx_info)]),
+    "vx_solve_batch": (ctypes.c_int, [_P, _I32, _P, _P, _D, _I32, PROGRESS_CB, _P, _P, _I32,
+                                      ctypes.POINTER(vx_info)]),
     "vx_solve_jacobi": (ctypes.c_int, [_P, _P, _P, _P, _P, _D, _I32, PROGRESS_CB, _P, _P,
                                        _I32, ctypes.POINTER(vx_info)]),
     "vx_matvec": (ctypes.c_int, [_P, _I32, _I32, _P, _P]),
@@ -100,6 +103,7 @@ SIGNATURES = {
     "vx_inv_diag": (ctypes.c_int, [_P, _I32, _P]),
     "vx_von_mises": (ctypes.c_int, [_P, _P, _P, _P, _P]),
     "vx_strain_energy": (ctypes.c_int, [_P, _P, _P, _P]),
+    "vx_post_fields": (ctypes.c_int, [_P, _P, _P, _P, _D, _D, _P, _P, _P, _P]),
     "vx_drop_islands": (ctypes.c_int, [_P, _P, _P, _I64, _P, ctypes.POINTER(_I64)]),
     "vx_time_kernel": (ctypes.c_int, [_P, _I32, _I32, _I32, _I32, ctypes.POINTER(_D)]),
     "vx_element_apply_host": (None, [_D, _P, _P]),

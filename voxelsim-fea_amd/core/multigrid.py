@@ -53,7 +53,7 @@ class MGSolver:
                  min_elems=4, max_levels=6, compute_mode="AUTO", cpu_threads=0,
                  verbose=False, precision="fp64", coarse_sweeps=30, inner_tol=1e-5,
                  max_outer=20, z_chunk=0, smoother="jacobi", cheb_ratio=0.3,
-                 system="full", e_coarsen="mean"):
+                 system="full", e_coarsen="mean", max_batch=1):
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
         self.nelem = self.nx * self.ny * self.nz
         self.ndof = 3 * (self.nx + 1) * (self.ny + 1) * (self.nz + 1)
@@ -72,7 +72,9 @@ class MGSolver:
             coarse_sweeps=coarse_sweeps, precision=_PRECISION[precision],
             inner_tol=inner_tol, max_outer=max_outer, z_chunk=z_chunk,
             smoother=_SMOOTHER[smoother], cheb_ratio=cheb_ratio,
-            e_coarsen={"mean": 0, "geometric": 1, "harmonic": 2}[e_coarsen])
+            e_coarsen={"mean": 0, "geometric": 1, "harmonic": 2}[e_coarsen],
+            max_batch=max_batch)
+        self.max_batch = max_batch
         h = ctypes.c_void_p()
         _native.check(self._lib.vx_create(self.nx, self.ny, self.nz, float(nu),
                                           ctypes.byref(opt), ctypes.byref(h)))
@@ -157,6 +159,33 @@ class MGSolver:
         self.last_outer_iters = info.outer_iters
         self.last_solve_ms = info.solve_ms
         self.resid_history = [float(v) for v in hist[:min(info.iters, hist_cap)]]
+        self._check(rc, allow=(_native.VX_ERR_NOT_CONVERGED,))
+        return rc
+
+    def solve_batch_device(self, F_dev, U_dev, tol=1e-4, max_cg=None, progress_cb=None,
+                           hist_cap=2000):
+        """All load cases of F_dev [nrhs, ndof] (nrhs <= max_batch) in one
+        batched solve on the current density; U_dev [nrhs, ndof] receives the
+        displacements.  Returns the native status; per-case statistics go to
+        ``last_batch`` (list of dicts) and ``batch_history``."""
+        nrhs = int(F_dev.shape[0])
+        self._stream_bind()
+        hist = np.zeros(nrhs * hist_cap)
+        infos = (_native.vx_info * nrhs)()
+
+        def _cb(it, ratio, t, _user):
+            progress_cb(int(it), float(ratio), float(t))
+
+        cb = _native.PROGRESS_CB(_cb) if progress_cb is not None else _native.PROGRESS_CB()
+        rc = self._lib.vx_solve_batch(self._h, nrhs, _native.ptr(F_dev), _native.ptr(U_dev),
+                                      float(tol), int(max_cg or 0), cb, None,
+                                      hist.ctypes.data_as(ctypes.c_void_p), hist_cap, infos)
+        self.last_batch = [{"iters": i.iters, "converged": bool(i.converged),
+                            "outer_iters": i.outer_iters, "resid_ratio": i.resid_ratio,
+                            "true_resid_ratio": i.true_resid_ratio} for i in infos]
+        self.batch_history = [list(hist[q * hist_cap:q * hist_cap + min(infos[q].iters, hist_cap)])
+                              for q in range(nrhs)]
+        self.last_solve_ms = infos[0].solve_ms if nrhs else float("nan")
         self._check(rc, allow=(_native.VX_ERR_NOT_CONVERGED,))
         return rc
 

@@ -27,16 +27,34 @@ constexpr int PRE = (FOOTVALS + NT - 1) / NT;
 
 enum { M_MATVEC = 0, M_RESID = 1, M_JACOBI = 2, M_CHEB = 3 };
 
+// PCG scalars of one right-hand side, in device memory.
+struct Scalars {
+  double rz;       // r.z of the current direction
+  double pAp;
+  double alpha;
+  double rr;       // ||r||^2
+  double ratio;    // sqrt(rr)/fnorm
+  double rz_new;
+  double beta;
+  double fnorm;
+  double tmp;
+  double tmp2;
+  int frozen;      // 1: this right-hand side has stopped (alpha = 0)
+  int pad;
+};
+
 template <typename T>
 struct TileArgs {
   int nx, ny, nz;        // elements
   int Lx, Ly, Lz;        // nodes
   int zc;                // owned node planes per work-group
   int gx, gy, gz;        // tile grid (the launch is 1D, XCD-ordered)
+  int nb;                // right-hand sides in the launch (vectors [nb][vstride])
+  long long vstride;     // elements between consecutive right-hand sides
   const T* __restrict__ u;
   const T* __restrict__ pold;   // PUPD: previous direction; the kernel applies
   T* __restrict__ pnew;         //   K to p = u + beta * pold and stores p
-  const double* __restrict__ beta;
+  const Scalars* __restrict__ scal;   // PUPD: beta of each right-hand side
   const T* __restrict__ E;
   const uint8_t* __restrict__ mask;
   const T* __restrict__ b;
@@ -120,13 +138,23 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs<T> a) {
   __shared__ T sQ[2][4][TILE][TILE + 1];
   __shared__ double sred[NT / 64];
 
-  // XCD-aware tile numbering
+  // XCD-aware tile numbering over (right-hand side, tile)
   const int G = a.gx * a.gy * a.gz;
-  const int per = (G + 7) >> 3;
+  const int GT = G * a.nb;
+  const int per = (GT + 7) >> 3;
   const int v = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
-  if (v >= G) return;
-  const int bx = v % a.gx;
-  const int byz = v / a.gx;
+  if (v >= GT) return;
+  const int rhs = v / G;
+  const int vt = v - rhs * G;
+  const long long voff = (long long)rhs * a.vstride;
+  const T* __restrict__ au = a.u + voff;
+  const T* __restrict__ apold = PUPD ? a.pold + voff : nullptr;
+  T* __restrict__ apnew = PUPD ? a.pnew + voff : nullptr;
+  const T* __restrict__ ab = a.b ? a.b + voff : nullptr;
+  const T* __restrict__ aup = a.up ? a.up + voff : nullptr;
+  T* __restrict__ aout = a.out + voff;
+  const int bx = vt % a.gx;
+  const int byz = vt / a.gx;
   const int by = byz % a.gy;
   const int bz = byz / a.gy;
 
@@ -162,7 +190,7 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs<T> a) {
       if (x >= 0 && x < a.Lx && y >= 0 && y < a.Ly) goff[k] = (y * a.Lx + x) * 3 + c;
     }
   }
-  const T beta = PUPD ? (T)(*a.beta) : T(0);
+  const T beta = PUPD ? (T)a.scal[rhs].beta : T(0);
 
   T pre[PRE];
   auto fetch = [&](int pz) {
@@ -172,8 +200,8 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs<T> a) {
     for (int k = 0; k < PRE; ++k) {
       T val = T(0);
       if (zin && goff[k] >= 0) {
-        val = a.u[base + goff[k]];
-        if (PUPD) val += beta * a.pold[base + goff[k]];
+        val = au[base + goff[k]];
+        if (PUPD) val += beta * apold[base + goff[k]];
       }
       pre[k] = val;
     }
@@ -219,7 +247,7 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs<T> a) {
   // the next output plane are loaded one step ahead
   constexpr bool NEEDB = MODE != M_MATVEC;
   constexpr bool NEEDUP = MODE == M_CHEB;
-  const bool has_up = NEEDUP && a.up != nullptr;
+  const bool has_up = NEEDUP && aup != nullptr;
   auto loadE = [&](int iz) -> T {
     return (col_in && iz >= 0 && iz < a.nz) ? a.E[iz * nxy + e0] : T(0);
   };
@@ -231,8 +259,8 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs<T> a) {
       mk_n = a.mask[n];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        if (NEEDB) b_n[c] = a.b[3 * n + c];
-        if (NEEDUP) up_n[c] = has_up ? a.up[3 * n + c] : T(0);
+        if (NEEDB) b_n[c] = ab[3 * n + c];
+        if (NEEDUP) up_n[c] = has_up ? aup[3 * n + c] : T(0);
       }
     }
   };
@@ -294,20 +322,20 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs<T> a) {
       for (int c = 0; c < 3; ++c) y[c] = A0[c] + sQ[q][c][ty - 1][tx];
       if (PUPD) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) a.pnew[3 * n + c] = ub[c];
+        for (int c = 0; c < 3; ++c) apnew[3 * n + c] = ub[c];
       }
       if (MODE == M_MATVEC) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           const T o = ((mk >> c) & 1u) ? y[c] : T(0);
-          a.out[3 * n + c] = o;
+          aout[3 * n + c] = o;
           if (DOT) dacc += (double)o * (double)ub[c];
         }
       } else if (MODE == M_RESID) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           const T o = bb[c] - (((mk >> c) & 1u) ? y[c] : T(0));
-          a.out[3 * n + c] = o;
+          aout[3 * n + c] = o;
           if (DOT) dacc += (double)o * (double)o;
         }
       } else {
@@ -322,7 +350,7 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs<T> a) {
             const T du = ub[c] - upv[c];
             o = ub[c] + (((mk >> c) & 1u) ? a.cu * du + a.cr * (minv * (bb[c] - y[c])) : T(0));
           }
-          a.out[3 * n + c] = o;
+          aout[3 * n + c] = o;
           if (DOT) dacc += (double)o * (double)bb[c];
         }
       }
@@ -403,6 +431,8 @@ __global__ __launch_bounds__(1024) void k_coarse_smooth(int nx, int ny, int nz,
   const int Nn = Lx * Ly * (nz + 1);
   const int Ne = nx * ny * nz;
   const int tid = threadIdx.x, nt = blockDim.x;
+  b += (long long)blockIdx.x * 3 * Nn;
+  out += (long long)blockIdx.x * 3 * Nn;
   for (int n = tid; n < Nn; n += nt) {
     smk[n] = mask[n];
     smi[n] = minv[n];
@@ -472,6 +502,8 @@ template <typename T>
 __global__ void k_zero_step(long long Nn, const uint8_t* __restrict__ mask,
                             const T* __restrict__ minv, const T* __restrict__ b,
                             T* __restrict__ out, T c) {
+  b += blockIdx.y * 3 * Nn;
+  out += blockIdx.y * 3 * Nn;
   for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < Nn;
        n += (long long)gridDim.x * blockDim.x) {
     const unsigned mk = mask[n];
@@ -488,9 +520,12 @@ template <typename T>
 __global__ void k_restrict3(int Lcx, int Lcy, int Lcz, const T* __restrict__ rf,
                             const uint8_t* __restrict__ cmask, T* __restrict__ rc) {
   const int jx = blockIdx.x * blockDim.x + threadIdx.x;
-  const int jy = blockIdx.y, jz = blockIdx.z;
+  const int rhs = blockIdx.z / Lcz;
+  const int jy = blockIdx.y, jz = blockIdx.z - rhs * Lcz;
   if (jx >= Lcx) return;
   const int Lfx = 2 * Lcx - 1, Lfy = 2 * Lcy - 1, Lfz = 2 * Lcz - 1;
+  rf += (long long)rhs * 3 * Lfx * Lfy * Lfz;
+  rc += (long long)rhs * 3 * Lcx * Lcy * Lcz;
   T s0 = T(0), s1 = T(0), s2 = T(0);
 #pragma unroll
   for (int dz = -1; dz <= 1; ++dz) {
@@ -527,9 +562,12 @@ template <typename T>
 __global__ void k_prolong_add3(int Lfx, int Lfy, int Lfz, const T* __restrict__ ec,
                                const uint8_t* __restrict__ fmask, T* __restrict__ x) {
   const int ix = blockIdx.x * blockDim.x + threadIdx.x;
-  const int iy = blockIdx.y, iz = blockIdx.z;
+  const int rhs = blockIdx.z / Lfz;
+  const int iy = blockIdx.y, iz = blockIdx.z - rhs * Lfz;
   if (ix >= Lfx) return;
-  const int Lcx = (Lfx + 1) / 2, Lcy = (Lfy + 1) / 2;
+  const int Lcx = (Lfx + 1) / 2, Lcy = (Lfy + 1) / 2, Lcz = (Lfz + 1) / 2;
+  ec += (long long)rhs * 3 * Lcx * Lcy * Lcz;
+  x += (long long)rhs * 3 * Lfx * Lfy * Lfz;
   const int cx = ix >> 1, cy = iy >> 1, cz = iz >> 1;
   const int ox = ix & 1, oy = iy & 1, oz = iz & 1;
   const T wx = ox ? T(0.5) : T(1), wy = oy ? T(0.5) : T(1), wz = oz ? T(0.5) : T(1);
@@ -667,18 +705,7 @@ __global__ void k_inject_mask(int Lcx, int Lcy, int Lcz, const uint8_t* __restri
 // PCG vector kernels.  Each writes one partial per work-group, in a fixed
 // grid, so the reductions are deterministic.
 // ---------------------------------------------------------------------------
-struct Scalars {
-  double rz;       // r.z of the current direction
-  double pAp;
-  double alpha;
-  double rr;       // ||r||^2
-  double ratio;    // sqrt(rr)/fnorm
-  double rz_new;
-  double beta;
-  double fnorm;
-  double tmp;
-  double tmp2;
-};
+
 
 enum {
   F_SET_TMP = 0,    // s.tmp = sum
@@ -691,6 +718,8 @@ enum {
 
 __global__ void k_finalize(const double* __restrict__ partial, int n, int op, Scalars* s) {
   __shared__ double sred[4];
+  partial += (long long)blockIdx.x * n;
+  s += blockIdx.x;
   double v = 0.0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) v += partial[i];
   v = block_sum(v, sred);
@@ -701,10 +730,11 @@ __global__ void k_finalize(const double* __restrict__ partial, int n, int op, Sc
       case F_RZ0:
         s->rz = v;
         s->beta = 0.0;
+        s->frozen = 0;
         break;
       case F_ALPHA:
         s->pAp = v;
-        s->alpha = (v == 0.0) ? 0.0 : s->rz / v;
+        s->alpha = (v == 0.0 || s->frozen) ? 0.0 : s->rz / v;
         break;
       case F_RATIO:
         s->rr = v;
@@ -719,7 +749,12 @@ __global__ void k_finalize(const double* __restrict__ partial, int n, int op, Sc
   }
 }
 
-__global__ void k_set_fnorm(Scalars* s, double v) { s->fnorm = v; }
+__global__ void k_set_fnorm(Scalars* s, double v) { s[blockIdx.x].fnorm = v; }
+
+// frozen flags of all right-hand sides from a device int array
+__global__ void k_set_frozen(Scalars* s, const int* __restrict__ fr, int nb) {
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) s[i].frozen = fr[i];
+}
 
 // 4 consecutive values through one (float4) or two (double2) vector accesses
 __device__ __forceinline__ void ld4(const float* p, float v[4]) {
@@ -745,7 +780,12 @@ __global__ void k_cg_update(long long n, const Scalars* __restrict__ s, T* __res
                             T* __restrict__ r, const T* __restrict__ p, const T* __restrict__ Ap,
                             double* __restrict__ partial) {
   __shared__ double sred[4];
-  const T a = (T)s->alpha;
+  const long long off = (long long)blockIdx.y * n;
+  u += off;
+  r += off;
+  p += off;
+  Ap += off;
+  const T a = (T)s[blockIdx.y].alpha;
   double acc = 0.0;
   const long long n4 = n >> 2;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n4;
@@ -772,7 +812,7 @@ __global__ void k_cg_update(long long n, const Scalars* __restrict__ s, T* __res
     acc += (double)rn * (double)rn;
   }
   acc = block_sum(acc, sred);
-  if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+  if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = acc;
 }
 
 // p = z + beta p
@@ -787,9 +827,11 @@ __global__ void k_p_update(long long n, const Scalars* __restrict__ s, const T* 
 
 // out = mask * in (per DOF), partial sum out.out
 template <typename Tin, typename Tout>
-__global__ void k_mask_copy(long long nn, const uint8_t* __restrict__ mask, const Tin* __restrict__ in,
-                            Tout* __restrict__ out, double* __restrict__ partial) {
+__global__ void k_mask_copy(long long nn, const uint8_t* __restrict__ mask, const Tin* in,
+                            Tout* out, double* __restrict__ partial) {
   __shared__ double sred[4];
+  in += blockIdx.y * 3 * nn;
+  out += blockIdx.y * 3 * nn;
   double acc = 0.0;
   for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < nn;
        n += (long long)gridDim.x * blockDim.x) {
@@ -803,7 +845,7 @@ __global__ void k_mask_copy(long long nn, const uint8_t* __restrict__ mask, cons
   }
   if (partial) {
     acc = block_sum(acc, sred);
-    if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+    if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = acc;
   }
 }
 
@@ -812,18 +854,24 @@ template <typename T>
 __global__ void k_dot(long long n, const T* __restrict__ a, const T* __restrict__ b,
                       double* __restrict__ partial) {
   __shared__ double sred[4];
+  a += blockIdx.y * n;
+  b += blockIdx.y * n;
   double acc = 0.0;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
     acc += (double)a[i] * (double)b[i];
   acc = block_sum(acc, sred);
-  if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+  if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = acc;
 }
 
 // r32 = (float)(r64 * scale) with scale = 1/||r64|| from s->tmp = ||r64||^2
 __global__ void k_scale_to_f32(long long n, const Scalars* __restrict__ s,
                                const double* __restrict__ r64, float* __restrict__ r32) {
-  const double sc = 1.0 / sqrt(s->tmp);
+  // tmp2 != 0 marks a right-hand side whose refinement is done: zero input
+  const double nr = sqrt(s[blockIdx.y].tmp);
+  const double sc = (nr > 0.0 && s[blockIdx.y].tmp2 == 0.0) ? 1.0 / nr : 0.0;
+  r64 += blockIdx.y * n;
+  r32 += blockIdx.y * n;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
     r32[i] = (float)(r64[i] * sc);
@@ -832,7 +880,9 @@ __global__ void k_scale_to_f32(long long n, const Scalars* __restrict__ s,
 // u64 += d32 * ||r64|| (undo the inner scaling)
 __global__ void k_add_scaled(long long n, const Scalars* __restrict__ s,
                              const float* __restrict__ d32, double* __restrict__ u64) {
-  const double sc = sqrt(s->tmp);
+  const double sc = sqrt(s[blockIdx.y].tmp);
+  d32 += blockIdx.y * n;
+  u64 += blockIdx.y * n;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
     u64[i] += (double)d32[i] * sc;

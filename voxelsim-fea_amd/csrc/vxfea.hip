@@ -83,8 +83,11 @@ struct vx_solver {
   double* jac_minv = nullptr; // per node
   double* partial = nullptr;
   int n_partial = 0;
-  Scalars* dscal = nullptr;
-  Scalars* hscal = nullptr;   // pinned
+  int nb = 1;                 // right-hand sides of the current operation
+  Scalars* dscal = nullptr;   // [max_batch]
+  Scalars* hscal = nullptr;   // pinned mirror
+  int* dflags = nullptr;      // [max_batch] frozen flags staging
+  int* hflags = nullptr;      // pinned
   std::vector<void*> allocs;
 };
 
@@ -161,8 +164,10 @@ struct SmoothCoef {
 struct PUpd {
   const void* pold = nullptr;
   void* pnew = nullptr;
-  const double* beta = nullptr;
 };
+
+// Every launcher below covers the s->nb right-hand sides of the current
+// operation (vectors [nb][3 Nn] of the level, contiguous).
 
 template <typename T, int MODE, bool DOT, bool PUPD = false>
 int launch_tile_L(vx_solver* s, Level& L, const T* u, const T* b, T* out, double* partial,
@@ -171,10 +176,12 @@ int launch_tile_L(vx_solver* s, Level& L, const T* u, const T* b, T* out, double
   a.nx = L.nx; a.ny = L.ny; a.nz = L.nz;
   a.Lx = L.Lx; a.Ly = L.Ly; a.Lz = L.Lz;
   a.zc = zchunk(s, L);
+  a.nb = s->nb;
+  a.vstride = 3 * L.Nn;
   a.u = u; a.E = Sel<T>::E(L); a.mask = L.mask; a.b = b; a.out = out;
   a.pold = pu ? static_cast<const T*>(pu->pold) : nullptr;
   a.pnew = pu ? static_cast<T*>(pu->pnew) : nullptr;
-  a.beta = pu ? pu->beta : nullptr;
+  a.scal = s->dscal;
   a.omega = (T)(sc ? sc->omega : s->opt.omega);
   a.up = sc ? static_cast<const T*>(sc->up) : nullptr;
   a.cu = (T)(sc ? sc->cu : 0.0);
@@ -186,8 +193,8 @@ int launch_tile_L(vx_solver* s, Level& L, const T* u, const T* b, T* out, double
   a.gx = (int)g.x; a.gy = (int)g.y; a.gz = (int)g.z;
   const long long G = (long long)g.x * g.y * g.z;
   if (n_part) *n_part = (int)G;
-  if (DOT && G > s->n_partial) return fail(s, VX_ERR_ARG, "partial buffer too small");
-  const long long launch = ((G + 7) / 8) * 8;   // XCD-ordered 1D grid
+  if (DOT && G * s->nb > s->n_partial) return fail(s, VX_ERR_ARG, "partial buffer too small");
+  const long long launch = ((G * s->nb + 7) / 8) * 8;   // XCD-ordered 1D grid
   hipLaunchKernelGGL((k_tile<T, MODE, DOT, PUPD>), dim3((unsigned)launch), dim3(NT), 0, s->stream,
                      a);
   LAUNCHCHK();
@@ -200,9 +207,21 @@ int launch_tile(vx_solver* s, int li, const T* u, const T* b, T* out, double* pa
   return launch_tile_L<T, MODE, DOT>(s, s->lv[li], u, b, out, partial, n_part, sc);
 }
 
+// per right-hand side: sum its n partials and apply op to its scalars
 int finalize(vx_solver* s, int n, int op) {
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s->stream, s->partial, n, op, s->dscal);
+  hipLaunchKernelGGL(k_finalize, dim3(s->nb), dim3(256), 0, s->stream, s->partial, n, op,
+                     s->dscal);
   LAUNCHCHK();
+  return VX_OK;
+}
+
+// partials of a.b per right-hand side over vectors of length n each
+template <typename T>
+int dot_partials(vx_solver* s, long long n, const T* a, const T* b, int* n_part) {
+  hipLaunchKernelGGL(k_dot<T>, dim3(EW_BLOCKS, s->nb), dim3(EW_THREADS), 0, s->stream, n, a, b,
+                     s->partial);
+  LAUNCHCHK();
+  if (n_part) *n_part = EW_BLOCKS;
   return VX_OK;
 }
 
@@ -212,15 +231,10 @@ int sweep(vx_solver* s, int li, const T* u, const T* b, T* out, bool dot, int* n
           double omega) {
   Level& L = s->lv[li];
   if (!u) {
-    hipLaunchKernelGGL(k_zero_step<T>, dim3(ew_grid(L.Nn)), dim3(EW_THREADS), 0, s->stream,
+    hipLaunchKernelGGL(k_zero_step<T>, dim3(ew_grid(L.Nn), s->nb), dim3(EW_THREADS), 0, s->stream,
                        L.Nn, L.mask, Sel<T>::mi(L), b, out, (T)omega);
     LAUNCHCHK();
-    if (dot) {
-      hipLaunchKernelGGL(k_dot<T>, dim3(EW_BLOCKS), dim3(EW_THREADS), 0, s->stream, 3 * L.Nn,
-                         out, b, s->partial);
-      LAUNCHCHK();
-      if (n_part) *n_part = EW_BLOCKS;
-    }
+    if (dot) CHK(dot_partials<T>(s, 3 * L.Nn, out, b, n_part));
     return VX_OK;
   }
   SmoothCoef sc;
@@ -239,11 +253,10 @@ int cheb_n(vx_solver* s, int li, const T* u_in, const T* b, int k, T* dst, T* tm
   const double beta = s->lam_max, alpha = s->opt.cheb_ratio * beta;
   const double theta = 0.5 * (beta + alpha), delta = 0.5 * (beta - alpha);
   const double sigma = theta / delta;
-  // assign targets so that write j (0-based) goes to bufs[(k-1-j) % 3]'s rotation ending at dst
   T* ring[3] = {dst, tmp, tmp2};
   auto target = [&](int j) { return ring[((k - 1 - j) % 3 + 3) % 3]; };
-  const T* xprev = u_in;   // x_{j-1}
-  const T* xcur = u_in;    // x_j
+  const T* xprev = u_in;
+  const T* xcur = u_in;
   double rho_prev = 1.0 / sigma, rho = rho_prev;
   for (int j = 0; j < k; ++j) {
     T* o = target(j);
@@ -272,13 +285,12 @@ int cheb_n(vx_solver* s, int li, const T* u_in, const T* b, int k, T* dst, T* tm
   return VX_OK;
 }
 
-// k sweeps starting at u_in (nullptr = zero) writing the result to `dst`,
-// using `tmp` as the other ping-pong buffer.  dot: last sweep adds partials
-// of out.b.
+// k damped-Jacobi sweeps starting at u_in (nullptr = zero) writing the result
+// to `dst`, using `tmp` as the other ping-pong buffer.  dot: last sweep adds
+// partials of out.b.
 template <typename T>
 int smooth_n(vx_solver* s, int li, const T* u_in, const T* b, int k, T* dst, T* tmp, bool dot,
              int* n_part) {
-  // choose the first target so that the k-th write lands in dst
   T* bufs[2] = {dst, tmp};
   int cur = (k % 2 == 1) ? 0 : 1;   // first write goes to bufs[cur]
   const T* src = u_in;
@@ -293,10 +305,17 @@ int smooth_n(vx_solver* s, int li, const T* u_in, const T* b, int k, T* dst, T* 
 }
 
 template <typename T>
+int copy_vec(vx_solver* s, T* dst, const T* src, long long n) {
+  HIPCHK(hipMemcpyAsync(dst, src, sizeof(T) * n, hipMemcpyDeviceToDevice, s->stream));
+  return VX_OK;
+}
+
+// level work vectors for max_batch right-hand sides
+template <typename T>
 int alloc_level_vectors(vx_solver* s) {
   for (size_t li = 0; li < s->lv.size(); ++li) {
     Level& L = s->lv[li];
-    const size_t n = 3 * (size_t)L.Nn;
+    const size_t n = 3 * (size_t)L.Nn * s->opt.max_batch;
     if (li > 0) CHK(dalloc(s, &Sel<T>::b(L), n));
     CHK(dalloc(s, &Sel<T>::x(L), n));
     CHK(dalloc(s, &Sel<T>::t(L), n));
@@ -309,13 +328,12 @@ int alloc_level_vectors(vx_solver* s) {
 template <typename T>
 int smooth_step(vx_solver* s, int li, const T* u_in, const T* b, int k, T* dst, T* tmp, T* tmp2,
                 bool dot, int* n_part) {
+  const long long n = 3 * s->lv[li].Nn * s->nb;
   if (s->opt.smoother == VX_SMOOTH_CHEBYSHEV) {
     if (u_in == dst && k % 3 == 1) {
       // the first write would land on the start vector: go through tmp
       CHK(cheb_n<T>(s, li, u_in, b, k, tmp, dst, tmp2, dot, n_part));
-      HIPCHK(hipMemcpyAsync(dst, tmp, sizeof(T) * 3 * s->lv[li].Nn, hipMemcpyDeviceToDevice,
-                            s->stream));
-      return VX_OK;
+      return copy_vec<T>(s, dst, tmp, n);
     }
     return cheb_n<T>(s, li, u_in, b, k, dst, tmp, tmp2, dot, n_part);
   }
@@ -323,15 +341,13 @@ int smooth_step(vx_solver* s, int li, const T* u_in, const T* b, int k, T* dst, 
     // Jacobi from dst: ping-pong through tmp, copy back for odd counts
     if (k % 2 == 0) return smooth_n<T>(s, li, u_in, b, k, dst, tmp, dot, n_part);
     CHK(smooth_n<T>(s, li, u_in, b, k, tmp, dst, dot, n_part));
-    HIPCHK(hipMemcpyAsync(dst, tmp, sizeof(T) * 3 * s->lv[li].Nn, hipMemcpyDeviceToDevice,
-                          s->stream));
-    return VX_OK;
+    return copy_vec<T>(s, dst, tmp, n);
   }
   return smooth_n<T>(s, li, u_in, b, k, dst, tmp, dot, n_part);
 }
 
-// All coarsest-level smoothing steps in one single-work-group launch
-// (same step sequence as smooth_step from zero).
+// All coarsest-level smoothing steps in one single-work-group launch per
+// right-hand side (same step sequence as smooth_step from zero).
 template <typename T>
 int coarse_fused(vx_solver* s, Level& L, const T* b, T* x) {
   CoarseCoef cc{};
@@ -354,7 +370,7 @@ int coarse_fused(vx_solver* s, Level& L, const T* b, T* x) {
     cc.cheb = 0;
     cc.w0 = s->opt.omega;
   }
-  hipLaunchKernelGGL(k_coarse_smooth<T>, dim3(1), dim3(1024), 0, s->stream, L.nx, L.ny, L.nz,
+  hipLaunchKernelGGL(k_coarse_smooth<T>, dim3(s->nb), dim3(1024), 0, s->stream, L.nx, L.ny, L.nz,
                      Sel<T>::E(L), L.mask, Sel<T>::mi(L), b, x, kconst<T>(s), cc);
   LAUNCHCHK();
   return VX_OK;
@@ -367,6 +383,7 @@ template <typename T>
 int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, T* w0, bool dot, int* n_part) {
   const int nl = (int)s->lv.size();
   const int ns = s->opt.n_smooth;
+  const int nb = s->nb;
   // descend
   for (int li = 0; li < nl; ++li) {
     Level& L = s->lv[li];
@@ -388,7 +405,7 @@ int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, T* w0, bool dot, int* n_part
     // residual into t, then restrict into b of level li+1
     CHK((launch_tile<T, M_RESID, false>(s, li, x, b, t, nullptr, nullptr)));
     Level& C = s->lv[li + 1];
-    hipLaunchKernelGGL(k_restrict3<T>, dim3((C.Lx + 63) / 64, C.Ly, C.Lz), dim3(64), 0,
+    hipLaunchKernelGGL(k_restrict3<T>, dim3((C.Lx + 63) / 64, C.Ly, C.Lz * nb), dim3(64), 0,
                        s->stream, C.Lx, C.Ly, C.Lz, t, C.mask, Sel<T>::b(C));
     LAUNCHCHK();
   }
@@ -400,7 +417,7 @@ int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, T* w0, bool dot, int* n_part
     T* x = li == 0 ? z0 : Sel<T>::x(L);
     T* t = li == 0 ? t0 : Sel<T>::t(L);
     T* w = li == 0 ? w0 : Sel<T>::w(L);
-    hipLaunchKernelGGL(k_prolong_add3<T>, dim3((L.Lx + 63) / 64, L.Ly, L.Lz), dim3(64), 0,
+    hipLaunchKernelGGL(k_prolong_add3<T>, dim3((L.Lx + 63) / 64, L.Ly, L.Lz * nb), dim3(64), 0,
                        s->stream, L.Lx, L.Ly, L.Lz, Sel<T>::x(C), L.mask, x);
     LAUNCHCHK();
     CHK(smooth_step<T>(s, li, x, b, ns, x, t, w, dot && li == 0, n_part));
@@ -414,9 +431,28 @@ int require_ready(vx_solver* s) {
   return VX_OK;
 }
 
+// copy the nb scalar structs to the pinned host mirror; event marks arrival
 int read_scalars(vx_solver* s, hipEvent_t ev) {
-  HIPCHK(hipMemcpyAsync(s->hscal, s->dscal, sizeof(Scalars), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipMemcpyAsync(s->hscal, s->dscal, sizeof(Scalars) * s->nb, hipMemcpyDeviceToHost,
+                        s->stream));
   HIPCHK(hipEventRecord(ev, s->stream));
+  return VX_OK;
+}
+
+int sync_scalars(vx_solver* s) {
+  HIPCHK(hipMemcpyAsync(s->hscal, s->dscal, sizeof(Scalars) * s->nb, hipMemcpyDeviceToHost,
+                        s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return VX_OK;
+}
+
+// push the host's frozen flags of all right-hand sides to the device
+int push_frozen(vx_solver* s, const std::vector<int>& fr) {
+  for (int r = 0; r < s->nb; ++r) s->hflags[r] = fr[r];
+  HIPCHK(hipMemcpyAsync(s->dflags, s->hflags, sizeof(int) * s->nb, hipMemcpyHostToDevice,
+                        s->stream));
+  hipLaunchKernelGGL(k_set_frozen, dim3(1), dim3(256), 0, s->stream, s->dscal, s->dflags, s->nb);
+  LAUNCHCHK();
   return VX_OK;
 }
 
@@ -427,38 +463,42 @@ struct PcgResult {
   double ratio = NAN;
 };
 
-// Preconditioned CG of core/multigrid.py:474-546 on level-0 vectors of type T.
-// On entry r holds the masked initial residual and u the initial iterate;
-// the device scalar fnorm is the ratio's denominator.  The loop stops when
-// ||r||/fnorm < tol_rel; the history and the callback see that ratio times
-// ratio_scale (1 for a plain solve; the outer residual ratio for an inner
-// solve of iterative refinement) and cb_tol.
+// Preconditioned CG of core/multigrid.py:474-546 on level-0 vectors of type T,
+// for the s->nb right-hand sides at once.  On entry r holds the masked
+// initial residuals and u the initial iterates; the device scalar fnorm of
+// each right-hand side is its ratio's denominator.  A right-hand side stops
+// (its updates frozen) when ||r||/fnorm < tol_rel, or on its stall guard; the
+// loop ends when all have stopped or max_iter is reached.  The history and
+// the callback see ratios times ratio_scale[r] (1 for a plain solve; the
+// outer residual ratio for an inner solve of iterative refinement).
 template <typename T>
 int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* p2, T* Ap, T* t0, double tol_rel, int max_iter,
-        bool stall_guard, double ratio_scale, double cb_tol, int iter_offset, vx_progress_cb cb,
-        void* user, double* hist, int hist_cap, PcgResult* res) {
+        bool stall_guard, const std::vector<double>& ratio_scale, double cb_tol,
+        const std::vector<int>& iter_offset, vx_progress_cb cb, void* user, double* hist,
+        int hist_cap, std::vector<PcgResult>& res, const std::vector<int>& skip) {
   Level& L0 = s->lv[0];
+  const int nb = s->nb;
   const long long n = 3 * L0.Nn;
   int np = 0;
   hipEvent_t ev;
   HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
   auto cleanup = [&]() { (void)hipEventDestroy(ev); };
 
+  res.assign(nb, PcgResult());
+  std::vector<int> frozen(skip.begin(), skip.end());
   int rc = vcycle<T>(s, r, z, t0, Sel<T>::w(L0), true, &np);
-  if (rc == VX_OK) rc = finalize(s, np, F_RZ0);   // also beta = 0
+  if (rc == VX_OK) rc = finalize(s, np, F_RZ0);   // also beta = 0, frozen = 0
+  if (rc == VX_OK && std::any_of(frozen.begin(), frozen.end(), [](int f) { return f; }))
+    rc = push_frozen(s, frozen);
   if (rc != VX_OK) { cleanup(); return rc; }
   // both direction buffers start at 0 so that p = z + 0 * p_prev is exact
-  if (hipMemsetAsync(p, 0, sizeof(T) * n, s->stream) != hipSuccess ||
-      hipMemsetAsync(p2, 0, sizeof(T) * n, s->stream) != hipSuccess) {
+  if (hipMemsetAsync(p, 0, sizeof(T) * n * nb, s->stream) != hipSuccess ||
+      hipMemsetAsync(p2, 0, sizeof(T) * n * nb, s->stream) != hipSuccess) {
     cleanup();
     return fail(s, VX_ERR_HIP, "memset p");
   }
   T* pb[2] = {p, p2};
-  double best = INFINITY;
-  double prev_ratio = NAN, prev2_ratio = NAN;
-  res->iters = 0;
-  res->converged = false;
-  res->ratio = NAN;
+  std::vector<double> best(nb, INFINITY), prev(nb, NAN), prev2(nb, NAN);
   for (int it = 0; it < max_iter; ++it) {
     T* pcur = pb[it & 1];
     T* pprev = pb[(it + 1) & 1];
@@ -466,24 +506,26 @@ int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* p2, T* Ap, T* t0, double tol_re
     PUpd pu;
     pu.pold = pprev;
     pu.pnew = pcur;
-    pu.beta = &s->dscal->beta;
     rc = launch_tile_L<T, M_MATVEC, true, true>(s, L0, z, nullptr, Ap, s->partial, &np, nullptr,
                                                  &pu);
     if (rc == VX_OK) rc = finalize(s, np, F_ALPHA);
     if (rc != VX_OK) break;
-    hipLaunchKernelGGL(k_cg_update<T>, dim3(EW_BLOCKS), dim3(EW_THREADS), 0, s->stream, n,
+    hipLaunchKernelGGL(k_cg_update<T>, dim3(EW_BLOCKS, nb), dim3(EW_THREADS), 0, s->stream, n,
                        s->dscal, u, r, pcur, Ap, s->partial);
     if (hipGetLastError() != hipSuccess) { rc = fail(s, VX_ERR_HIP, "cg_update launch"); break; }
     rc = finalize(s, EW_BLOCKS, F_RATIO);
     if (rc == VX_OK) rc = read_scalars(s, ev);
     if (rc != VX_OK) break;
     // Queue the next preconditioner application while the host waits for the
-    // ratio, unless convergence looks imminent (then waiting is cheaper than
-    // a wasted V-cycle).
-    bool predicted_done = false;
-    if (!std::isnan(prev_ratio) && !std::isnan(prev2_ratio) && prev2_ratio > 0) {
-      const double next = prev_ratio * (prev_ratio / prev2_ratio);
-      predicted_done = next < 3.0 * tol_rel;
+    // ratios, unless every open right-hand side looks about to converge
+    // (then waiting is cheaper than a wasted V-cycle).
+    bool predicted_done = true;
+    for (int q = 0; q < nb; ++q) {
+      if (frozen[q]) continue;
+      bool d = false;
+      if (!std::isnan(prev[q]) && !std::isnan(prev2[q]) && prev2[q] > 0)
+        d = prev[q] * (prev[q] / prev2[q]) < 3.0 * tol_rel;
+      predicted_done = predicted_done && d;
     }
     auto queue_next = [&]() -> int {
       int np2 = 0;
@@ -498,33 +540,54 @@ int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* p2, T* Ap, T* t0, double tol_re
       queued = true;
     }
     if (hipEventSynchronize(ev) != hipSuccess) { rc = fail(s, VX_ERR_HIP, "event sync"); break; }
-    const Scalars sc = *s->hscal;
-    if (sc.pAp == 0.0) {
-      res->converged = res->ratio < tol_rel;
-      break;
+    bool changed = false;
+    double worst = 0.0;
+    for (int q = 0; q < nb; ++q) {
+      if (frozen[q]) continue;
+      const Scalars sc = s->hscal[q];
+      PcgResult& R = res[q];
+      if (sc.pAp == 0.0) {   // nothing left along p (core/multigrid.py:500-511)
+        R.converged = R.ratio < tol_rel;
+        frozen[q] = 1;
+        changed = true;
+        continue;
+      }
+      const double ratio = sc.ratio;
+      R.iters = it + 1;
+      R.ratio = ratio;
+      const int gi = iter_offset[q] + it;
+      if (hist && gi < hist_cap) hist[(long long)q * hist_cap + gi] = ratio * ratio_scale[q];
+      worst = std::max(worst, ratio * ratio_scale[q]);
+      if (ratio < tol_rel) {
+        R.converged = true;
+        frozen[q] = 1;
+        changed = true;
+        continue;
+      }
+      best[q] = std::min(best[q], ratio);
+      if (stall_guard && it + 1 >= 10 && ratio > 5.0 * best[q]) {
+        R.stalled = true;
+        frozen[q] = 1;
+        changed = true;
+        char buf[200];
+        snprintf(buf, sizeof buf,
+                 "MG-PCG stalled/diverged at iter %d (resid ratio %.3e, best seen %.3e)", it + 1,
+                 ratio, best[q]);
+        s->err = buf;
+        continue;
+      }
+      prev2[q] = prev[q];
+      prev[q] = ratio;
     }
-    const double ratio = sc.ratio;
-    res->iters = it + 1;
-    res->ratio = ratio;
-    const int gi = iter_offset + it;
-    if (hist && gi < hist_cap) hist[gi] = ratio * ratio_scale;
-    if (cb) cb(gi + 1, ratio * ratio_scale, cb_tol, user);
-    if (ratio < tol_rel) {
-      res->converged = true;
-      break;
+    if (cb) cb(iter_offset[0] + it + 1, worst, cb_tol, user);
+    const bool all_done = std::all_of(frozen.begin(), frozen.end(), [](int f) { return f; });
+    if (all_done) break;
+    if (changed) {
+      // stopped right-hand sides get alpha = 0 from the next finalize on;
+      // the flags land before the queued V-cycle's beta, which is harmless
+      rc = push_frozen(s, frozen);
+      if (rc != VX_OK) break;
     }
-    best = std::min(best, ratio);
-    if (stall_guard && it + 1 >= 10 && ratio > 5.0 * best) {
-      res->stalled = true;
-      char buf[200];
-      snprintf(buf, sizeof buf,
-               "MG-PCG stalled/diverged at iter %d (resid ratio %.3e, best seen %.3e)", it + 1,
-               ratio, best);
-      s->err = buf;
-      break;
-    }
-    prev2_ratio = prev_ratio;
-    prev_ratio = ratio;
     if (!queued) {
       rc = queue_next();
       if (rc != VX_OK) break;
@@ -557,6 +620,8 @@ void vx_default_options(vx_options* o) {
   o->inner_tol = 1e-5;
   o->max_outer = 20;
   o->z_chunk = 0;
+  o->e_coarsen = 0;
+  o->max_batch = 1;
 }
 
 int vx_device_count(void) {
@@ -598,6 +663,7 @@ int vx_create(int32_t nx, int32_t ny, int32_t nz, double nu, const vx_options* o
   } else {
     vx_default_options(&s->opt);
   }
+  if (s->opt.max_batch < 1) s->opt.max_batch = 1;
   if (s->opt.n_smooth < 1 || s->opt.coarse_sweeps < 1 || s->opt.max_levels < 1) {
     delete s;
     return VX_ERR_ARG;
@@ -631,7 +697,7 @@ int vx_create(int32_t nx, int32_t ny, int32_t nz, double nu, const vx_options* o
     dim3 g = tile_grid(s, L, zc);
     maxpart = std::max<long long>(maxpart, (long long)g.x * g.y * g.z);
   }
-  s->n_partial = (int)std::max<long long>(maxpart, EW_BLOCKS);
+  s->n_partial = (int)(std::max<long long>(maxpart, EW_BLOCKS) * s->opt.max_batch);
   int rc = VX_OK;
   for (auto& L : s->lv) {
     if ((rc = dalloc(s, &L.mask, L.Nn)) != VX_OK) break;
@@ -642,8 +708,13 @@ int vx_create(int32_t nx, int32_t ny, int32_t nz, double nu, const vx_options* o
     if ((rc = dalloc(s, &L.E32, L.Ne)) != VX_OK) break;
   }
   if (rc == VX_OK) rc = dalloc(s, &s->partial, (size_t)s->n_partial);
-  if (rc == VX_OK) rc = dalloc(s, &s->dscal, 1);
-  if (rc == VX_OK && hipHostMalloc((void**)&s->hscal, sizeof(Scalars)) != hipSuccess)
+  if (rc == VX_OK) rc = dalloc(s, &s->dscal, (size_t)s->opt.max_batch);
+  if (rc == VX_OK) rc = dalloc(s, &s->dflags, (size_t)s->opt.max_batch);
+  if (rc == VX_OK &&
+      hipHostMalloc((void**)&s->hscal, sizeof(Scalars) * s->opt.max_batch) != hipSuccess)
+    rc = VX_ERR_NOMEM;
+  if (rc == VX_OK &&
+      hipHostMalloc((void**)&s->hflags, sizeof(int) * s->opt.max_batch) != hipSuccess)
     rc = VX_ERR_NOMEM;
   if (rc != VX_OK) {
     vx_destroy(s);
@@ -659,6 +730,7 @@ void vx_destroy(vx_handle s) {
   else (void)hipDeviceSynchronize();
   for (void* p : s->allocs) (void)hipFree(p);
   if (s->hscal) (void)hipHostFree(s->hscal);
+  if (s->hflags) (void)hipHostFree(s->hflags);
   delete s;
 }
 
@@ -740,7 +812,7 @@ int vx_set_density(vx_handle s, const double* E, const uint8_t* active, double e
 }  // extern "C"
 
 static int alloc_pcg64(vx_solver* s) {
-  const size_t n = 3 * (size_t)s->lv[0].Nn;
+  const size_t n = 3 * (size_t)s->lv[0].Nn * s->opt.max_batch;
   CHK(dalloc(s, &s->r64, n));
   CHK(dalloc(s, &s->z64, n));
   CHK(dalloc(s, &s->p64, n));
@@ -751,7 +823,7 @@ static int alloc_pcg64(vx_solver* s) {
 }
 
 static int alloc_pcg32(vx_solver* s) {
-  const size_t n = 3 * (size_t)s->lv[0].Nn;
+  const size_t n = 3 * (size_t)s->lv[0].Nn * s->opt.max_batch;
   CHK(dalloc(s, &s->r32, n));
   CHK(dalloc(s, &s->z32, n));
   CHK(dalloc(s, &s->p32, n));
@@ -763,15 +835,21 @@ static int alloc_pcg32(vx_solver* s) {
 
 extern "C" {
 
-int vx_solve(vx_handle s, const double* f, double* u, const double* x0, double tol,
-             int32_t max_iter, vx_progress_cb cb, void* user, double* hist, int32_t hist_cap,
-             vx_info* info) {
-  if (!s || !f || !u) return fail(s, VX_ERR_ARG, "null solve argument");
+}  // extern "C"
+
+// Solve of nb right-hand sides (vectors [nb][3 Nn]); vx_solve is nb = 1.
+static int solve_impl(vx_solver* s, int nb, const double* F, double* U, const double* X0,
+                      double tol, int max_iter, vx_progress_cb cb, void* user, double* hist,
+                      int hist_cap, vx_info* infos) {
+  if (!s || !F || !U) return fail(s, VX_ERR_ARG, "null solve argument");
+  if (nb < 1 || nb > s->opt.max_batch)
+    return fail(s, VX_ERR_ARG, "batch size outside [1, max_batch]");
   CHK(require_ready(s));
   if (max_iter <= 0) max_iter = 1000;   // core/multigrid.py:481
   const bool mixed = s->opt.precision == VX_MIXED;
   CHK(alloc_pcg64(s));
   if (mixed) CHK(alloc_pcg32(s));
+  s->nb = nb;
   Level& L0 = s->lv[0];
   const long long n = 3 * L0.Nn;
   hipEvent_t e0, e1;
@@ -779,131 +857,162 @@ int vx_solve(vx_handle s, const double* f, double* u, const double* x0, double t
   HIPCHK(hipEventCreate(&e1));
   HIPCHK(hipEventRecord(e0, s->stream));
 
-  // f masked; fnorm = ||f_free|| (core/multigrid.py:460-471)
-  hipLaunchKernelGGL((k_mask_copy<double, double>), dim3(EW_BLOCKS), dim3(EW_THREADS), 0,
-                     s->stream, L0.Nn, L0.mask, f, s->fm64, s->partial);
+  // f masked; fnorm = ||f_free|| per right-hand side (core/multigrid.py:460-471)
+  hipLaunchKernelGGL((k_mask_copy<double, double>), dim3(EW_BLOCKS, nb), dim3(EW_THREADS), 0,
+                     s->stream, L0.Nn, L0.mask, F, s->fm64, s->partial);
   LAUNCHCHK();
   CHK(finalize(s, EW_BLOCKS, F_FNORM));
-  // initial iterate
-  if (x0) {
-    hipLaunchKernelGGL((k_mask_copy<double, double>), dim3(EW_BLOCKS), dim3(EW_THREADS), 0,
-                       s->stream, L0.Nn, L0.mask, x0, u, nullptr);
+  if (X0) {
+    hipLaunchKernelGGL((k_mask_copy<double, double>), dim3(EW_BLOCKS, nb), dim3(EW_THREADS), 0,
+                       s->stream, L0.Nn, L0.mask, X0, U, nullptr);
     LAUNCHCHK();
   } else {
-    HIPCHK(hipMemsetAsync(u, 0, sizeof(double) * n, s->stream));
+    HIPCHK(hipMemsetAsync(U, 0, sizeof(double) * n * nb, s->stream));
   }
-  HIPCHK(hipMemcpyAsync(s->hscal, s->dscal, sizeof(Scalars), hipMemcpyDeviceToHost, s->stream));
-  HIPCHK(hipStreamSynchronize(s->stream));
-  const double fnorm = s->hscal->fnorm;
-  vx_info inf{};
-  inf.n_levels = (int)s->lv.size();
-  if (fnorm == 0.0) {
-    HIPCHK(hipMemsetAsync(u, 0, sizeof(double) * n, s->stream));
-    HIPCHK(hipStreamSynchronize(s->stream));
-    inf.converged = 1;
-    inf.resid_ratio = 0.0;
-    inf.true_resid_ratio = 0.0;
-    if (info) *info = inf;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    return VX_OK;
+  CHK(sync_scalars(s));
+  std::vector<double> fnorm(nb);
+  std::vector<int> zero(nb, 0);
+  for (int q = 0; q < nb; ++q) {
+    fnorm[q] = s->hscal[q].fnorm;
+    zero[q] = fnorm[q] == 0.0;
+  }
+  std::vector<vx_info> inf(nb);
+  for (int q = 0; q < nb; ++q) {
+    inf[q] = vx_info{};
+    inf[q].n_levels = (int)s->lv.size();
+    if (zero[q]) {   // zero load: u = 0 (core/multigrid.py:471-473)
+      inf[q].converged = 1;
+      inf[q].resid_ratio = inf[q].true_resid_ratio = 0.0;
+      HIPCHK(hipMemsetAsync(U + q * n, 0, sizeof(double) * n, s->stream));
+    }
   }
   int rc = VX_OK;
-  PcgResult pr;
+  std::vector<PcgResult> pr(nb);
+  std::vector<int> stalled(nb, 0);
   int np = 0;
-  if (!mixed) {
-    // r = f - A u  (or f when u = 0)
-    if (x0) {
-      rc = launch_tile<double, M_RESID, false>(s, 0, u, s->fm64, s->r64, nullptr, nullptr);
-    } else if (hipMemcpyAsync(s->r64, s->fm64, sizeof(double) * n, hipMemcpyDeviceToDevice,
-                              s->stream) != hipSuccess) {
-      rc = fail(s, VX_ERR_HIP, "copy f->r");
-    }
-    if (rc == VX_OK)
-      rc = pcg<double>(s, u, s->r64, s->z64, s->p64, s->q64, s->Ap64, s->lv[0].t64, tol,
-                       max_iter, true, 1.0, tol, 0, cb, user, hist, hist_cap, &pr);
-    inf.iters = pr.iters;
-    inf.converged = pr.converged ? 1 : 0;
-    inf.resid_ratio = pr.ratio;
-  } else {
-    // Iterative refinement: FP64 true residual, FP32 inner MG-PCG.
-    int total = 0;
-    double true_ratio = NAN;
-    for (int outer = 0; outer <= s->opt.max_outer && rc == VX_OK; ++outer) {
-      if (outer == 0 && !x0) {
-        rc = (hipMemcpyAsync(s->r64, s->fm64, sizeof(double) * n, hipMemcpyDeviceToDevice,
-                             s->stream) == hipSuccess)
-                 ? VX_OK
-                 : fail(s, VX_ERR_HIP, "copy f->r");
-        if (rc == VX_OK) {
-          hipLaunchKernelGGL(k_dot<double>, dim3(EW_BLOCKS), dim3(EW_THREADS), 0, s->stream, n,
-                             s->r64, s->r64, s->partial);
-          np = EW_BLOCKS;
+  if (!std::all_of(zero.begin(), zero.end(), [](int z) { return z; })) {
+    if (!mixed) {
+      // r = f - A u  (or f when u = 0)
+      if (X0) rc = launch_tile<double, M_RESID, false>(s, 0, U, s->fm64, s->r64, nullptr, nullptr);
+      else rc = copy_vec<double>(s, s->r64, s->fm64, n * nb);
+      std::vector<double> one(nb, 1.0);
+      std::vector<int> off(nb, 0);
+      if (rc == VX_OK)
+        rc = pcg<double>(s, U, s->r64, s->z64, s->p64, s->q64, s->Ap64, L0.t64, tol, max_iter,
+                         true, one, tol, off, cb, user, hist, hist_cap, pr, zero);
+      for (int q = 0; q < nb; ++q) {
+        if (zero[q]) continue;
+        inf[q].iters = pr[q].iters;
+        inf[q].converged = pr[q].converged ? 1 : 0;
+        inf[q].resid_ratio = pr[q].ratio;
+        stalled[q] = pr[q].stalled;
+      }
+    } else {
+      // Iterative refinement: FP64 true residual, FP32 inner MG-PCG.
+      std::vector<int> total(nb, 0), done(zero);
+      std::vector<double> true_ratio(nb, 0.0);
+      for (int outer = 0; outer <= s->opt.max_outer && rc == VX_OK; ++outer) {
+        if (outer == 0 && !X0) {
+          rc = copy_vec<double>(s, s->r64, s->fm64, n * nb);
+          if (rc == VX_OK) rc = dot_partials<double>(s, n, s->r64, s->r64, &np);
+        } else {
+          rc = launch_tile<double, M_RESID, true>(s, 0, U, s->fm64, s->r64, s->partial, &np);
         }
-      } else {
-        rc = launch_tile<double, M_RESID, true>(s, 0, u, s->fm64, s->r64, s->partial, &np);
+        if (rc == VX_OK) rc = finalize(s, np, F_SET_TMP);
+        if (rc == VX_OK) rc = sync_scalars(s);
+        if (rc != VX_OK) break;
+        int open = 0, most = 0;
+        double worst = 0.0;
+        for (int q = 0; q < nb; ++q) {
+          if (zero[q]) continue;
+          true_ratio[q] = std::sqrt(s->hscal[q].tmp) / fnorm[q];
+          inf[q].outer_iters = outer;
+          if (true_ratio[q] < tol) done[q] = 1;
+          if (!done[q]) {
+            ++open;
+            worst = std::max(worst, true_ratio[q]);
+          }
+          most = std::max(most, total[q]);
+        }
+        if (!open || outer == s->opt.max_outer || most >= max_iter) break;
+        // mark finished right-hand sides (tmp2) so their inner input is 0
+        for (int q = 0; q < nb; ++q) s->hscal[q].tmp2 = done[q] ? 1.0 : 0.0;
+        HIPCHK(hipMemcpyAsync(s->dscal, s->hscal, sizeof(Scalars) * nb, hipMemcpyHostToDevice,
+                              s->stream));
+        hipLaunchKernelGGL(k_scale_to_f32, dim3(EW_BLOCKS, nb), dim3(EW_THREADS), 0, s->stream, n,
+                           s->dscal, s->r64, s->r32);
+        LAUNCHCHK();
+        HIPCHK(hipMemsetAsync(s->d32, 0, sizeof(float) * n * nb, s->stream));
+        // each inner right-hand side has unit norm
+        hipLaunchKernelGGL(k_set_fnorm, dim3(nb), dim3(1), 0, s->stream, s->dscal, 1.0);
+        LAUNCHCHK();
+        const double want = std::max(s->opt.inner_tol, 0.5 * tol / worst);
+        rc = pcg<float>(s, s->d32, s->r32, s->z32, s->p32, s->q32, s->Ap32, L0.t32, want,
+                        max_iter - most, false, true_ratio, tol, total, cb, user, hist, hist_cap,
+                        pr, done);
+        if (rc != VX_OK) break;
+        bool progressed = false;
+        for (int q = 0; q < nb; ++q) {
+          if (done[q]) continue;
+          total[q] += pr[q].iters;
+          progressed = progressed || pr[q].iters > 0;
+        }
+        hipLaunchKernelGGL(k_add_scaled, dim3(EW_BLOCKS, nb), dim3(EW_THREADS), 0, s->stream, n,
+                           s->dscal, s->d32, U);
+        LAUNCHCHK();
+        if (!progressed) break;
       }
-      if (rc == VX_OK) rc = finalize(s, np, F_SET_TMP);
-      if (rc != VX_OK) break;
-      if (hipMemcpyAsync(s->hscal, s->dscal, sizeof(Scalars), hipMemcpyDeviceToHost,
-                         s->stream) != hipSuccess ||
-          hipStreamSynchronize(s->stream) != hipSuccess) {
-        rc = fail(s, VX_ERR_HIP, "scalar readback");
-        break;
+      for (int q = 0; q < nb; ++q) {
+        if (zero[q]) continue;
+        inf[q].iters = total[q];
+        inf[q].resid_ratio = inf[q].true_resid_ratio = true_ratio[q];
+        inf[q].converged = true_ratio[q] < tol ? 1 : 0;
       }
-      true_ratio = std::sqrt(s->hscal->tmp) / fnorm;
-      inf.outer_iters = outer;
-      if (true_ratio < tol || outer == s->opt.max_outer || total >= max_iter) break;
-      // inner solve of A d = r / ||r|| to inner_tol (tight enough for the last step)
-      hipLaunchKernelGGL(k_scale_to_f32, dim3(EW_BLOCKS), dim3(EW_THREADS), 0, s->stream, n,
-                         s->dscal, s->r64, s->r32);
-      LAUNCHCHK();
-      HIPCHK(hipMemsetAsync(s->d32, 0, sizeof(float) * n, s->stream));
-      const double want = std::max(s->opt.inner_tol, 0.5 * tol / true_ratio);
-      // the inner right-hand side has unit norm
-      hipLaunchKernelGGL(k_set_fnorm, dim3(1), dim3(1), 0, s->stream, s->dscal, 1.0);
-      LAUNCHCHK();
-      rc = pcg<float>(s, s->d32, s->r32, s->z32, s->p32, s->q32, s->Ap32, s->lv[0].t32, want,
-                      max_iter - total, false, true_ratio, tol, total, cb, user, hist, hist_cap,
-                      &pr);
-      total += pr.iters;
-      if (rc != VX_OK) break;
-      hipLaunchKernelGGL(k_add_scaled, dim3(EW_BLOCKS), dim3(EW_THREADS), 0, s->stream, n,
-                         s->dscal, s->d32, u);
-      LAUNCHCHK();
-      if (pr.iters == 0) break;
     }
-    inf.iters = total;
-    inf.resid_ratio = true_ratio;
-    inf.true_resid_ratio = true_ratio;
-    inf.converged = (true_ratio < tol) ? 1 : 0;
   }
   // final mask of u (core/multigrid.py:547) and the true FP64 residual
-  if (rc == VX_OK || rc == VX_ERR_STALL) {
-    hipLaunchKernelGGL((k_mask_copy<double, double>), dim3(EW_BLOCKS), dim3(EW_THREADS), 0,
-                       s->stream, L0.Nn, L0.mask, u, u, nullptr);
+  if (rc == VX_OK) {
+    hipLaunchKernelGGL((k_mask_copy<double, double>), dim3(EW_BLOCKS, nb), dim3(EW_THREADS), 0,
+                       s->stream, L0.Nn, L0.mask, U, U, nullptr);
     HIPCHK(hipEventRecord(e1, s->stream));
     if (!mixed) {
-      int rc2 = launch_tile<double, M_RESID, true>(s, 0, u, s->fm64, s->z64, s->partial, &np);
-      if (rc2 == VX_OK) rc2 = finalize(s, np, F_SET_TMP);
-      if (rc2 == VX_OK) {
-        HIPCHK(hipMemcpyAsync(s->hscal, s->dscal, sizeof(Scalars), hipMemcpyDeviceToHost,
-                              s->stream));
+      rc = launch_tile<double, M_RESID, true>(s, 0, U, s->fm64, s->z64, s->partial, &np);
+      if (rc == VX_OK) rc = finalize(s, np, F_SET_TMP);
+    }
+    if (rc == VX_OK) rc = sync_scalars(s);
+    if (rc == VX_OK) {
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      for (int q = 0; q < nb; ++q) {
+        if (!mixed && !zero[q]) inf[q].true_resid_ratio = std::sqrt(s->hscal[q].tmp) / fnorm[q];
+        inf[q].solve_ms = ms;
       }
     }
-    HIPCHK(hipStreamSynchronize(s->stream));
-    if (!mixed) inf.true_resid_ratio = std::sqrt(s->hscal->tmp) / fnorm;
-    float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    inf.solve_ms = ms;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  if (info) *info = inf;
+  if (infos)
+    for (int q = 0; q < nb; ++q) infos[q] = inf[q];
+  s->nb = 1;
   if (rc != VX_OK) return rc;
-  if (pr.stalled) return VX_ERR_STALL;
-  if (!inf.converged) return VX_ERR_NOT_CONVERGED;
+  if (std::any_of(stalled.begin(), stalled.end(), [](int v) { return v; })) return VX_ERR_STALL;
+  for (int q = 0; q < nb; ++q)
+    if (!inf[q].converged) return VX_ERR_NOT_CONVERGED;
   return VX_OK;
+}
+
+extern "C" {
+
+int vx_solve(vx_handle s, const double* f, double* u, const double* x0, double tol,
+             int32_t max_iter, vx_progress_cb cb, void* user, double* hist, int32_t hist_cap,
+             vx_info* info) {
+  return solve_impl(s, 1, f, u, x0, tol, max_iter, cb, user, hist, hist_cap, info);
+}
+
+int vx_solve_batch(vx_handle s, int32_t nrhs, const double* F, double* U, double tol,
+                   int32_t max_iter, vx_progress_cb cb, void* user, double* hist,
+                   int32_t hist_cap, vx_info* infos) {
+  return solve_impl(s, nrhs, F, U, nullptr, tol, max_iter, cb, user, hist, hist_cap, infos);
 }
 
 // ---------------------------------------------------------------------------
@@ -919,6 +1028,7 @@ int vx_solve_jacobi(vx_handle s, const double* E, const uint8_t* active, const d
                     double* u, double tol, int32_t max_iter, vx_progress_cb cb, void* user,
                     double* hist, int32_t hist_cap, vx_info* info) {
   if (!s || !E || !f || !u) return fail(s, VX_ERR_ARG, "null Jacobi argument");
+  s->nb = 1;
   if (!s->fixed_set) return fail(s, VX_ERR_ARG, "call vx_set_fixed() before solving");
   CHK(alloc_pcg64(s));
   Level& L0 = s->lv[0];
@@ -1030,6 +1140,39 @@ int vx_solve_jacobi(vx_handle s, const double* E, const uint8_t* active, const d
   return inf.converged ? VX_OK : VX_ERR_NOT_CONVERGED;
 }
 
+}  // extern "C"
+
+extern "C" int vx_post_fields_impl(int nx, int ny, int nz, void* stream, const double* u,
+                                   const double* vm, const uint8_t* active, double vsize,
+                                   double yield, float* stress3d, float* safety3d, float* disp3d,
+                                   double* out5, char* err, int errlen);
+
+extern "C" {
+
+// Post-solve fields of the solve worker (core/solve_worker.py:358-431).
+int vx_post_fields(vx_handle s, const double* u, const double* E, const uint8_t* active,
+                   double vsize, double yield_pa, float* stress3d, float* safety3d,
+                   float* disp3d, double* scalars5) {
+  if (!s || !u || !E || !stress3d || !safety3d || !disp3d || !scalars5 || !(vsize > 0))
+    return fail(s, VX_ERR_ARG, "bad post-field arguments");
+  Level& L = s->lv[0];
+  double* vm = nullptr;
+  HIPCHK(hipMalloc(&vm, L.Ne * sizeof(double)));
+  hipLaunchKernelGGL(k_von_mises, dim3(ew_grid(L.Ne)), dim3(EW_THREADS), 0, s->stream, L.nx, L.ny,
+                     L.nz, s->nu, u, E, active, vm);
+  hipError_t le = hipGetLastError();
+  if (le != hipSuccess) {
+    (void)hipFree(vm);
+    return fail(s, VX_ERR_HIP, std::string("von Mises launch: ") + hipGetErrorString(le));
+  }
+  char err[256] = {0};
+  const int rc = vx_post_fields_impl(L.nx, L.ny, L.nz, s->stream, u, vm, active, vsize, yield_pa,
+                                     stress3d, safety3d, disp3d, scalars5, err, sizeof err);
+  (void)hipFree(vm);
+  if (rc != VX_OK) return fail(s, rc, err);
+  return VX_OK;
+}
+
 // element_strain_energy (core/fea.py:595-602): u_e^T KE u_e with unit E.
 int vx_strain_energy(vx_handle s, const double* u, const uint8_t* active, double* ce) {
   if (!s || !u || !ce) return fail(s, VX_ERR_ARG, "null strain-energy argument");
@@ -1048,6 +1191,7 @@ int vx_strain_energy(vx_handle s, const double* u, const uint8_t* active, double
 // ---------------------------------------------------------------------------
 static int check_level(vx_solver* s, int l) {
   if (!s) return VX_ERR_ARG;
+  s->nb = 1;
   if (l < 0 || l >= (int)s->lv.size()) return fail(s, VX_ERR_ARG, "level out of range");
   return require_ready(s);
 }
