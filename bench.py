@@ -127,6 +127,130 @@ def kernel_only(p, E_dev, act_dev, MGSolver):
                       "copy_bytes_read": 1 << 30, "copy_bytes_written": 1 << 30}), flush=True)
 
 
+def _oracle_solve(p, max_levels=6, active_system=False):
+    """Complete CPU oracle solve of a problem (reference algorithm)."""
+    import numpy as np
+    from oracle.fea_ref import VoxelFEARef
+    vf = VoxelFEARef(p.nx, p.ny, p.nz, nu=p.nu, active_elems=p.active,
+                     use_multigrid=not active_system, mg_kwargs={"max_levels": max_levels})
+    vf.set_fixed(p.fixed_dofs)
+    t0 = time.perf_counter()
+    u = vf.solve(p.E_stiffness, p.force if p.force.ndim == 1 else p.force[0], tol=p.tol)
+    return u, time.perf_counter() - t0, vf.last_iters, vf.last_converged
+
+
+def other_workload(args):
+    """Measurement of one of the other configurations (not the headline)."""
+    import numpy as np
+    import torch
+    from core import _native
+    from core import workloads as W
+    from core.multigrid import MGSolver
+
+    def dev(a, dt=None):
+        return torch.tensor(a, device="cuda") if dt is None else torch.tensor(a, device="cuda",
+                                                                              dtype=dt)
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, out
+
+    out = {"workload": args.workload}
+    if args.workload in ("cantilever", "voids_block"):
+        p = W.cantilever() if args.workload == "cantilever" else W.voids_block()
+        E, A, f = dev(p.E_stiffness), dev(p.active.astype(np.uint8)), dev(p.force)
+        u = torch.empty_like(f)
+        used = np.repeat(W.element_nodes_mask(p.active.reshape(p.nx, p.ny, p.nz)).ravel(), 3) \
+            if not p.active.all() else np.ones(p.ndof, bool)
+        res, sol = {}, {}
+        variants = (("reference_fp64", dict(max_levels=p.max_levels)),
+                    ("mixed_full", dict(OPT, system="full", max_levels=p.max_levels)),
+                    ("mixed_active", dict(OPT, max_levels=p.max_levels)))
+        for name, kw in variants:
+            mg = MGSolver(p.nx, p.ny, p.nz, p.fixed_dofs, nu=p.nu, compute_mode="GPU", **kw)
+            run = lambda: (mg.set_density(E, active=A, e_min=1e-9),
+                           mg.solve_device(f, u, tol=p.tol))
+            try:
+                timed(run)
+                t, _ = timed(run)
+                res[name] = {"time_s": t, "iters": mg.last_iters,
+                             "converged": mg.last_converged,
+                             "true_resid": mg.last_true_resid_ratio}
+                sol[name] = u.cpu().numpy()
+            except _native.StallError as exc:
+                res[name] = {"stalled": str(exc)}
+            mg.close()
+        out.update(res)
+
+        def rel(x, y):
+            return float(np.abs(x[used] - y[used]).max() / np.abs(y[used]).max())
+        if "reference_fp64" in sol:
+            for k in ("mixed_full", "mixed_active"):
+                if k in sol:
+                    out[k + "_rel_err_vs_fp64"] = rel(sol[k], sol["reference_fp64"])
+        if args.oracle_full or args.workload == "cantilever":
+            log("oracle: complete CPU solve")
+            uo, t_o, it_o, conv_o = _oracle_solve(p, p.max_levels)
+            out["cpu_oracle"] = {"time_s": t_o, "iters": it_o, "converged": conv_o,
+                                 "cores": 1}
+            for k in sol:
+                out[k + "_rel_err_vs_oracle"] = rel(sol[k], uo)
+    elif args.workload == "multi_rhs_part":
+        p = W.multi_rhs_part()
+        nrhs = p.force.shape[0]
+        E, A, F = dev(p.E_stiffness), dev(p.active.astype(np.uint8)), dev(p.force)
+        U = torch.empty_like(F)
+        kw = dict(OPT)
+        bat = MGSolver(p.nx, p.ny, p.nz, p.fixed_dofs, nu=p.nu, compute_mode="GPU",
+                       max_batch=nrhs, **kw)
+        run = lambda: (bat.set_density(E, active=A), bat.solve_batch_device(F, U, tol=p.tol))
+        timed(run)
+        t_b, _ = timed(run)
+        out["batched_mixed"] = {"time_s": t_b, "nrhs": nrhs,
+                                "iters_max": max(i["iters"] for i in bat.last_batch),
+                                "all_converged": all(i["converged"] for i in bat.last_batch),
+                                "true_resid_max": max(i["true_resid_ratio"] for i in bat.last_batch)}
+        Ub = U.clone()
+        one = MGSolver(p.nx, p.ny, p.nz, p.fixed_dofs, nu=p.nu, compute_mode="GPU", **kw)
+        one.set_density(E, active=A)
+        u = torch.empty_like(F[0])
+
+        def seq():
+            one.set_density(E, active=A)
+            for q in range(nrhs):
+                one.solve_device(F[q], u, tol=p.tol)
+                U[q].copy_(u)
+        timed(seq)
+        t_s, _ = timed(seq)
+        out["sequential_mixed"] = {"time_s": t_s}
+        out["batched_speedup"] = t_s / t_b
+        out["batched_vs_sequential_max_rel_diff"] = float(
+            ((Ub - U).abs().amax(dim=1) / U.abs().amax(dim=1)).max().item())
+        out["load_cases_per_s"] = nrhs / t_b
+    elif args.workload == "uneven_parts":
+        parts = W.uneven_parts()
+        rows = []
+        tot = 0.0
+        for q in parts:
+            E, A, f = dev(q.E_stiffness), dev(q.active.astype(np.uint8)), dev(q.force)
+            u = torch.empty_like(f)
+            mg = MGSolver(q.nx, q.ny, q.nz, q.fixed_dofs, nu=q.nu, compute_mode="GPU", **OPT)
+            run = lambda: (mg.set_density(E, active=A), mg.solve_device(f, u, tol=q.tol,
+                                                                         max_cg=5000))
+            timed(run)
+            t, _ = timed(run)
+            tot += t
+            rows.append({"dims": [q.nx, q.ny, q.nz], "levels": len(mg.levels), "time_s": t,
+                         "iters": mg.last_iters, "converged": mg.last_converged})
+            mg.close()
+        out["parts"] = rows
+        out["total_time_s"] = tot
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,6 +260,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ref-iters", type=int, default=3000,
                     help="Jacobi-PCG fallback iterations of the reference comparator")
+    ap.add_argument("--workload", default="two_material",
+                    choices=["two_material", "cantilever", "voids_block", "multi_rhs_part",
+                             "uneven_parts"],
+                    help="two_material is the headline line; the others print their own "
+                         "per-configuration measurement")
+    ap.add_argument("--oracle-full", action="store_true",
+                    help="per-configuration modes: run the complete CPU oracle solve")
     ap.add_argument("--kernel-only", action="store_true",
                     help="run only the dominant kernel (and a plain copy of known size) for "
                          "hardware-counter passes")
@@ -150,6 +281,10 @@ def main():
     from core import _native
     from core import workloads as W
     from core.multigrid import MGSolver
+
+    if args.workload != "two_material":
+        print(json.dumps(other_workload(args)), flush=True)
+        return
 
     log(f"generating the two-material workload, n={args.n}")
     p = W.two_material(n=args.n)
