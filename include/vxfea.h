@@ -193,6 +193,39 @@ int vx_drop_islands(vx_handle h, const uint8_t* active_dev, const int64_t* fixed
 int vx_time_kernel(vx_handle h, int32_t kind, int32_t level, int32_t precision,
                    int32_t reps, double* avg_ms);
 
+/* ---------------------------------------------------------------------------
+ * Learned coarse-grid corrector network (no counterpart in the reference: its
+ * V-cycle, core/multigrid.py:426-436, has no learned part).  A 4-level 3D
+ * U-Net on a node grid: 14 convolutions 3x3x3 (channels 16 -> c0 -> c1 -> c2
+ * -> c2 and back, stride-2 down-sampling, trilinear up-sampling with additive
+ * skips), BF16 activations and weights, FP32 accumulation on the matrix
+ * cores.  Input and output have 16 channels; tensors are [B][voxel][16]
+ * float32 with voxel = x + X*(y + Y*z).  The flat float32 parameter vector
+ * holds, per layer, W[27][Cin][Cout] (tap = dx + 3*(dy + 3*dz)) then b[Cout].
+ * ------------------------------------------------------------------------- */
+typedef struct vx_net* vxn_handle;
+
+/* channels c0, c1, c2 in {16, 32, 64}; parameters start at 0 */
+int vxn_create(int32_t c0, int32_t c1, int32_t c2, vxn_handle* out);
+void vxn_destroy(vxn_handle n);
+const char* vxn_last_error(vxn_handle n);
+int vxn_set_stream(vxn_handle n, void* hip_stream);
+int64_t vxn_num_params(vxn_handle n);
+/* copy the parameters from / to a device float32 vector; set also clears
+ * the Adam moments */
+int vxn_set_params(vxn_handle n, const float* params_dev);
+int vxn_get_params(vxn_handle n, float* params_dev);
+/* out = net(in) for B samples on an X*Y*Z node grid; keeps the activations
+ * for vxn_backward */
+int vxn_forward(vxn_handle n, int32_t B, int32_t X, int32_t Y, int32_t Z, const float* in_dev,
+                float* out_dev);
+/* gradient of sum(out * dout) w.r.t. the parameters (overwrites grad_dev)
+ * for the inputs of the last vxn_forward */
+int vxn_backward(vxn_handle n, const float* dout_dev, float* grad_dev);
+/* one Adam step (step counts from 1) on the parameters */
+int vxn_adam_step(vxn_handle n, const float* grad_dev, float lr, float beta1, float beta2,
+                  float eps, int32_t step);
+
 /* Host-side element operator (the kernels' own code compiled for the CPU):
  * y24 = KE(nu) * u24 for one unit element in the reference's local DOF order
  * (core/fea.py:33-36), E = 1. */

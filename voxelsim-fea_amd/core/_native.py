@@ -107,6 +107,17 @@ SIGNATURES = {
     "vx_drop_islands": (ctypes.c_int, [_P, _P, _P, _I64, _P, ctypes.POINTER(_I64)]),
     "vx_time_kernel": (ctypes.c_int, [_P, _I32, _I32, _I32, _I32, ctypes.POINTER(_D)]),
     "vx_element_apply_host": (None, [_D, _P, _P]),
+    "vxn_create": (ctypes.c_int, [_I32, _I32, _I32, ctypes.POINTER(_P)]),
+    "vxn_destroy": (None, [_P]),
+    "vxn_last_error": (ctypes.c_char_p, [_P]),
+    "vxn_set_stream": (ctypes.c_int, [_P, _P]),
+    "vxn_num_params": (_I64, [_P]),
+    "vxn_set_params": (ctypes.c_int, [_P, _P]),
+    "vxn_get_params": (ctypes.c_int, [_P, _P]),
+    "vxn_forward": (ctypes.c_int, [_P, _I32, _I32, _I32, _I32, _P, _P]),
+    "vxn_backward": (ctypes.c_int, [_P, _P, _P]),
+    "vxn_adam_step": (ctypes.c_int, [_P, _P, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                     ctypes.c_float, _I32]),
 }
 
 _lib = None
@@ -168,6 +179,14 @@ def check(rc, handle=None, allow=()):
     if rc == VX_ERR_STALL:
         raise StallError(rc, msg)
     raise NativeError(rc, msg)
+
+
+def check_net(rc, handle):
+    """Status check for the network handle (vxn_*)."""
+    if rc == VX_OK:
+        return rc
+    m = load().vxn_last_error(handle) if handle is not None else b""
+    raise NativeError(rc, m.decode() if m else "")
 
 
 def ptr(t):
