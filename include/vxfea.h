@@ -65,7 +65,9 @@ typedef struct {
   int32_t e_coarsen;          /* coarse-level E: 0 arithmetic mean of the 8 children
                                  (core/multigrid.py:287-290), 1 geometric, 2 harmonic */
   int32_t max_batch;          /* right-hand sides per vx_solve_batch call (workspace size) */
-  int32_t reserved[6];
+  int32_t flexible;           /* 1: flexible CG direction update (-z.Ap / p.Ap) also without a
+                                 corrector (the default update is Fletcher-Reeves) */
+  int32_t reserved[5];
 } vx_options;
 
 /* Per-solve statistics (mirrors MGSolver.last_* of core/multigrid.py:482-517). */
@@ -77,6 +79,9 @@ typedef struct {
   double resid_ratio;         /* last recurrence ratio ||r||/||f_free|| */
   double true_resid_ratio;    /* ||f_free - A u|| / ||f_free|| recomputed in FP64 */
   double solve_ms;            /* wall time of the solve on the handle's stream */
+  int32_t corrector;          /* learned corrector: 0 not attached, 1 used throughout,
+                                 2 switched off by the FP64 true-residual check */
+  int32_t reserved;
 } vx_info;
 
 /* Called synchronously on the caller's thread after every PCG iteration
@@ -225,6 +230,65 @@ int vxn_backward(vxn_handle n, const float* dout_dev, float* grad_dev);
 /* one Adam step (step counts from 1) on the parameters */
 int vxn_adam_step(vxn_handle n, const float* grad_dev, float lr, float beta1, float beta2,
                   float eps, int32_t step);
+
+/* Learned coarse-grid corrector inside the V-cycle (no counterpart in the
+ * reference; its V-cycle, core/multigrid.py:426-436, has none).  With a net
+ * attached and enabled, every VX_MIXED solve's FP32 V-cycle stops descending
+ * at `level`: there it pre-smooths and forms the residual t, computes the
+ * plain V-cycle's coarse-grid correction v (restrict, coarser levels,
+ * prolong) and the network's correction
+ *   y[3n+c] = gain * rms(t) * minv[n] * net(t / rms(t), v, features)[n][c]
+ * (free components; minv the level's inverse Jacobi diagonal), adds the
+ * combination alpha v + beta y of least energy error (a 2x2 Galerkin system
+ * per right-hand side, so never worse than v alone at that level), and
+ * post-smooths.  The outer CG then uses the flexible direction update.
+ * Every iterative-refinement step recomputes the FP64 true residual; if an
+ * inner solve with the corrector fails to halve it (or goes non-finite) the
+ * corrector is switched off for the rest of that solve and the plain V-cycle
+ * continues (vx_info.corrector = 2); so does an inner solve that uses up
+ * iter_budget iterations (> 0; e.g. the plain solve's count) without
+ * reaching its tolerance.  net = NULL detaches; gain <= 0 picks
+ * the largest node count along an axis of the level; enabled = 0 attaches
+ * for vx_corrector_prepare / vx_corrector_train only.  The net handle must
+ * outlive its use. */
+int vx_set_corrector(vx_handle h, vxn_handle net, int32_t level, float gain, int32_t enabled,
+                     int32_t iter_budget);
+
+/* What the attached network adds to the plain coarse-grid correction, per
+ * sample (t_i, v_i as from vx_corrector_prepare): the drop of the energy
+ * error ||A^-1 t - c||_A^2 from c = v to the best c = alpha v + beta y,
+ * divided by v.t (>= 0; 0 = nothing gained).  gain_host [nsamp]. */
+int vx_corrector_eval(vx_handle h, int32_t nsamp, const float* T_dev, const float* V_dev,
+                      double* gain_host);
+
+/* Record, during the following solves, the level-`level` residual after
+ * pre-smoothing of every `every`-th V-cycle (first right-hand side) into
+ * buf_dev [cap][3 * nodes of the level] float32, up to cap of them (the
+ * training inputs of the corrector).  buf_dev = NULL stops. */
+int vx_capture(vx_handle h, int32_t level, float* buf_dev, int32_t cap, int32_t every);
+int vx_capture_count(vx_handle h);
+
+/* Training pairs for the attached corrector's level: for each of the nsamp
+ * residuals t_i (T_dev [nsamp][3 * nodes] float32) the plain V-cycle's
+ * coarse-grid correction v_i (V_dev, same shape), the residual it leaves,
+ * r_i = t_i - A v_i (R_dev, same shape), and v_i . t_i (vt_dev [nsamp]
+ * float64). */
+int vx_corrector_prepare(vx_handle h, int32_t nsamp, const float* T_dev, float* V_dev,
+                         float* R_dev, double* vt_dev);
+
+/* Train the attached corrector on its level: `steps` Adam steps (learning
+ * rate lr with cosine decay to 5 %) on mini-batches of `batch` samples drawn
+ * by a seeded generator, minimising the mean over the batch of
+ *   (E(alpha v + beta y) - E(v)) / (v.t),   E(c) = ||A^-1 t - c||_A^2,
+ * the energy error of the correction the V-cycle applies (the best
+ * combination of v and the network's y) relative to v alone (<= 0).  Its
+ * gradient, -2 beta (t - alpha A v - beta A y) / (v.t), needs only the
+ * level's own operator.  The network must start from non-zero outputs (at
+ * y = 0 the gradient vanishes).  loss_host (may be NULL) receives the loss of
+ * every step. */
+int vx_corrector_train(vx_handle h, int32_t nsamp, const float* T_dev, const float* V_dev,
+                       const float* R_dev, const double* vt_dev, int32_t batch, int32_t steps,
+                       float lr, uint64_t seed, float* loss_host);
 
 /* Host-side element operator (the kernels' own code compiled for the CPU):
  * y24 = KE(nu) * u24 for one unit element in the reference's local DOF order

@@ -51,7 +51,8 @@ class vx_options(ctypes.Structure):
         ("z_chunk", ctypes.c_int32),
         ("e_coarsen", ctypes.c_int32),
         ("max_batch", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 6),
+        ("flexible", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 5),
     ]
 
 
@@ -64,6 +65,8 @@ class vx_info(ctypes.Structure):
         ("resid_ratio", ctypes.c_double),
         ("true_resid_ratio", ctypes.c_double),
         ("solve_ms", ctypes.c_double),
+        ("corrector", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -107,6 +110,13 @@ SIGNATURES = {
     "vx_drop_islands": (ctypes.c_int, [_P, _P, _P, _I64, _P, ctypes.POINTER(_I64)]),
     "vx_time_kernel": (ctypes.c_int, [_P, _I32, _I32, _I32, _I32, ctypes.POINTER(_D)]),
     "vx_element_apply_host": (None, [_D, _P, _P]),
+    "vx_set_corrector": (ctypes.c_int, [_P, _P, _I32, ctypes.c_float, _I32, _I32]),
+    "vx_corrector_eval": (ctypes.c_int, [_P, _I32, _P, _P, _P]),
+    "vx_capture": (ctypes.c_int, [_P, _I32, _P, _I32, _I32]),
+    "vx_capture_count": (ctypes.c_int, [_P]),
+    "vx_corrector_prepare": (ctypes.c_int, [_P, _I32, _P, _P, _P, _P]),
+    "vx_corrector_train": (ctypes.c_int, [_P, _I32, _P, _P, _P, _P, _I32, _I32, ctypes.c_float,
+                                          ctypes.c_uint64, _P]),
     "vxn_create": (ctypes.c_int, [_I32, _I32, _I32, ctypes.POINTER(_P)]),
     "vxn_destroy": (None, [_P]),
     "vxn_last_error": (ctypes.c_char_p, [_P]),

@@ -116,3 +116,49 @@ class CorrectorNet:
         self._bind()
         self._check(self._lib.vxn_adam_step(self._h, _native.ptr(self.grad), lr, beta1, beta2,
                                             eps, self.step))
+
+
+def train_corrector(mg, f_dev, level, net=None, n_samples=64, batch=4, steps=600, lr=3e-4,
+                    seed=0, tol=1e-6, channels=(16, 32, 64), every=None, out_scale=0.1):
+    """Train a corrector for ``mg`` (precision "mixed", density already set)
+    on residual -> correction pairs from its own plain MG-PCG:
+
+    1. a plain solve of ``f_dev`` (corrector off) counts the V-cycles;
+    2. a second plain solve records ``n_samples`` level residuals t spread
+       over the whole solve (every k-th V-cycle);
+    3. the plain V-cycle's coarse-grid correction v of each t (the pair) and
+       the residual it leaves, r = t - A v;
+    4. Adam, on the GPU, on the energy error of the correction the V-cycle
+       applies (the best combination of v and the network's y).
+
+    The trained net is attached (enabled) to ``mg`` with the plain solve's
+    iteration count as its budget; returns (net, report)."""
+    import time
+    import torch
+    net = net or CorrectorNet(*channels, seed=seed, out_scale=out_scale)
+    mg.detach_corrector()
+    u = torch.empty_like(f_dev)
+    t0 = time.perf_counter()
+    mg.solve_device(f_dev, u, tol=tol)
+    plain_iters = mg.last_iters
+    if every is None:
+        every = max(1, plain_iters // n_samples)
+    T = mg.capture(level, n_samples, every)
+    mg.solve_device(f_dev, u, tol=tol)
+    mg.stop_capture()
+    n = mg.capture_count()
+    T = T[:n].contiguous()
+    t1 = time.perf_counter()
+    mg.attach_corrector(net, level, enabled=False)
+    V, R, vt = mg.corrector_pairs(T)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    loss = mg.corrector_train(T, V, R, vt, batch=batch, steps=steps, lr=lr, seed=seed + 1)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    mg.attach_corrector(net, level, enabled=True, iter_budget=plain_iters)
+    report = {"level": level, "samples": n, "every": every, "plain_iters": plain_iters,
+              "loss_first": float(loss[:10].mean()), "loss_last": float(loss[-20:].mean()),
+              "capture_s": t1 - t0, "pairs_s": t2 - t1, "train_s": t3 - t2,
+              "steps": steps, "batch": batch, "lr": lr}
+    return net, report

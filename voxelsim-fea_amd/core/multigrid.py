@@ -53,7 +53,7 @@ class MGSolver:
                  min_elems=4, max_levels=6, compute_mode="AUTO", cpu_threads=0,
                  verbose=False, precision="fp64", coarse_sweeps=30, inner_tol=1e-5,
                  max_outer=20, z_chunk=0, smoother="jacobi", cheb_ratio=0.3,
-                 system="full", e_coarsen="mean", max_batch=1):
+                 system="full", e_coarsen="mean", max_batch=1, flexible=False):
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
         self.nelem = self.nx * self.ny * self.nz
         self.ndof = 3 * (self.nx + 1) * (self.ny + 1) * (self.nz + 1)
@@ -73,7 +73,7 @@ class MGSolver:
             inner_tol=inner_tol, max_outer=max_outer, z_chunk=z_chunk,
             smoother=_SMOOTHER[smoother], cheb_ratio=cheb_ratio,
             e_coarsen={"mean": 0, "geometric": 1, "harmonic": 2}[e_coarsen],
-            max_batch=max_batch)
+            max_batch=max_batch, flexible=1 if flexible else 0)
         self.max_batch = max_batch
         h = ctypes.c_void_p()
         _native.check(self._lib.vx_create(self.nx, self.ny, self.nz, float(nu),
@@ -158,6 +158,7 @@ class MGSolver:
         self.last_true_resid_ratio = info.true_resid_ratio
         self.last_outer_iters = info.outer_iters
         self.last_solve_ms = info.solve_ms
+        self.last_corrector = {0: "none", 1: "used", 2: "fell back"}[info.corrector]
         self.resid_history = [float(v) for v in hist[:min(info.iters, hist_cap)]]
         self._check(rc, allow=(_native.VX_ERR_NOT_CONVERGED,))
         return rc
@@ -182,7 +183,8 @@ class MGSolver:
                                       hist.ctypes.data_as(ctypes.c_void_p), hist_cap, infos)
         self.last_batch = [{"iters": i.iters, "converged": bool(i.converged),
                             "outer_iters": i.outer_iters, "resid_ratio": i.resid_ratio,
-                            "true_resid_ratio": i.true_resid_ratio} for i in infos]
+                            "true_resid_ratio": i.true_resid_ratio,
+                            "corrector": i.corrector} for i in infos]
         self.batch_history = [list(hist[q * hist_cap:q * hist_cap + min(infos[q].iters, hist_cap)])
                               for q in range(nrhs)]
         self.last_solve_ms = infos[0].solve_ms if nrhs else float("nan")
@@ -232,6 +234,76 @@ class MGSolver:
             print(f"[VoxelSim FEA] MG solve: {self.last_iters} CG iters "
                   f"(converged={self.last_converged}), {self.last_solve_ms:.1f} ms on device")
         return u_dev.cpu().numpy()
+
+    # -- learned coarse-grid corrector --------------------------------------
+    def level_nodes(self, li):
+        a, b, c = self.levels[li]["dims"]
+        return (a + 1) * (b + 1) * (c + 1)
+
+    def attach_corrector(self, net, level, gain=0.0, enabled=True, iter_budget=0):
+        """Use ``net`` (core.learned.CorrectorNet) in the coarse-grid
+        correction at ``level`` of the FP32 V-cycle (precision="mixed");
+        ``iter_budget`` > 0 falls back to the plain V-cycle after that many
+        inner iterations without convergence."""
+        self._stream_bind()
+        self._check(self._lib.vx_set_corrector(self._h, net.handle, int(level), float(gain),
+                                                1 if enabled else 0, int(iter_budget)))
+        self._corrector = net
+
+    def detach_corrector(self):
+        self._check(self._lib.vx_set_corrector(self._h, None, 0, 0.0, 0, 0))
+        self._corrector = None
+
+    def corrector_eval(self, T, V):
+        """Per sample: energy-error drop (/ v.t) of the best combination of
+        the plain correction and the attached network's, over the plain one."""
+        out = np.zeros(int(T.shape[0]))
+        self._stream_bind()
+        self._check(self._lib.vx_corrector_eval(self._h, int(T.shape[0]), _native.ptr(T),
+                                                 _native.ptr(V),
+                                                 out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    def capture(self, level, cap, every=1):
+        """Record the level residuals of the next solves; returns the buffer
+        [cap, 3 * nodes] (valid rows: ``capture_count()``)."""
+        torch = _torch()
+        buf = torch.zeros(int(cap), 3 * self.level_nodes(level), dtype=torch.float32,
+                          device="cuda")
+        self._check(self._lib.vx_capture(self._h, int(level), _native.ptr(buf), int(cap),
+                                         int(every)))
+        self._capture_buf = buf
+        return buf
+
+    def capture_count(self):
+        return int(self._lib.vx_capture_count(self._h))
+
+    def stop_capture(self):
+        self._check(self._lib.vx_capture(self._h, 0, None, 0, 1))
+
+    def corrector_pairs(self, T):
+        """Plain coarse-grid corrections V, residuals after them R = T - A V
+        and v.t for the captured level residuals T (FP32 V-cycle)."""
+        torch = _torch()
+        V = torch.empty_like(T)
+        R = torch.empty_like(T)
+        vt = torch.empty(int(T.shape[0]), dtype=torch.float64, device="cuda")
+        self._stream_bind()
+        self._check(self._lib.vx_corrector_prepare(self._h, int(T.shape[0]), _native.ptr(T),
+                                                    _native.ptr(V), _native.ptr(R),
+                                                    _native.ptr(vt)))
+        return V, R, vt
+
+    def corrector_train(self, T, V, R, vt, batch=4, steps=200, lr=1e-3, seed=0):
+        """Adam training of the attached corrector; returns the loss per step."""
+        loss = np.zeros(int(steps), dtype=np.float32)
+        self._stream_bind()
+        self._check(self._lib.vx_corrector_train(self._h, int(T.shape[0]), _native.ptr(T),
+                                                  _native.ptr(V), _native.ptr(R),
+                                                  _native.ptr(vt), int(batch), int(steps),
+                                                  float(lr), int(seed),
+                                                  loss.ctypes.data_as(ctypes.c_void_p)))
+        return loss
 
     # -- single operators (parity tests) -----------------------------------
     def _level_vec(self, li, a):

@@ -39,6 +39,7 @@ struct Scalars {
   double fnorm;
   double tmp;
   double tmp2;
+  double cs;       // corrector: sum of squares of its input residual
   int frozen;      // 1: this right-hand side has stopped (alpha = 0)
   int pad;
 };
@@ -713,7 +714,10 @@ enum {
   F_RZ0 = 2,        // s.rz = sum
   F_ALPHA = 3,      // s.pAp = sum; s.alpha = sum == 0 ? 0 : rz / sum
   F_RATIO = 4,      // s.rr = sum; s.ratio = sqrt(sum) / fnorm
-  F_BETA = 5        // s.rz_new = sum; s.beta = rz == 0 ? 0 : sum / rz; s.rz = sum
+  F_BETA = 5,       // s.rz_new = sum; s.beta = rz == 0 ? 0 : sum / rz; s.rz = sum
+  F_SET_CS = 6,     // s.cs = sum
+  F_RZNEW = 7,      // s.rz_new = sum
+  F_BETA_FLEX = 8   // sum = z.Ap: s.beta = -sum / pAp (flexible CG); s.rz = s.rz_new
 };
 
 __global__ void k_finalize(const double* __restrict__ partial, int n, int op, Scalars* s) {
@@ -744,6 +748,12 @@ __global__ void k_finalize(const double* __restrict__ partial, int n, int op, Sc
         s->rz_new = v;
         s->beta = (s->rz == 0.0) ? 0.0 : v / s->rz;
         s->rz = v;
+        break;
+      case F_SET_CS: s->cs = v; break;
+      case F_RZNEW: s->rz_new = v; break;
+      case F_BETA_FLEX:
+        s->beta = (s->pAp == 0.0) ? 0.0 : -v / s->pAp;
+        s->rz = s->rz_new;
         break;
     }
   }
@@ -1098,6 +1108,274 @@ __global__ void k_count_u8(long long n, const uint8_t* __restrict__ a, unsigned 
     c += a[i] ? 1 : 0;
   for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o, 64);
   if ((threadIdx.x & 63) == 0 && c) atomicAdd(cnt, c);
+}
+
+// ---------------------------------------------------------------------------
+// Learned coarse-grid corrector.  At the corrector level the V-cycle forms
+// the residual t, the plain coarse-grid correction v (restrict, coarser
+// levels, prolong) and the network's correction y, and adds the combination
+// alpha v + beta y with the smallest energy error ||A^-1 t - alpha v - beta y||_A
+// (a 2x2 Galerkin system per right-hand side).  Per node the network sees 16
+// BF16 channels:
+//   0-2  t / rms(t),
+//   3    mean E of the adjacent elements / max E of the level,
+//   4    1 + log2(max(that ratio, 2^-20)) / 20  (0 where no element is stiff),
+//   5    fraction of free components (0, 1/3, 2/3, 1),
+//   6-8  v / (gain * rms(t) * minv[n])   (v in the units of the output),
+//   9-15 zero,
+// and its first 3 output channels give y[3n+c] = gain * rms * minv[n] * out[n][c]
+// on free components.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint16_t f2bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+// max of non-negative floats: their bit patterns order like the values
+__global__ void k_max_nonneg(long long n, const float* __restrict__ a, unsigned int* out) {
+  __shared__ unsigned int red[256];
+  unsigned int m = 0u;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    m = max(m, __float_as_uint(fmaxf(a[i], 0.f)));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(out, red[0]);
+}
+
+// per-node features [Nn][4] of a level (E internal order, x fastest)
+__global__ void k_net_feat(int nx, int ny, int nz, const float* __restrict__ E,
+                           const uint8_t* __restrict__ mask, const unsigned int* __restrict__ emax,
+                           float* __restrict__ feat) {
+  const int Lx = nx + 1, Ly = ny + 1, Lz = nz + 1;
+  const long long Nn = (long long)Lx * Ly * Lz;
+  const float em = __uint_as_float(*emax);
+  for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < Nn;
+       n += (long long)gridDim.x * blockDim.x) {
+    const int ix = (int)(n % Lx), iy = (int)((n / Lx) % Ly), iz = (int)(n / ((long long)Lx * Ly));
+    float sum = 0.f;
+    int cnt = 0;
+    for (int dz = -1; dz <= 0; ++dz)
+      for (int dy = -1; dy <= 0; ++dy)
+        for (int dx = -1; dx <= 0; ++dx) {
+          const int x = ix + dx, y = iy + dy, z = iz + dz;
+          if (x < 0 || x >= nx || y < 0 || y >= ny || z < 0 || z >= nz) continue;
+          sum += E[((long long)z * ny + y) * nx + x];
+          ++cnt;
+        }
+    const float r = (cnt > 0 && em > 0.f) ? sum / (float)cnt / em : 0.f;
+    feat[4 * n + 0] = r;
+    feat[4 * n + 1] = r > 0.f ? 1.f + log2f(fmaxf(r, 9.5367431640625e-07f)) / 20.f : 0.f;
+    feat[4 * n + 2] = (float)__popc((unsigned)mask[n] & 7u) / 3.f;
+    feat[4 * n + 3] = 0.f;
+  }
+}
+
+// rms[r] = sqrt(cs / ndof) per right-hand side
+__global__ void k_net_rms(const Scalars* __restrict__ sc, int nb, double ndof,
+                          float* __restrict__ rms) {
+  for (int r = threadIdx.x; r < nb; r += blockDim.x) rms[r] = (float)sqrt(sc[r].cs / ndof);
+}
+
+// rms[r] = sqrt(sum of partials / ndof); one work-group per sample
+__global__ void k_rms_from_partials(const double* __restrict__ part, int np, double ndof,
+                                    float* __restrict__ rms) {
+  __shared__ double sred[4];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) v += part[(long long)blockIdx.x * np + i];
+  v = block_sum(v, sred);
+  if (threadIdx.x == 0) rms[blockIdx.x] = (float)sqrt(v / ndof);
+}
+
+// network input [nb][Nn][16] bf16 from t and v [nb][3 Nn] (blockIdx.y = rhs)
+__global__ void k_net_input(long long Nn, const float* __restrict__ t, const float* __restrict__ v,
+                            const float* __restrict__ feat, const float* __restrict__ minv,
+                            const float* __restrict__ rms, float gain, uint16_t* __restrict__ in) {
+  const int r = blockIdx.y;
+  const float inv = rms[r] > 0.f ? 1.f / rms[r] : 0.f;
+  t += (long long)r * 3 * Nn;
+  v += (long long)r * 3 * Nn;
+  in += (long long)r * Nn * 16;
+  for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < Nn;
+       n += (long long)gridDim.x * blockDim.x) {
+    const float den = gain * rms[r] * minv[n];
+    const float iv = den > 0.f ? 1.f / den : 0.f;
+    const uint32_t c0 = f2bf16(t[3 * n] * inv), c1 = f2bf16(t[3 * n + 1] * inv);
+    const uint32_t c2 = f2bf16(t[3 * n + 2] * inv);
+    const uint32_t c3 = f2bf16(feat[4 * n]), c4 = f2bf16(feat[4 * n + 1]);
+    const uint32_t c5 = f2bf16(feat[4 * n + 2]);
+    const uint32_t c6 = f2bf16(v[3 * n] * iv), c7 = f2bf16(v[3 * n + 1] * iv);
+    const uint32_t c8 = f2bf16(v[3 * n + 2] * iv);
+    uint4* o = reinterpret_cast<uint4*>(in + n * 16);
+    o[0] = make_uint4(c0 | (c1 << 16), c2 | (c3 << 16), c4 | (c5 << 16), c6 | (c7 << 16));
+    o[1] = make_uint4(c8, 0u, 0u, 0u);
+  }
+}
+
+// partials of the 5 products v.Av, v.Ay, y.Ay, v.t, y.t per right-hand side:
+// part[(r * 5 + k) * gridDim.x + block]
+__global__ void k_dot5(long long n3, const float* __restrict__ v, const float* __restrict__ y,
+                       const float* __restrict__ Av, const float* __restrict__ Ay,
+                       const float* __restrict__ t, double* __restrict__ part) {
+  __shared__ double sred[4];
+  const int r = blockIdx.y;
+  const long long o = (long long)r * n3;
+  double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n3;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double vi = v[o + i], yi = y[o + i], avi = Av[o + i], ayi = Ay[o + i], ti = t[o + i];
+    a[0] += vi * avi;
+    a[1] += vi * ayi;
+    a[2] += yi * ayi;
+    a[3] += vi * ti;
+    a[4] += yi * ti;
+  }
+  for (int k = 0; k < 5; ++k) {
+    const double sum = block_sum(a[k], sred);
+    if (threadIdx.x == 0) part[((long long)r * 5 + k) * gridDim.x + blockIdx.x] = sum;
+    __syncthreads();
+  }
+}
+
+// per right-hand side (one work-group each): the energy-optimal (alpha, beta)
+// of [v.Av v.Ay; v.Ay y.Ay] (alpha, beta) = (v.t, y.t); beta = 0 when y adds
+// no independent direction
+__global__ void k_combine_coef(const double* __restrict__ part, int np, double* __restrict__ coef) {
+  __shared__ double sred[4];
+  const int r = blockIdx.x;
+  double m[5];
+  for (int k = 0; k < 5; ++k) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) v += part[((long long)r * 5 + k) * np + i];
+    m[k] = block_sum(v, sred);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double a = m[0], b = m[1], c = m[2], d = m[3], e = m[4];
+    const double det = a * c - b * b;
+    double al = a > 0.0 ? d / a : 0.0, be = 0.0;
+    if (a > 0.0 && c > 0.0 && det > 1e-10 * a * c) {
+      al = (c * d - b * e) / det;
+      be = (a * e - b * d) / det;
+    } else if (!(a > 0.0) && c > 0.0) {
+      al = 0.0;
+      be = e / c;
+    }
+    coef[2 * r] = al;
+    coef[2 * r + 1] = be;
+  }
+}
+
+// x += alpha v + beta y per right-hand side (blockIdx.y)
+__global__ void k_axpy2(long long n3, const double* __restrict__ coef, const float* __restrict__ v,
+                        const float* __restrict__ y, float* __restrict__ x) {
+  const int r = blockIdx.y;
+  const float al = (float)coef[2 * r], be = (float)coef[2 * r + 1];
+  const long long o = (long long)r * n3;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n3;
+       i += (long long)gridDim.x * blockDim.x)
+    x[o + i] += al * v[o + i] + be * y[o + i];
+}
+
+// r = t - Av
+__global__ void k_sub(long long n, const float* __restrict__ t, const float* __restrict__ Av,
+                      float* __restrict__ r) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    r[i] = t[i] - Av[i];
+}
+
+// x[3n+c] (+)= gain * rms * minv[n] * out[n][c] on free components (blockIdx.y = rhs)
+__global__ void k_net_apply(long long Nn, const float* __restrict__ out,
+                            const float* __restrict__ minv, const uint8_t* __restrict__ mask,
+                            const float* __restrict__ rms, float gain, float* __restrict__ x,
+                            int accumulate) {
+  const int r = blockIdx.y;
+  const float sc = gain * rms[r];
+  out += (long long)r * Nn * 16;
+  x += (long long)r * 3 * Nn;
+  for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < Nn;
+       n += (long long)gridDim.x * blockDim.x) {
+    const unsigned mk = mask[n];
+    const float w = sc * minv[n];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float v = ((mk >> c) & 1u) ? w * out[n * 16 + c] : 0.f;
+      if (accumulate) x[3 * n + c] += v;
+      else x[3 * n + c] = v;
+    }
+  }
+}
+
+// chain rule through k_net_apply: dout[n][c] = gain * rms * minv[n] * gy[3n+c]
+// on free components, 0 elsewhere and on channels 3-15
+__global__ void k_net_apply_bwd(long long Nn, const float* __restrict__ gy,
+                                const float* __restrict__ minv, const uint8_t* __restrict__ mask,
+                                const float* __restrict__ rms, float gain,
+                                float* __restrict__ dout) {
+  const int r = blockIdx.y;
+  const float sc = gain * rms[r];
+  gy += (long long)r * 3 * Nn;
+  dout += (long long)r * Nn * 16;
+  for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < Nn;
+       n += (long long)gridDim.x * blockDim.x) {
+    const unsigned mk = mask[n];
+    const float w = sc * minv[n];
+    float4* o = reinterpret_cast<float4*>(dout + n * 16);
+    o[0] = make_float4(((mk >> 0) & 1u) ? w * gy[3 * n] : 0.f,
+                       ((mk >> 1) & 1u) ? w * gy[3 * n + 1] : 0.f,
+                       ((mk >> 2) & 1u) ? w * gy[3 * n + 2] : 0.f, 0.f);
+    o[1] = make_float4(0.f, 0.f, 0.f, 0.f);
+    o[2] = o[1];
+    o[3] = o[1];
+  }
+}
+
+// Training loss per sample: the energy of the error left by the correction
+// the V-cycle applies, the best combination alpha v + beta y (k_combine_coef),
+// relative to the plain correction v alone and to v.t:
+//   l = (E(alpha v + beta y) - E(v)) / (v.t),  E(c) = ||A^-1 t - c||_A^2
+// (<= 0; from the five products of k_dot5, without A^-1 t).  By the envelope
+// theorem its gradient w.r.t. y is -2 beta (t - alpha A v - beta A y) / (v.t).
+__global__ void k_train_gy(long long n3, const float* __restrict__ t, const float* __restrict__ Av,
+                           const float* __restrict__ Ay, const double* __restrict__ coef,
+                           const double* __restrict__ vt, float* __restrict__ gy) {
+  const int r = blockIdx.y;
+  const float al = (float)coef[2 * r], be = (float)coef[2 * r + 1];
+  const float sc = (float)(-2.0 * coef[2 * r + 1] / vt[r]);
+  const long long o = (long long)r * n3;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n3;
+       i += (long long)gridDim.x * blockDim.x)
+    gy[o + i] = sc * (t[o + i] - al * Av[o + i] - be * Ay[o + i]);
+}
+
+// mean over the B samples of l (one work-group); part as written by k_dot5
+__global__ void k_train_loss(const double* __restrict__ part, int np, int B,
+                             const double* __restrict__ coef, double* __restrict__ loss) {
+  __shared__ double sred[4];
+  double tot = 0.0;
+  for (int r = 0; r < B; ++r) {
+    double m[5];
+    for (int k = 0; k < 5; ++k) {
+      double v = 0.0;
+      for (int i = threadIdx.x; i < np; i += blockDim.x) v += part[((long long)r * 5 + k) * np + i];
+      m[k] = block_sum(v, sred);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0 && m[3] > 0.0) {
+      const double al = coef[2 * r], be = coef[2 * r + 1];
+      const double ec = -2.0 * (al * m[3] + be * m[4]) + al * al * m[0] +
+                        2.0 * al * be * m[1] + be * be * m[2];
+      const double ev = -2.0 * m[3] + m[0];
+      tot += (ec - ev) / m[3];
+    }
+  }
+  if (threadIdx.x == 0) *loss = tot / B;
 }
 
 }  // namespace vx

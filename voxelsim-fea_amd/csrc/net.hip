@@ -21,6 +21,7 @@
 #include <string>
 #include <vector>
 
+#include "net_internal.h"
 #include "vxfea.h"
 
 namespace vxnet {
@@ -472,9 +473,12 @@ int ensure_act(vx_net* n, int B, int X, int Y, int Z) {
   NCHK(nalloc(n, &n->a2, sz(2, n->c[2])));
   NCHK(nalloc(n, &n->a1, sz(1, n->c[1])));
   NCHK(nalloc(n, &n->a0, sz(0, n->c[0])));
+  // widest tensor per level: input 16 / c0 (0), c0 / c1 (1), c1 / c2 (2), c2 (3)
+  const int wide[4] = {std::max(16, n->c[0]), std::max(n->c[0], n->c[1]),
+                       std::max(n->c[1], n->c[2]), n->c[2]};
   for (int l = 0; l < 4; ++l) {
-    NCHK(nalloc(n, &n->dtmp[l], sz(l, 64)));
-    NCHK(nalloc(n, &n->g[l], sz(l, 64)));
+    NCHK(nalloc(n, &n->dtmp[l], sz(l, wide[l])));
+    NCHK(nalloc(n, &n->g[l], sz(l, wide[l])));
   }
   for (int l = 0; l < 3; ++l) NCHK(nalloc(n, &n->ds[l], sz(l, n->c[l])));
   n->have_act = true;
@@ -542,11 +546,8 @@ int up_add(vx_net* n, const uint16_t* h, int lc, const uint16_t* skip, int C, ui
   return VX_OK;
 }
 
-int forward(vx_net* n, const float* in_f, float* out_f) {
+int forward_from_input(vx_net* n, float* out_f) {
   if (!n->packed) NCHK(pack_all(n));
-  const long long nin = n->V[0] * n->B * 16;
-  hipLaunchKernelGGL(k_f32_to_bf16, dim3(grid1(nin)), dim3(256), 0, n->stream, in_f, nin, n->in);
-  NLAUNCH();
   NCHK(layer_fwd(n, 0, n->in, nullptr));
   for (int i = 1; i < 8; ++i) NCHK(layer_fwd(n, i, n->h[i - 1], nullptr));
   NCHK(up_add(n, n->h[7], 3, n->h[5], n->c[2], n->a2));
@@ -558,6 +559,13 @@ int forward(vx_net* n, const float* in_f, float* out_f) {
   NCHK(up_add(n, n->h[11], 1, n->h[1], n->c[0], n->a0));
   NCHK(layer_fwd(n, 12, n->a0, nullptr));
   return layer_fwd(n, 13, n->h[12], out_f);
+}
+
+int forward(vx_net* n, const float* in_f, float* out_f) {
+  const long long nin = n->V[0] * n->B * 16;
+  hipLaunchKernelGGL(k_f32_to_bf16, dim3(grid1(nin)), dim3(256), 0, n->stream, in_f, nin, n->in);
+  NLAUNCH();
+  return forward_from_input(n, out_f);
 }
 
 // backward of layer i: d = gradient w.r.t. its output (fp32), x = its input;
@@ -650,6 +658,21 @@ int backward(vx_net* n, const float* dout, float* grad) {
 }
 
 }  // namespace
+
+namespace vxnet_api {
+int prepare(vx_net* n, hipStream_t st, int B, int X, int Y, int Z) {
+  n->stream = st;
+  return ensure_act(n, B, X, Y, Z);
+}
+uint16_t* input(vx_net* n) { return n->in; }
+int forward_bf16(vx_net* n, float* out) { return forward_from_input(n, out); }
+int backward(vx_net* n, const float* dout, float* grad) { return ::backward(n, dout, grad); }
+int adam(vx_net* n, const float* grad, float lr, float b1, float b2, float eps, int step) {
+  return vxn_adam_step(n, grad, lr, b1, b2, eps, step);
+}
+long long num_params(const vx_net* n) { return n->nparams; }
+const char* error(const vx_net* n) { return n->err.c_str(); }
+}  // namespace vxnet_api
 
 extern "C" {
 

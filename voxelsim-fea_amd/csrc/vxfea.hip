@@ -11,6 +11,7 @@
 #include <array>
 #include <chrono>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -18,6 +19,7 @@
 #include <vector>
 
 #include "kernels.h"
+#include "net_internal.h"
 #include "vxfea.h"
 
 using namespace vx;
@@ -88,6 +90,26 @@ struct vx_solver {
   Scalars* hscal = nullptr;   // pinned mirror
   int* dflags = nullptr;      // [max_batch] frozen flags staging
   int* hflags = nullptr;      // pinned
+  // learned coarse-grid corrector (vx_set_corrector)
+  vx_net* corr = nullptr;
+  int corr_level = -1;
+  float corr_gain = 1.f;
+  int corr_budget = 0;        // inner iterations allowed per learned solve (0: no limit)
+  bool corr_enabled = false;  // attached and used by solves
+  bool corr_on = false;       // active inside the current solve (the fallback clears it)
+  float* corr_feat = nullptr; // [Nn][4] of the corrector level
+  float* corr_out = nullptr;  // [max_batch][Nn][16]
+  float *corr_v = nullptr, *corr_y = nullptr, *corr_Av = nullptr, *corr_Ay = nullptr;
+                              // [max_batch][3 Nn] each
+  double* corr_part = nullptr;  // [max_batch][5][EW_BLOCKS]
+  double* corr_coef = nullptr;  // [max_batch][2]
+  float* corr_rms = nullptr;  // [max_batch]
+  unsigned int* corr_emax = nullptr;
+  long long corr_cap_nn = 0;  // nodes the corrector buffers were sized for
+  // capture of corrector-level residuals during solves (vx_capture)
+  float* cap_buf = nullptr;
+  int cap_level = -1, cap_cap = 0, cap_count = 0, cap_every = 1;
+  long long cap_calls = 0;
   std::vector<void*> allocs;
 };
 
@@ -376,51 +398,162 @@ int coarse_fused(vx_solver* s, Level& L, const T* b, T* x) {
   return VX_OK;
 }
 
-// V-cycle (core/multigrid.py:426-436): b at level 0 -> result in z0.
-// t0 and w0 are level-0 scratch vectors (distinct from b0, z0).
-// dot: partials of z0.b0 from the last level-0 write.
 template <typename T>
-int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, T* w0, bool dot, int* n_part) {
+int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, T* w0, bool dot, int* n_part, int l0);
+
+// The plain V-cycle's coarse-grid correction of level li for residual t:
+// restrict, V-cycle from level li+1, prolong (v = 0 + P x); at the coarsest
+// level, its smoothing from zero.  tmp, tmp2: level-li scratch.
+template <typename T>
+int plain_coarse(vx_solver* s, int li, const T* t, T* v, T* tmp, T* tmp2) {
+  const int nl = (int)s->lv.size();
+  Level& L = s->lv[li];
+  const int nb = s->nb;
+  if (li == nl - 1) {
+    const int k = s->opt.coarse_sweeps;
+    if (li > 0 && L.Nn <= coarse_max_nodes<T>() && L.Ne <= coarse_max_elems<T>() &&
+        k <= COARSE_MAX_STEPS)
+      return coarse_fused<T>(s, L, t, v);
+    return smooth_step<T>(s, li, nullptr, t, k, v, tmp, tmp2, false, nullptr);
+  }
+  Level& C = s->lv[li + 1];
+  hipLaunchKernelGGL(k_restrict3<T>, dim3((C.Lx + 63) / 64, C.Ly, C.Lz * nb), dim3(64), 0,
+                     s->stream, C.Lx, C.Ly, C.Lz, t, C.mask, Sel<T>::b(C));
+  LAUNCHCHK();
+  CHK(vcycle<T>(s, Sel<T>::b(C), Sel<T>::x(C), Sel<T>::t(C), Sel<T>::w(C), false, nullptr,
+                li + 1));
+  HIPCHK(hipMemsetAsync(v, 0, sizeof(T) * 3 * L.Nn * nb, s->stream));
+  hipLaunchKernelGGL(k_prolong_add3<T>, dim3((L.Lx + 63) / 64, L.Ly, L.Lz * nb), dim3(64), 0,
+                     s->stream, L.Lx, L.Ly, L.Lz, Sel<T>::x(C), L.mask, v);
+  LAUNCHCHK();
+  return VX_OK;
+}
+
+// network correction y [nb][3 Nn] at level li from t and v (FP32)
+int corrector_net(vx_solver* s, int li, const float* t, const float* v, float* y, float* rms) {
+  Level& L = s->lv[li];
+  int rc = vxnet_api::prepare(s->corr, s->stream, s->nb, L.Lx, L.Ly, L.Lz);
+  if (rc != VX_OK) return fail(s, rc, std::string("corrector: ") + vxnet_api::error(s->corr));
+  hipLaunchKernelGGL(k_net_input, dim3(ew_grid(L.Nn), s->nb), dim3(EW_THREADS), 0, s->stream,
+                     L.Nn, t, v, s->corr_feat, L.mi32, rms, s->corr_gain,
+                     vxnet_api::input(s->corr));
+  LAUNCHCHK();
+  rc = vxnet_api::forward_bf16(s->corr, s->corr_out);
+  if (rc != VX_OK) return fail(s, rc, std::string("corrector: ") + vxnet_api::error(s->corr));
+  hipLaunchKernelGGL(k_net_apply, dim3(ew_grid(L.Nn), s->nb), dim3(EW_THREADS), 0, s->stream,
+                     L.Nn, s->corr_out, L.mi32, L.mask, rms, s->corr_gain, y, 0);
+  LAUNCHCHK();
+  return VX_OK;
+}
+
+// Learned coarse-grid correction at level li for the s->nb right-hand sides:
+// x += alpha v + beta y, the energy-optimal combination of the plain
+// correction v and the network's y.  FP32 work only.
+int corrector_apply(vx_solver* s, int li, const float* t, float* x) {
+  Level& L = s->lv[li];
+  const long long n3 = 3 * L.Nn;
+  const int nb = s->nb;
+  // (A v and A y are formed later: their buffers are the scratch here)
+  CHK(plain_coarse<float>(s, li, t, s->corr_v, s->corr_Av, s->corr_Ay));
+  int np = 0;
+  CHK(dot_partials<float>(s, n3, t, t, &np));
+  CHK(finalize(s, np, F_SET_CS));
+  hipLaunchKernelGGL(k_net_rms, dim3(1), dim3(64), 0, s->stream, s->dscal, nb, (double)n3,
+                     s->corr_rms);
+  LAUNCHCHK();
+  CHK(corrector_net(s, li, t, s->corr_v, s->corr_y, s->corr_rms));
+  CHK((launch_tile_L<float, M_MATVEC, false>(s, L, s->corr_v, nullptr, s->corr_Av, nullptr,
+                                             nullptr)));
+  CHK((launch_tile_L<float, M_MATVEC, false>(s, L, s->corr_y, nullptr, s->corr_Ay, nullptr,
+                                             nullptr)));
+  hipLaunchKernelGGL(k_dot5, dim3(EW_BLOCKS, nb), dim3(EW_THREADS), 0, s->stream, n3, s->corr_v,
+                     s->corr_y, s->corr_Av, s->corr_Ay, t, s->corr_part);
+  hipLaunchKernelGGL(k_combine_coef, dim3(nb), dim3(256), 0, s->stream, s->corr_part, EW_BLOCKS,
+                     s->corr_coef);
+  hipLaunchKernelGGL(k_axpy2, dim3(ew_grid(n3), nb), dim3(EW_THREADS), 0, s->stream, n3,
+                     s->corr_coef, s->corr_v, s->corr_y, x);
+  LAUNCHCHK();
+  return VX_OK;
+}
+
+// store the level residual of the first right-hand side (vx_capture)
+template <typename T>
+int capture_residual(vx_solver* s, int li, const T* t) {
+  if (!s->cap_buf || li != s->cap_level || s->cap_count >= s->cap_cap) return VX_OK;
+  if ((s->cap_calls++ % s->cap_every) != 0) return VX_OK;
+  const long long n3 = 3 * s->lv[li].Nn;
+  float* dst = s->cap_buf + (long long)s->cap_count * n3;
+  if (std::is_same<T, float>::value) {
+    HIPCHK(hipMemcpyAsync(dst, t, sizeof(float) * n3, hipMemcpyDeviceToDevice, s->stream));
+  } else {
+    hipLaunchKernelGGL((k_convert<T, float>), dim3(ew_grid(n3)), dim3(EW_THREADS), 0, s->stream,
+                       n3, t, dst);
+    LAUNCHCHK();
+  }
+  ++s->cap_count;
+  return VX_OK;
+}
+
+// V-cycle (core/multigrid.py:426-436) from level l0: b at level l0 -> result
+// in z0.  t0 and w0 are level-l0 scratch vectors (distinct from b0, z0).
+// dot: partials of z0.b0 from the last level-l0 write.  With an attached
+// corrector (FP32 work) the cycle bottoms out at the corrector level, whose
+// coarse-grid correction (restrict, coarser levels, prolong) is replaced by
+// the network's correction between its pre- and post-smoothing.
+template <typename T>
+int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, T* w0, bool dot, int* n_part, int l0) {
   const int nl = (int)s->lv.size();
   const int ns = s->opt.n_smooth;
   const int nb = s->nb;
+  const bool learned = std::is_same<T, float>::value && s->corr_on && s->corr_level >= l0 &&
+                       s->corr_level < nl;
+  const int bottom = learned ? s->corr_level : nl - 1;
   // descend
-  for (int li = 0; li < nl; ++li) {
+  for (int li = l0; li <= bottom; ++li) {
     Level& L = s->lv[li];
-    const T* b = li == 0 ? b0 : Sel<T>::b(L);
-    T* x = li == 0 ? z0 : Sel<T>::x(L);
-    T* t = li == 0 ? t0 : Sel<T>::t(L);
-    T* w = li == 0 ? w0 : Sel<T>::w(L);
+    const T* b = li == l0 ? b0 : Sel<T>::b(L);
+    T* x = li == l0 ? z0 : Sel<T>::x(L);
+    T* t = li == l0 ? t0 : Sel<T>::t(L);
+    T* w = li == l0 ? w0 : Sel<T>::w(L);
+    if (li == bottom && learned) {
+      CHK(smooth_step<T>(s, li, nullptr, b, ns, x, t, w, false, nullptr));
+      CHK((launch_tile<T, M_RESID, false>(s, li, x, b, t, nullptr, nullptr)));
+      if (l0 == 0) CHK(capture_residual<T>(s, li, t));
+      CHK(corrector_apply(s, li, reinterpret_cast<const float*>(t), reinterpret_cast<float*>(x)));
+      CHK(smooth_step<T>(s, li, x, b, ns, x, t, w, dot && li == l0, n_part));
+      break;
+    }
     if (li == nl - 1) {
       const int k = s->opt.coarse_sweeps;
       if (li > 0 && L.Nn <= coarse_max_nodes<T>() && L.Ne <= coarse_max_elems<T>() &&
           k <= COARSE_MAX_STEPS) {
         CHK(coarse_fused<T>(s, L, b, x));
       } else {
-        CHK(smooth_step<T>(s, li, nullptr, b, k, x, t, w, dot && li == 0, n_part));
+        CHK(smooth_step<T>(s, li, nullptr, b, k, x, t, w, dot && li == l0, n_part));
       }
       break;
     }
     CHK(smooth_step<T>(s, li, nullptr, b, ns, x, t, w, false, nullptr));
     // residual into t, then restrict into b of level li+1
     CHK((launch_tile<T, M_RESID, false>(s, li, x, b, t, nullptr, nullptr)));
+    if (l0 == 0) CHK(capture_residual<T>(s, li, t));
     Level& C = s->lv[li + 1];
     hipLaunchKernelGGL(k_restrict3<T>, dim3((C.Lx + 63) / 64, C.Ly, C.Lz * nb), dim3(64), 0,
                        s->stream, C.Lx, C.Ly, C.Lz, t, C.mask, Sel<T>::b(C));
     LAUNCHCHK();
   }
   // ascend
-  for (int li = nl - 2; li >= 0; --li) {
+  for (int li = bottom - 1; li >= l0; --li) {
     Level& L = s->lv[li];
     Level& C = s->lv[li + 1];
-    const T* b = li == 0 ? b0 : Sel<T>::b(L);
-    T* x = li == 0 ? z0 : Sel<T>::x(L);
-    T* t = li == 0 ? t0 : Sel<T>::t(L);
-    T* w = li == 0 ? w0 : Sel<T>::w(L);
+    const T* b = li == l0 ? b0 : Sel<T>::b(L);
+    T* x = li == l0 ? z0 : Sel<T>::x(L);
+    T* t = li == l0 ? t0 : Sel<T>::t(L);
+    T* w = li == l0 ? w0 : Sel<T>::w(L);
     hipLaunchKernelGGL(k_prolong_add3<T>, dim3((L.Lx + 63) / 64, L.Ly, L.Lz * nb), dim3(64), 0,
                        s->stream, L.Lx, L.Ly, L.Lz, Sel<T>::x(C), L.mask, x);
     LAUNCHCHK();
-    CHK(smooth_step<T>(s, li, x, b, ns, x, t, w, dot && li == 0, n_part));
+    CHK(smooth_step<T>(s, li, x, b, ns, x, t, w, dot && li == l0, n_part));
   }
   return VX_OK;
 }
@@ -471,12 +604,19 @@ struct PcgResult {
 // loop ends when all have stopped or max_iter is reached.  The history and
 // the callback see ratios times ratio_scale[r] (1 for a plain solve; the
 // outer residual ratio for an inner solve of iterative refinement).
+//
+// l0 > 0 runs the same iteration on level l0's system (its V-cycle starts
+// there).  With the learned corrector active (FP32) the preconditioner is
+// not a fixed linear operator, so the direction update is the flexible one,
+// beta = -z_new.A p / p.A p (A-orthogonal to the previous direction), which
+// equals Fletcher-Reeves for a fixed symmetric preconditioner.
 template <typename T>
 int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* p2, T* Ap, T* t0, double tol_rel, int max_iter,
         bool stall_guard, const std::vector<double>& ratio_scale, double cb_tol,
         const std::vector<int>& iter_offset, vx_progress_cb cb, void* user, double* hist,
-        int hist_cap, std::vector<PcgResult>& res, const std::vector<int>& skip) {
-  Level& L0 = s->lv[0];
+        int hist_cap, std::vector<PcgResult>& res, const std::vector<int>& skip, int l0 = 0) {
+  Level& L0 = s->lv[l0];
+  const bool flexible = (std::is_same<T, float>::value && s->corr_on) || s->opt.flexible;
   const int nb = s->nb;
   const long long n = 3 * L0.Nn;
   int np = 0;
@@ -486,7 +626,7 @@ int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* p2, T* Ap, T* t0, double tol_re
 
   res.assign(nb, PcgResult());
   std::vector<int> frozen(skip.begin(), skip.end());
-  int rc = vcycle<T>(s, r, z, t0, Sel<T>::w(L0), true, &np);
+  int rc = vcycle<T>(s, r, z, t0, Sel<T>::w(L0), true, &np, l0);
   if (rc == VX_OK) rc = finalize(s, np, F_RZ0);   // also beta = 0, frozen = 0
   if (rc == VX_OK && std::any_of(frozen.begin(), frozen.end(), [](int f) { return f; }))
     rc = push_frozen(s, frozen);
@@ -529,9 +669,11 @@ int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* p2, T* Ap, T* t0, double tol_re
     }
     auto queue_next = [&]() -> int {
       int np2 = 0;
-      CHK(vcycle<T>(s, r, z, t0, Sel<T>::w(L0), true, &np2));
-      CHK(finalize(s, np2, F_BETA));
-      return VX_OK;
+      CHK(vcycle<T>(s, r, z, t0, Sel<T>::w(L0), true, &np2, l0));
+      if (!flexible) return finalize(s, np2, F_BETA);
+      CHK(finalize(s, np2, F_RZNEW));
+      CHK(dot_partials<T>(s, n, z, Ap, &np2));
+      return finalize(s, np2, F_BETA_FLEX);
     };
     bool queued = false;
     if (!predicted_done) {
@@ -597,7 +739,71 @@ int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* p2, T* Ap, T* t0, double tol_re
   return rc;
 }
 
+// corrector buffers sized for level `level` (and max_batch right-hand sides)
+int corr_buffers(vx_solver* s, int level) {
+  const Level& L = s->lv[level];
+  const size_t mb = (size_t)s->opt.max_batch;
+  if (L.Nn > s->corr_cap_nn) {
+    // older (smaller) buffers stay owned by s->allocs
+    s->corr_feat = s->corr_out = s->corr_v = s->corr_y = s->corr_Av = s->corr_Ay = nullptr;
+    CHK(dalloc(s, &s->corr_feat, 4 * (size_t)L.Nn));
+    CHK(dalloc(s, &s->corr_out, 16 * (size_t)L.Nn * mb));
+    CHK(dalloc(s, &s->corr_v, 3 * (size_t)L.Nn * mb));
+    CHK(dalloc(s, &s->corr_y, 3 * (size_t)L.Nn * mb));
+    CHK(dalloc(s, &s->corr_Av, 3 * (size_t)L.Nn * mb));
+    CHK(dalloc(s, &s->corr_Ay, 3 * (size_t)L.Nn * mb));
+    s->corr_cap_nn = L.Nn;
+  }
+  CHK(dalloc(s, &s->corr_part, 5 * (size_t)EW_BLOCKS * mb));
+  CHK(dalloc(s, &s->corr_coef, 2 * mb));
+  CHK(dalloc(s, &s->corr_rms, mb));
+  CHK(dalloc(s, &s->corr_emax, 1));
+  return VX_OK;
+}
+
+// node features of the corrector level from its E and mask
+int corr_features(vx_solver* s, int level) {
+  Level& L = s->lv[level];
+  HIPCHK(hipMemsetAsync(s->corr_emax, 0, sizeof(unsigned int), s->stream));
+  hipLaunchKernelGGL(k_max_nonneg, dim3(ew_grid(L.Ne)), dim3(EW_THREADS), 0, s->stream, L.Ne,
+                     L.E32, s->corr_emax);
+  hipLaunchKernelGGL(k_net_feat, dim3(ew_grid(L.Nn)), dim3(EW_THREADS), 0, s->stream, L.nx, L.ny,
+                     L.nz, L.E32, L.mask, s->corr_emax, s->corr_feat);
+  LAUNCHCHK();
+  return VX_OK;
+}
+
 }  // namespace
+
+static int alloc_pcg64(vx_solver* s);
+
+// PCG on level l's own system (V-cycle from l down): x = A_l^-1 b to tol.
+static int level_solve(vx_solver* s, int l, const double* b, double* x, double tol, int max_iter,
+                       PcgResult* out) {
+  CHK(alloc_pcg64(s));
+  s->nb = 1;
+  Level& L = s->lv[l];
+  const long long n3 = 3 * L.Nn;
+  hipLaunchKernelGGL((k_mask_copy<double, double>), dim3(EW_BLOCKS, 1), dim3(EW_THREADS), 0,
+                     s->stream, L.Nn, L.mask, b, s->r64, s->partial);
+  LAUNCHCHK();
+  CHK(finalize(s, EW_BLOCKS, F_FNORM));
+  HIPCHK(hipMemsetAsync(x, 0, sizeof(double) * n3, s->stream));
+  CHK(sync_scalars(s));
+  *out = PcgResult();
+  if (s->hscal[0].fnorm == 0.0) {
+    out->converged = true;
+    out->ratio = 0.0;
+    return VX_OK;
+  }
+  std::vector<double> one(1, 1.0);
+  std::vector<int> zero(1, 0);
+  std::vector<PcgResult> pr;
+  CHK(pcg<double>(s, x, s->r64, s->z64, s->p64, s->q64, s->Ap64, L.t64, tol, max_iter, false, one,
+                  tol, zero, nullptr, nullptr, nullptr, 0, pr, zero, l));
+  *out = pr[0];
+  return VX_OK;
+}
 
 // ===========================================================================
 // C ABI
@@ -806,6 +1012,7 @@ int vx_set_density(vx_handle s, const double* E, const uint8_t* active, double e
     LAUNCHCHK();
   }
   s->density_set = true;
+  if (s->corr) return corr_features(s, s->corr_level);
   return VX_OK;
 }
 
@@ -889,6 +1096,9 @@ static int solve_impl(vx_solver* s, int nb, const double* F, double* U, const do
   int rc = VX_OK;
   std::vector<PcgResult> pr(nb);
   std::vector<int> stalled(nb, 0);
+  // corrector: 0 none, 1 used throughout, 2 fell back to the plain V-cycle
+  s->corr_on = mixed && s->corr != nullptr && s->corr_enabled;
+  int corr_state = s->corr_on ? 1 : 0;
   int np = 0;
   if (!std::all_of(zero.begin(), zero.end(), [](int z) { return z; })) {
     if (!mixed) {
@@ -908,9 +1118,15 @@ static int solve_impl(vx_solver* s, int nb, const double* F, double* U, const do
         stalled[q] = pr[q].stalled;
       }
     } else {
-      // Iterative refinement: FP64 true residual, FP32 inner MG-PCG.
+      // Iterative refinement: FP64 true residual, FP32 inner MG-PCG.  With
+      // the learned corrector, each inner solve must cut the worst true
+      // residual at least in half; otherwise (or on a non-finite inner
+      // ratio) the corrector is switched off for the rest of the solve and
+      // the plain V-cycle takes over.
       std::vector<int> total(nb, 0), done(zero);
       std::vector<double> true_ratio(nb, 0.0);
+      double prev_worst = INFINITY;
+      bool learned_step = false;
       for (int outer = 0; outer <= s->opt.max_outer && rc == VX_OK; ++outer) {
         if (outer == 0 && !X0) {
           rc = copy_vec<double>(s, s->r64, s->fm64, n * nb);
@@ -934,6 +1150,11 @@ static int solve_impl(vx_solver* s, int nb, const double* F, double* U, const do
           }
           most = std::max(most, total[q]);
         }
+        if (learned_step && open && !(worst < 0.5 * prev_worst)) {
+          s->corr_on = false;
+          corr_state = 2;
+        }
+        prev_worst = worst;
         if (!open || outer == s->opt.max_outer || most >= max_iter) break;
         // mark finished right-hand sides (tmp2) so their inner input is 0
         for (int q = 0; q < nb; ++q) s->hscal[q].tmp2 = done[q] ? 1.0 : 0.0;
@@ -947,10 +1168,36 @@ static int solve_impl(vx_solver* s, int nb, const double* F, double* U, const do
         hipLaunchKernelGGL(k_set_fnorm, dim3(nb), dim3(1), 0, s->stream, s->dscal, 1.0);
         LAUNCHCHK();
         const double want = std::max(s->opt.inner_tol, 0.5 * tol / worst);
-        rc = pcg<float>(s, s->d32, s->r32, s->z32, s->p32, s->q32, s->Ap32, L0.t32, want,
-                        max_iter - most, false, true_ratio, tol, total, cb, user, hist, hist_cap,
-                        pr, done);
+        learned_step = s->corr_on;
+        int cap = max_iter - most;
+        if (learned_step && s->corr_budget > 0) cap = std::min(cap, s->corr_budget);
+        rc = pcg<float>(s, s->d32, s->r32, s->z32, s->p32, s->q32, s->Ap32, L0.t32, want, cap,
+                        false, true_ratio, tol, total, cb, user, hist, hist_cap, pr, done);
         if (rc != VX_OK) break;
+        if (learned_step && s->corr_budget > 0) {
+          // over budget without reaching the inner tolerance: keep the (energy-
+          // decreasing) partial update, continue with the plain V-cycle
+          for (int q = 0; q < nb; ++q)
+            if (!done[q] && !pr[q].converged && pr[q].iters >= cap) {
+              s->corr_on = false;
+              corr_state = 2;
+            }
+        }
+        bool finite = true;
+        for (int q = 0; q < nb; ++q)
+          if (!done[q] && pr[q].iters > 0 && !std::isfinite(pr[q].ratio)) finite = false;
+        if (!finite) {
+          if (!learned_step) {
+            rc = fail(s, VX_ERR_HIP, "non-finite residual in the FP32 inner solve");
+            break;
+          }
+          // drop the inner result, retry from the same iterate without the corrector
+          s->corr_on = false;
+          corr_state = 2;
+          learned_step = false;
+          prev_worst = INFINITY;
+          continue;
+        }
         bool progressed = false;
         for (int q = 0; q < nb; ++q) {
           if (done[q]) continue;
@@ -991,6 +1238,8 @@ static int solve_impl(vx_solver* s, int nb, const double* F, double* U, const do
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  for (int q = 0; q < nb; ++q) inf[q].corrector = corr_state;
+  s->corr_on = false;
   if (infos)
     for (int q = 0; q < nb; ++q) infos[q] = inf[q];
   s->nb = 1;
@@ -1281,12 +1530,12 @@ int vx_vcycle(vx_handle s, int32_t prec, const double* b, double* z) {
   if (prec == VX_FP64) {
     CHK(alloc_pcg64(s));
     HIPCHK(hipMemcpyAsync(s->r64, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s->stream));
-    CHK(vcycle<double>(s, s->r64, s->z64, s->lv[0].t64, s->lv[0].w64, false, nullptr));
+    CHK(vcycle<double>(s, s->r64, s->z64, s->lv[0].t64, s->lv[0].w64, false, nullptr, 0));
     HIPCHK(hipMemcpyAsync(z, s->z64, sizeof(double) * n, hipMemcpyDeviceToDevice, s->stream));
   } else {
     CHK(alloc_pcg32(s));
     CHK(to_T<float>(s, b, s->r32, n));
-    CHK(vcycle<float>(s, s->r32, s->z32, s->lv[0].t32, s->lv[0].w32, false, nullptr));
+    CHK(vcycle<float>(s, s->r32, s->z32, s->lv[0].t32, s->lv[0].w32, false, nullptr, 0));
     CHK(from_T<float>(s, s->z32, z, n));
   }
   HIPCHK(hipStreamSynchronize(s->stream));
@@ -1425,6 +1674,261 @@ int vx_time_kernel(vx_handle s, int32_t kind, int32_t l, int32_t prec, int32_t r
   *avg_ms = ms / reps;
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
+  return rc;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// learned coarse-grid corrector: attach, capture, targets, training
+// ===========================================================================
+extern "C" {
+
+int vx_set_corrector(vx_handle s, vxn_handle net, int32_t level, float gain, int32_t enabled,
+                     int32_t iter_budget) {
+  if (!s) return VX_ERR_ARG;
+  if (!net) {
+    s->corr = nullptr;
+    s->corr_level = -1;
+    s->corr_enabled = false;
+    return VX_OK;
+  }
+  if (s->opt.precision != VX_MIXED)
+    return fail(s, VX_ERR_ARG, "the corrector runs inside the FP32 V-cycle: precision VX_MIXED");
+  if (level < 0 || level >= (int)s->lv.size()) return fail(s, VX_ERR_ARG, "bad corrector level");
+  CHK(corr_buffers(s, level));
+  const Level& L = s->lv[level];
+  s->corr = net;
+  s->corr_level = level;
+  s->corr_gain = gain > 0.f ? gain : (float)std::max(L.Lx, std::max(L.Ly, L.Lz));
+  s->corr_enabled = enabled != 0;
+  s->corr_budget = std::max(0, (int)iter_budget);
+  if (s->density_set) CHK(corr_features(s, level));
+  return VX_OK;
+}
+
+int vx_corrector_eval(vx_handle s, int32_t nsamp, const float* T, const float* V,
+                      double* gain_host) {
+  if (!s || !T || !V || !gain_host || nsamp < 1)
+    return fail(s, VX_ERR_ARG, "bad corrector_eval arguments");
+  if (!s->corr) return fail(s, VX_ERR_ARG, "attach a corrector first (vx_set_corrector)");
+  CHK(require_ready(s));
+  CHK(alloc_pcg32(s));
+  Level& L = s->lv[s->corr_level];
+  const long long n3 = 3 * L.Nn;
+  s->nb = 1;
+  std::vector<double> m(5 * EW_BLOCKS);
+  for (int i = 0; i < nsamp; ++i) {
+    const float* t = T + (long long)i * n3;
+    const float* v = V + (long long)i * n3;
+    int np = 0;
+    CHK(dot_partials<float>(s, n3, t, t, &np));
+    CHK(finalize(s, np, F_SET_CS));
+    hipLaunchKernelGGL(k_net_rms, dim3(1), dim3(64), 0, s->stream, s->dscal, 1, (double)n3,
+                       s->corr_rms);
+    LAUNCHCHK();
+    CHK(corrector_net(s, s->corr_level, t, v, s->corr_y, s->corr_rms));
+    CHK((launch_tile_L<float, M_MATVEC, false>(s, L, v, nullptr, s->corr_Av, nullptr, nullptr)));
+    CHK((launch_tile_L<float, M_MATVEC, false>(s, L, s->corr_y, nullptr, s->corr_Ay, nullptr,
+                                               nullptr)));
+    hipLaunchKernelGGL(k_dot5, dim3(EW_BLOCKS, 1), dim3(EW_THREADS), 0, s->stream, n3, v,
+                       s->corr_y, s->corr_Av, s->corr_Ay, t, s->corr_part);
+    LAUNCHCHK();
+    HIPCHK(hipMemcpyAsync(m.data(), s->corr_part, sizeof(double) * 5 * EW_BLOCKS,
+                          hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    double d[5] = {0, 0, 0, 0, 0};
+    for (int k = 0; k < 5; ++k)
+      for (int j = 0; j < EW_BLOCKS; ++j) d[k] += m[k * EW_BLOCKS + j];
+    const double a = d[0], b = d[1], c = d[2], vt = d[3], yt = d[4];
+    // energy error of c: e.Ae - 2 c.t + c.Ac; gain of the best alpha v + beta y over v
+    double al = a > 0 ? vt / a : 0.0, be = 0.0;
+    const double det = a * c - b * b;
+    if (a > 0 && c > 0 && det > 1e-10 * a * c) {
+      al = (c * vt - b * yt) / det;
+      be = (a * yt - b * vt) / det;
+    }
+    const double e_v = -2.0 * vt + a;
+    const double e_c = -2.0 * (al * vt + be * yt) + al * al * a + 2 * al * be * b + be * be * c;
+    gain_host[i] = vt > 0 ? (e_v - e_c) / vt : 0.0;
+  }
+  return VX_OK;
+}
+
+int vx_capture(vx_handle s, int32_t level, float* buf, int32_t cap, int32_t every) {
+  if (!s) return VX_ERR_ARG;
+  if (!buf || cap <= 0) {
+    s->cap_buf = nullptr;
+    s->cap_cap = 0;
+    return VX_OK;
+  }
+  const int nl = (int)s->lv.size();
+  if (level < 0 || level >= nl || (level == nl - 1 && level != s->corr_level))
+    return fail(s, VX_ERR_ARG, "capture level must be one whose residual the V-cycle forms");
+  s->cap_buf = buf;
+  s->cap_level = level;
+  s->cap_cap = cap;
+  s->cap_every = std::max(1, (int)every);
+  s->cap_count = 0;
+  s->cap_calls = 0;
+  return VX_OK;
+}
+
+int vx_capture_count(vx_handle s) { return s ? s->cap_count : VX_ERR_ARG; }
+
+int vx_corrector_prepare(vx_handle s, int32_t nsamp, const float* T, float* V, float* R,
+                         double* vt) {
+  if (!s || !T || !V || !R || !vt || nsamp < 1)
+    return fail(s, VX_ERR_ARG, "bad corrector_prepare arguments");
+  if (!s->corr) return fail(s, VX_ERR_ARG, "attach a corrector first (vx_set_corrector)");
+  CHK(require_ready(s));
+  CHK(alloc_pcg32(s));
+  const int l = s->corr_level;
+  Level& L = s->lv[l];
+  const long long n3 = 3 * L.Nn;
+  const bool was_on = s->corr_on;
+  s->corr_on = false;
+  s->nb = 1;
+  int rc = VX_OK;
+  for (int i = 0; i < nsamp && rc == VX_OK; ++i) {
+    const float* t = T + (long long)i * n3;
+    float* v = V + (long long)i * n3;
+    float* r = R + (long long)i * n3;
+    rc = plain_coarse<float>(s, l, t, v, s->corr_Av, s->corr_Ay);
+    if (rc == VX_OK)
+      rc = launch_tile_L<float, M_MATVEC, false>(s, L, v, nullptr, s->corr_Av, nullptr, nullptr);
+    if (rc != VX_OK) break;
+    hipLaunchKernelGGL(k_sub, dim3(ew_grid(n3)), dim3(EW_THREADS), 0, s->stream, n3, t,
+                       s->corr_Av, r);
+    if (hipGetLastError() != hipSuccess) { rc = fail(s, VX_ERR_HIP, "k_sub"); break; }
+    int np = 0;
+    rc = dot_partials<float>(s, n3, v, t, &np);
+    if (rc == VX_OK) rc = finalize(s, np, F_SET_CS);
+    if (rc == VX_OK &&
+        hipMemcpyAsync(vt + i, reinterpret_cast<char*>(s->dscal) + offsetof(Scalars, cs),
+                       sizeof(double), hipMemcpyDeviceToDevice, s->stream) != hipSuccess)
+      rc = fail(s, VX_ERR_HIP, "v.t copy");
+  }
+  s->corr_on = was_on;
+  if (rc != VX_OK) return rc;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return VX_OK;
+}
+
+int vx_corrector_train(vx_handle s, int32_t nsamp, const float* T, const float* V, const float* R,
+                       const double* vt, int32_t batch, int32_t steps, float lr, uint64_t seed,
+                       float* loss_host) {
+  if (!s || !T || !V || !R || !vt || nsamp < 1 || batch < 1 || steps < 1)
+    return fail(s, VX_ERR_ARG, "bad corrector_train arguments");
+  if (!s->corr) return fail(s, VX_ERR_ARG, "attach a corrector first (vx_set_corrector)");
+  if (!s->density_set) return fail(s, VX_ERR_ARG, "call vx_set_density() before training");
+  Level& L = s->lv[s->corr_level];
+  const long long Nn = L.Nn, n3 = 3 * Nn;
+  const long long np = vxnet_api::num_params(s->corr);
+  // training buffers live for this call only
+  std::vector<void*> mine;
+  bool ok = true;
+  auto get = [&](size_t bytes) -> void* {
+    void* p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(bytes, 16)) != hipSuccess) {
+      ok = false;
+      return nullptr;
+    }
+    mine.push_back(p);
+    return p;
+  };
+  float* tt = (float*)get(sizeof(float) * batch * n3);
+  float* tv = (float*)get(sizeof(float) * batch * n3);
+  float* tr = (float*)get(sizeof(float) * batch * n3);
+  float* ty = (float*)get(sizeof(float) * batch * n3);
+  float* tAy = (float*)get(sizeof(float) * batch * n3);
+  float* tAv = (float*)get(sizeof(float) * batch * n3);
+  float* tgy = (float*)get(sizeof(float) * batch * n3);
+  float* tout = (float*)get(sizeof(float) * batch * Nn * 16);
+  float* tdout = (float*)get(sizeof(float) * batch * Nn * 16);
+  float* tgrad = (float*)get(sizeof(float) * np);
+  float* trms = (float*)get(sizeof(float) * batch);
+  double* tvt = (double*)get(sizeof(double) * batch);
+  double* part = (double*)get(sizeof(double) * 5 * batch * EW_BLOCKS);
+  double* tcoef = (double*)get(sizeof(double) * 2 * batch);
+  double* lossd = (double*)get(sizeof(double) * steps);
+  auto release = [&]() {
+    (void)hipStreamSynchronize(s->stream);
+    for (void* p : mine) (void)hipFree(p);
+  };
+  if (!ok) {
+    release();
+    return fail(s, VX_ERR_NOMEM, "training buffers");
+  }
+  int rc = VX_OK;
+  uint64_t x = seed ? seed : 0x9E3779B97F4A7C15ull;
+  s->nb = 1;
+  const double pi = 3.14159265358979323846;
+  for (int step = 1; step <= steps && rc == VX_OK; ++step) {
+    for (int b = 0; b < batch && rc == VX_OK; ++b) {
+      x = x * 6364136223846793005ull + 1442695040888963407ull;
+      const long long i = (long long)((x >> 33) % (uint64_t)nsamp);
+      if (hipMemcpyAsync(tt + b * n3, T + i * n3, sizeof(float) * n3, hipMemcpyDeviceToDevice,
+                         s->stream) != hipSuccess ||
+          hipMemcpyAsync(tv + b * n3, V + i * n3, sizeof(float) * n3, hipMemcpyDeviceToDevice,
+                         s->stream) != hipSuccess ||
+          hipMemcpyAsync(tr + b * n3, R + i * n3, sizeof(float) * n3, hipMemcpyDeviceToDevice,
+                         s->stream) != hipSuccess ||
+          hipMemcpyAsync(tvt + b, vt + i, sizeof(double), hipMemcpyDeviceToDevice,
+                         s->stream) != hipSuccess)
+        rc = fail(s, VX_ERR_HIP, "sample copy");
+    }
+    if (rc != VX_OK) break;
+    hipLaunchKernelGGL(k_dot<float>, dim3(EW_BLOCKS, batch), dim3(EW_THREADS), 0, s->stream, n3,
+                       tt, tt, part);
+    hipLaunchKernelGGL(k_rms_from_partials, dim3(batch), dim3(256), 0, s->stream, part, EW_BLOCKS,
+                       (double)n3, trms);
+    if (hipGetLastError() != hipSuccess) { rc = fail(s, VX_ERR_HIP, "rms"); break; }
+    rc = vxnet_api::prepare(s->corr, s->stream, batch, L.Lx, L.Ly, L.Lz);
+    if (rc != VX_OK) { fail(s, rc, vxnet_api::error(s->corr)); break; }
+    hipLaunchKernelGGL(k_net_input, dim3(ew_grid(Nn), batch), dim3(EW_THREADS), 0, s->stream, Nn,
+                       tt, tv, s->corr_feat, L.mi32, trms, s->corr_gain,
+                       vxnet_api::input(s->corr));
+    rc = vxnet_api::forward_bf16(s->corr, tout);
+    if (rc != VX_OK) { fail(s, rc, vxnet_api::error(s->corr)); break; }
+    hipLaunchKernelGGL(k_net_apply, dim3(ew_grid(Nn), batch), dim3(EW_THREADS), 0, s->stream, Nn,
+                       tout, L.mi32, L.mask, trms, s->corr_gain, ty, 0);
+    if (hipGetLastError() != hipSuccess) { rc = fail(s, VX_ERR_HIP, "apply"); break; }
+    for (int b = 0; b < batch && rc == VX_OK; ++b)
+      rc = launch_tile_L<float, M_MATVEC, false>(s, L, ty + b * n3, nullptr, tAy + b * n3,
+                                                 nullptr, nullptr);
+    if (rc != VX_OK) break;
+    // A v = t - r; the five products, the applied combination, loss, gradient
+    hipLaunchKernelGGL(k_sub, dim3(ew_grid(batch * n3)), dim3(EW_THREADS), 0, s->stream,
+                       batch * n3, tt, tr, tAv);
+    hipLaunchKernelGGL(k_dot5, dim3(EW_BLOCKS, batch), dim3(EW_THREADS), 0, s->stream, n3, tv, ty,
+                       tAv, tAy, tt, part);
+    hipLaunchKernelGGL(k_combine_coef, dim3(batch), dim3(256), 0, s->stream, part, EW_BLOCKS,
+                       tcoef);
+    hipLaunchKernelGGL(k_train_loss, dim3(1), dim3(256), 0, s->stream, part, EW_BLOCKS, batch,
+                       tcoef, lossd + (step - 1));
+    hipLaunchKernelGGL(k_train_gy, dim3(ew_grid(n3), batch), dim3(EW_THREADS), 0, s->stream, n3,
+                       tt, tAv, tAy, tcoef, tvt, tgy);
+    hipLaunchKernelGGL(k_net_apply_bwd, dim3(ew_grid(Nn), batch), dim3(EW_THREADS), 0, s->stream,
+                       Nn, tgy, L.mi32, L.mask, trms, s->corr_gain, tdout);
+    if (hipGetLastError() != hipSuccess) { rc = fail(s, VX_ERR_HIP, "loss"); break; }
+    rc = vxnet_api::backward(s->corr, tdout, tgrad);
+    if (rc != VX_OK) { fail(s, rc, vxnet_api::error(s->corr)); break; }
+    // cosine decay to 5 % of lr
+    const double f = 0.05 + 0.95 * 0.5 * (1.0 + std::cos(pi * (step - 1) / steps));
+    rc = vxnet_api::adam(s->corr, tgrad, (float)(lr * f), 0.9f, 0.999f, 1e-8f, step);
+    if (rc != VX_OK) { fail(s, rc, vxnet_api::error(s->corr)); break; }
+  }
+  if (rc == VX_OK && loss_host) {
+    std::vector<double> l(steps);
+    if (hipMemcpyAsync(l.data(), lossd, sizeof(double) * steps, hipMemcpyDeviceToHost,
+                       s->stream) != hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess)
+      rc = fail(s, VX_ERR_HIP, "loss readback");
+    else
+      for (int i = 0; i < steps; ++i) loss_host[i] = (float)l[i];
+  }
+  release();
   return rc;
 }
 
