@@ -233,21 +233,26 @@ def other_workload(args):
     elif args.workload == "uneven_parts":
         parts = W.uneven_parts()
         rows = []
-        tot = 0.0
+        tot = {}
         for q in parts:
             E, A, f = dev(q.E_stiffness), dev(q.active.astype(np.uint8)), dev(q.force)
             u = torch.empty_like(f)
-            mg = MGSolver(q.nx, q.ny, q.nz, q.fixed_dofs, nu=q.nu, compute_mode="GPU", **OPT)
-            run = lambda: (mg.set_density(E, active=A), mg.solve_device(f, u, tol=q.tol,
-                                                                         max_cg=5000))
-            timed(run)
-            t, _ = timed(run)
-            tot += t
-            rows.append({"dims": [q.nx, q.ny, q.nz], "levels": len(mg.levels), "time_s": t,
-                         "iters": mg.last_iters, "converged": mg.last_converged})
-            mg.close()
+            row = {"dims": [q.nx, q.ny, q.nz]}
+            for tag, pad in (("as_is", False), ("padded", True)):
+                mg = MGSolver(q.nx, q.ny, q.nz, q.fixed_dofs, nu=q.nu, compute_mode="GPU",
+                              pad=pad, **OPT)
+                run = lambda: (mg.set_density(E, active=A), mg.solve_device(f, u, tol=q.tol,
+                                                                             max_cg=5000))
+                timed(run)
+                t, _ = timed(run)
+                tot[tag] = tot.get(tag, 0.0) + t
+                row[tag] = {"grid": list(mg._pdims), "levels": len(mg.levels), "time_s": t,
+                            "iters": mg.last_iters, "converged": mg.last_converged}
+                mg.close()
+            rows.append(row)
         out["parts"] = rows
         out["total_time_s"] = tot
+        out["padded_speedup"] = tot["as_is"] / tot["padded"]
     return out
 
 

@@ -39,6 +39,20 @@ def _dev(a, dtype=None):
     return torch.as_tensor(np.ascontiguousarray(a), device="cuda").to(dt).contiguous()
 
 
+def padded_dims(dims, max_levels=6, overhead=1.3):
+    """Element dims grown to multiples of 2^k (largest k < max_levels whose
+    volume stays within ``overhead`` times the original), so that the
+    hierarchy rule (all dims even, core/multigrid.py:144-148) can coarsen k
+    times; the dims themselves when no such k exists."""
+    vol = float(np.prod(dims))
+    for k in range(max_levels - 1, 0, -1):
+        m = 2 ** k
+        p = tuple(-(-int(d) // m) * m for d in dims)
+        if np.prod(p) <= overhead * vol:
+            return p
+    return tuple(int(d) for d in dims)
+
+
 def _require_gpu(plan, what):
     if not plan.on_gpu:
         raise RuntimeError(
@@ -53,7 +67,7 @@ class MGSolver:
                  min_elems=4, max_levels=6, compute_mode="AUTO", cpu_threads=0,
                  verbose=False, precision="fp64", coarse_sweeps=30, inner_tol=1e-5,
                  max_outer=20, z_chunk=0, smoother="jacobi", cheb_ratio=0.3,
-                 system="full", e_coarsen="mean", max_batch=1, flexible=False):
+                 system="full", e_coarsen="mean", max_batch=1, flexible=False, pad=False):
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
         self.nelem = self.nx * self.ny * self.nz
         self.ndof = 3 * (self.nx + 1) * (self.ny + 1) * (self.nz + 1)
@@ -62,6 +76,14 @@ class MGSolver:
         if system not in ("full", "active"):
             raise ValueError("system must be 'full' (voids at E*e_min) or 'active'")
         self.system = system
+        # pad: solve on a grid grown with void elements to coarsenable dims.
+        # Only the active-DOF system is unchanged by that (void elements carry
+        # nothing and their DOFs leave the system).
+        if pad and system != "active":
+            raise ValueError("pad=True needs system='active'")
+        self._pdims = padded_dims((self.nx, self.ny, self.nz), max_levels) if pad else \
+            (self.nx, self.ny, self.nz)
+        self.padded = self._pdims != (self.nx, self.ny, self.nz)
         self.plan = compute_plan.choose(self.ndof, mode=compute_mode, cpu_threads=cpu_threads,
                                         verbose=verbose)
         _require_gpu(self.plan, "MGSolver")
@@ -76,8 +98,9 @@ class MGSolver:
             max_batch=max_batch, flexible=1 if flexible else 0)
         self.max_batch = max_batch
         h = ctypes.c_void_p()
-        _native.check(self._lib.vx_create(self.nx, self.ny, self.nz, float(nu),
-                                          ctypes.byref(opt), ctypes.byref(h)))
+        px, py, pz = self._pdims
+        _native.check(self._lib.vx_create(px, py, pz, float(nu), ctypes.byref(opt),
+                                          ctypes.byref(h)))
         self._h = h
         self._stream_bind()
         self.levels = []
@@ -87,7 +110,7 @@ class MGSolver:
             d = (a.value, b.value, c.value)
             self.levels.append({"dims": d, "ndof": 3 * (d[0] + 1) * (d[1] + 1) * (d[2] + 1),
                                 "g": float(2 ** li)})
-        fd = np.ascontiguousarray(np.asarray(fixed_dofs, dtype=np.int64).ravel())
+        fd = np.ascontiguousarray(self._pad_dof_ids(np.asarray(fixed_dofs, dtype=np.int64).ravel()))
         _native.check(self._lib.vx_set_fixed(h, fd.ctypes.data_as(ctypes.c_void_p),
                                              fd.size), h)
         self.last_iters = 0
@@ -97,6 +120,47 @@ class MGSolver:
         self.last_outer_iters = 0
         self.last_solve_ms = float("nan")
         self.resid_history = []
+
+    # -- padding ------------------------------------------------------------
+    def _pad_dof_ids(self, ids):
+        if not self.padded:
+            return ids
+        n, c = ids // 3, ids % 3
+        lx, ly = self.nx + 1, self.ny + 1
+        ix, iy, iz = n % lx, (n // lx) % ly, n // (lx * ly)
+        px, py, _ = self._pdims
+        return 3 * (ix + (px + 1) * (iy + (py + 1) * iz)) + c
+
+    def _pad_elems(self, a, dtype):
+        t = _dev(a, dtype)
+        if not self.padded:
+            return t
+        px, py, pz = self._pdims
+        out = _torch().zeros((px, py, pz), dtype=t.dtype, device=t.device)
+        out[:self.nx, :self.ny, :self.nz] = t.view(self.nx, self.ny, self.nz)
+        return out.view(-1)
+
+    def _node_view(self, v, dims):
+        nx, ny, nz = dims
+        return v.view(-1, nz + 1, ny + 1, nx + 1, 3)
+
+    def _pad_nodes(self, v):
+        """[..., user ndof] -> [..., padded ndof] (zeros outside the part)."""
+        if not self.padded:
+            return v
+        lead = v.shape[:-1]
+        px, py, pz = self._pdims
+        out = _torch().zeros(lead + (3 * (px + 1) * (py + 1) * (pz + 1),), dtype=v.dtype,
+                             device=v.device)
+        self._node_view(out, self._pdims)[:, :self.nz + 1, :self.ny + 1, :self.nx + 1] = \
+            self._node_view(v.contiguous(), (self.nx, self.ny, self.nz))
+        return out
+
+    def _unpad_nodes(self, vp, out):
+        if not self.padded:
+            return
+        out.view(-1, self.nz + 1, self.ny + 1, self.nx + 1, 3).copy_(
+            self._node_view(vp, self._pdims)[:, :self.nz + 1, :self.ny + 1, :self.nx + 1])
 
     # -- plumbing ----------------------------------------------------------
     def _stream_bind(self):
@@ -117,6 +181,10 @@ class MGSolver:
     def _check(self, rc, allow=()):
         return _native.check(rc, self._h, allow)
 
+    def _unpadded_only(self, what):
+        if self.padded:
+            raise RuntimeError(f"{what} works on the solver's own grid: not on a padded solver")
+
     # -- density -----------------------------------------------------------
     def set_density(self, Evec_full, active=None, e_min=1e-9):
         """Per-level E, 2x2x2 coarsening (core/multigrid.py:281-315).  With
@@ -125,8 +193,14 @@ class MGSolver:
         voids carry no stiffness and DOFs of no active element leave the
         system (the system of the reference's Jacobi-PCG path,
         core/fea.py:497-593)."""
-        self._E_dev = _dev(Evec_full)
-        self._active_dev = None if active is None else _dev(active, _torch().uint8)
+        torch = _torch()
+        if self.padded:
+            self._E_dev = self._pad_elems(Evec_full, torch.float64)
+            act = np.ones(self.nelem, np.uint8) if active is None else active
+            self._active_dev = self._pad_elems(act, torch.uint8)
+        else:
+            self._E_dev = _dev(Evec_full)
+            self._active_dev = None if active is None else _dev(active, torch.uint8)
         self._stream_bind()
         em = -1.0 if self.system == "active" else float(e_min)
         self._check(self._lib.vx_set_density(self._h, _native.ptr(self._E_dev),
@@ -148,10 +222,13 @@ class MGSolver:
                 user_cb(int(it), float(ratio), float(t))
 
         cb = _native.PROGRESS_CB(_cb) if progress_cb is not None else _native.PROGRESS_CB()
-        rc = self._lib.vx_solve(self._h, _native.ptr(f_dev), _native.ptr(u_dev),
-                                _native.ptr(x0_dev), float(tol), int(max_cg or 0), cb, None,
+        fp, xp0 = self._pad_nodes(f_dev), None if x0_dev is None else self._pad_nodes(x0_dev)
+        up = _torch().empty_like(fp) if self.padded else u_dev
+        rc = self._lib.vx_solve(self._h, _native.ptr(fp), _native.ptr(up),
+                                _native.ptr(xp0), float(tol), int(max_cg or 0), cb, None,
                                 hist.ctypes.data_as(ctypes.c_void_p), hist_cap,
                                 ctypes.byref(info))
+        self._unpad_nodes(up, u_dev)
         self.last_iters = info.iters
         self.last_converged = bool(info.converged)
         self.last_resid_ratio = info.resid_ratio
@@ -178,9 +255,12 @@ class MGSolver:
             progress_cb(int(it), float(ratio), float(t))
 
         cb = _native.PROGRESS_CB(_cb) if progress_cb is not None else _native.PROGRESS_CB()
-        rc = self._lib.vx_solve_batch(self._h, nrhs, _native.ptr(F_dev), _native.ptr(U_dev),
+        Fp = self._pad_nodes(F_dev)
+        Up = _torch().empty_like(Fp) if self.padded else U_dev
+        rc = self._lib.vx_solve_batch(self._h, nrhs, _native.ptr(Fp), _native.ptr(Up),
                                       float(tol), int(max_cg or 0), cb, None,
                                       hist.ctypes.data_as(ctypes.c_void_p), hist_cap, infos)
+        self._unpad_nodes(Up, U_dev)
         self.last_batch = [{"iters": i.iters, "converged": bool(i.converged),
                             "outer_iters": i.outer_iters, "resid_ratio": i.resid_ratio,
                             "true_resid_ratio": i.true_resid_ratio,
@@ -195,6 +275,7 @@ class MGSolver:
                             progress_cb=None, hist_cap=100000):
         """Jacobi-PCG on the active-DOF system on the device
         (core/fea.py:497-593); same fields as solve_device."""
+        self._unpadded_only('solve_jacobi_device')
         self._stream_bind()
         hist = np.zeros(hist_cap)
         info = _native.vx_info()
@@ -311,6 +392,7 @@ class MGSolver:
 
     def _apply(self, li, u, precision="fp64"):
         """Masked K u at level li (core/multigrid.py:318-341)."""
+        self._unpadded_only('_apply')
         ud = _dev(u)
         yd = _torch().empty_like(ud)
         self._stream_bind()
@@ -321,6 +403,7 @@ class MGSolver:
     def _smooth(self, li, u, b, iters, precision="fp64"):
         """``iters`` damped-Jacobi sweeps (core/multigrid.py:367-402); u=None
         starts from zero."""
+        self._unpadded_only('_smooth')
         bd = _dev(b)
         ud = None if u is None else _dev(u)
         od = _torch().empty_like(bd)
@@ -330,6 +413,7 @@ class MGSolver:
         return od.cpu().numpy()
 
     def _restrict(self, li, r_fine):
+        self._unpadded_only('_restrict')
         rf = _dev(r_fine)
         rc = _torch().empty(self.levels[li + 1]["ndof"], dtype=_torch().float64, device="cuda")
         self._stream_bind()
@@ -337,6 +421,7 @@ class MGSolver:
         return rc.cpu().numpy()
 
     def _prolongate(self, li, e_coarse):
+        self._unpadded_only('_prolongate')
         ec = _dev(e_coarse)
         ef = _torch().empty(self.levels[li]["ndof"], dtype=_torch().float64, device="cuda")
         self._stream_bind()
@@ -344,6 +429,7 @@ class MGSolver:
         return ef.cpu().numpy()
 
     def _vcycle(self, li, b, precision="fp64"):
+        self._unpadded_only('_vcycle')
         if li != 0:
             raise ValueError("the V-cycle entry point starts at level 0")
         bd = _dev(b)
@@ -354,6 +440,7 @@ class MGSolver:
         return zd.cpu().numpy()
 
     def inv_diag(self, li):
+        self._unpadded_only('inv_diag')
         md = _torch().empty(self.levels[li]["ndof"], dtype=_torch().float64, device="cuda")
         self._stream_bind()
         self._check(self._lib.vx_inv_diag(self._h, li, _native.ptr(md)))
@@ -361,6 +448,7 @@ class MGSolver:
 
     def time_kernel(self, kind="matvec", level=0, precision="fp64", reps=20):
         """Average device time (ms) of one level operator launch."""
+        self._unpadded_only('time_kernel')
         out = ctypes.c_double()
         self._stream_bind()
         self._check(self._lib.vx_time_kernel(self._h, {"matvec": 0, "jacobi": 1, "chebyshev": 2}[kind], level,
