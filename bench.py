@@ -270,6 +270,8 @@ def main():
                              "uneven_parts"],
                     help="two_material is the headline line; the others print their own "
                          "per-configuration measurement")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip training / measuring the learned corrector")
     ap.add_argument("--oracle-full", action="store_true",
                     help="per-configuration modes: run the complete CPU oracle solve")
     ap.add_argument("--kernel-only", action="store_true",
@@ -361,6 +363,8 @@ def main():
 
     # --- the measured solver ------------------------------------------------
     log(f"MI355X mixed-precision solver {OPT}")
+    torch.cuda.synchronize()
+    free0 = torch.cuda.mem_get_info()[0]
     opt = MGSolver(p.nx, p.ny, p.nz, p.fixed_dofs, nu=p.nu, compute_mode="GPU", **OPT)
 
     def run_opt():
@@ -378,8 +382,44 @@ def main():
             f"true={opt.last_true_resid_ratio:.2e} device_ms={opt.last_solve_ms:.1f}")
     torch.cuda.synchronize()
     t_opt = (time.perf_counter() - t0) / args.steps
-    a = p.active.astype(bool)
+    dev_mem_gb = (free0 - torch.cuda.mem_get_info()[0]) / 1e9
     rel_err = float(((u_dev - u64).abs().max() / u64.abs().max()).item())
+    mixed_iters, mixed_outer = opt.last_iters, opt.last_outer_iters
+    mixed_true, mixed_conv = opt.last_true_resid_ratio, opt.last_converged
+
+    # the same solve from host numpy arrays (the reference's calling form):
+    # host->device copies of E, active, f and the copy of u back included
+    def run_host():
+        E_h = torch.from_numpy(p.E_stiffness).cuda()
+        a_h = torch.from_numpy(p.active.astype(np.uint8)).cuda()
+        f_h = torch.from_numpy(p.force).cuda()
+        uh = torch.empty_like(f_h)
+        opt.set_density(E_h, active=a_h)
+        opt.solve_device(f_h, uh, tol=tol)
+        return uh.cpu().numpy()
+    t_pcie, _ = timed(run_host)
+
+    # --- learned coarse-grid corrector, measured beside the plain solver ---
+    learned = None
+    if not args.no_learned:
+        from core.learned import train_corrector
+        lc = min(3, len(opt.levels) - 1)
+        log(f"learned corrector: training at level {lc} {opt.levels[lc]['dims']}")
+        opt.set_density(E_dev, active=act_dev)
+        t_tr0 = time.perf_counter()
+        net, rep = train_corrector(opt, f_dev, lc, n_samples=96, steps=300, batch=4, lr=3e-4)
+        torch.cuda.synchronize()
+        t_train = time.perf_counter() - t_tr0
+        t_l, _ = timed(run_opt)
+        learned = {"time_s": t_l, "iters": opt.last_iters, "corrector": opt.last_corrector,
+                   "true_resid": opt.last_true_resid_ratio,
+                   "rel_disp_diff_vs_fp64": float(((u_dev - u64).abs().max()
+                                                   / u64.abs().max()).item()),
+                   "training_s": t_train, "training": rep}
+        log(f"learned: {learned['iters']} its ({learned['corrector']}), {t_l:.3f} s; "
+            f"training {t_train:.1f} s")
+        opt.detach_corrector()
+        net.close()
 
     # --- dominant kernel: FP32 level-0 tile kernel in its Chebyshev form ----
     reps = 50
@@ -424,9 +464,14 @@ def main():
         "speedup_vs_fp64_same_preconditioner": t64 / t_opt,
         "reference_fp64_mgpcg": refinfo,
         "fp64_mgpcg_chebyshev": fp64,
-        "mixed": {"iters": opt.last_iters, "outer": opt.last_outer_iters,
-                  "true_resid": opt.last_true_resid_ratio, "converged": opt.last_converged,
+        "value_solver": "mixed FP32/FP64 Chebyshev MG-PCG on the active-DOF system; the "
+                        "learned corrector is measured beside it ('learned')",
+        "mixed": {"iters": mixed_iters, "outer": mixed_outer,
+                  "true_resid": mixed_true, "converged": mixed_conv,
                   "rel_disp_diff_vs_fp64": rel_err},
+        "learned": learned,
+        "pcie_inclusive_s": t_pcie,
+        "device_mem_gb": dev_mem_gb,
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gbs / HBM_PEAK_GBS,
                      "traffic": measured_traffic() if args.n == 256 else None,

@@ -25,4 +25,8 @@ formula; see tests/test_oracle_*.py):
   fixtures written by tests/golden/make_golden.py from the reference itself.
 * post-solve fields: closed forms (von Mises of uniaxial / shear states,
   safety clip), and golden fixtures.
+* ``convnet_ref`` (the learned corrector's network, which has no reference
+  counterpart): convolution against a direct loop, prolongation against its
+  1-D matrices, adjoint identities, backward against central finite
+  differences of the float64 network (tests/test_oracle_net.py).
 """

@@ -159,8 +159,8 @@ def _lib():
 def test_library_exports_every_declared_symbol():
     lib = _lib()
     text = open(HDR).read()
-    names = set(re.findall(r"\b(vx_[a-z_0-9]+)\s*\(", text))
-    assert len(names) >= 20
+    names = set(re.findall(r"\b(vxn?_[a-z_0-9]+)\s*\(", text))
+    assert len(names) >= 40
     missing = [n for n in sorted(names) if not hasattr(lib, n)]
     assert not missing, missing
     from core import _native
