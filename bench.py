@@ -253,6 +253,28 @@ def other_workload(args):
         out["parts"] = rows
         out["total_time_s"] = tot
         out["padded_speedup"] = tot["as_is"] / tot["padded"]
+    elif args.workload == "inside":
+        # voxelisation inside test: a 81,920-triangle sphere in a 256^3 grid
+        from core import inside
+        from oracle import inside_ref
+        v, fc = W.icosphere(6, (0.5, 0.5, 0.5), 0.45)
+        dims, origin, vs = (256, 256, 256), (0.0, 0.0, 0.0), 1.0 / 256
+        run = lambda: inside.inside_grid_device(dims, origin, vs, v, fc, "centers")
+        timed(run)
+        t, m = timed(run)
+        npts = dims[0] * dims[1] * dims[2]
+        out.update({"triangles": int(len(fc)), "points": npts, "gpu_time_s": t,
+                    "points_per_s": npts / t, "inside_fraction": float(m.float().mean())})
+        # CPU: the oracle restatement of the reference's numpy path on a sample
+        rng = np.random.default_rng(0)
+        idx = rng.choice(npts, 2000, replace=False)
+        pts = inside_ref.voxel_centers(dims, origin, vs)[idx]
+        t0 = time.perf_counter()
+        ref = inside_ref.inside_mask(v, fc, pts, chunk=64)
+        tc = time.perf_counter() - t0
+        out["cpu_numpy_sample"] = {"points": 2000, "time_s": tc,
+                                   "extrapolated_full_s": tc * npts / 2000}
+        out["sample_matches"] = bool(np.array_equal(m.cpu().numpy()[idx].astype(bool), ref))
     return out
 
 
@@ -267,7 +289,7 @@ def main():
                     help="Jacobi-PCG fallback iterations of the reference comparator")
     ap.add_argument("--workload", default="two_material",
                     choices=["two_material", "cantilever", "voids_block", "multi_rhs_part",
-                             "uneven_parts"],
+                             "uneven_parts", "inside"],
                     help="two_material is the headline line; the others print their own "
                          "per-configuration measurement")
     ap.add_argument("--no-learned", action="store_true",

@@ -290,6 +290,28 @@ int vx_corrector_train(vx_handle h, int32_t nsamp, const float* T_dev, const flo
                        const float* R_dev, const double* vt_dev, int32_t batch, int32_t steps,
                        float lr, uint64_t seed, float* loss_host);
 
+/* ---------------------------------------------------------------------------
+ * Voxelisation inside test (core/inside_worker.py:118-162): for each point,
+ * the parity of forward crossings of the ray p + t dir (t > 1e-7) with the
+ * triangles (Moeller-Trumbore, det tolerance 1e-9, barycentric -1e-7 /
+ * 1 + 1e-7), in the reference's floating-point operation order; out[i] = 1
+ * inside.  verts_dev [nv][3] float64, faces_dev [nt][3] int64, dir_host [3]
+ * (the reference normalises (1, 0.0073301, 0.0031337)).  Work goes on
+ * `stream` (NULL: null stream) and the call synchronises it; the temporary
+ * triangle bins are allocated and freed inside the call.
+ * ------------------------------------------------------------------------- */
+int vx_inside_mask(void* stream, const double* verts_dev, int64_t nv, const int64_t* faces_dev,
+                   int64_t nt, const double* points_dev, int64_t np, const double* dir_host,
+                   uint8_t* out_dev);
+/* Same on the points of a voxel grid generated on the device
+ * (core/inside_worker.py:87-111): target 0 = voxel centres in element order
+ * (x slowest), 1 = nodes (x fastest); origin_host [3] is the grid's min
+ * corner. */
+int vx_inside_grid(void* stream, const double* verts_dev, int64_t nv, const int64_t* faces_dev,
+                   int64_t nt, int32_t nx, int32_t ny, int32_t nz, const double* origin_host,
+                   double vsize, int32_t target, const double* dir_host, uint8_t* out_dev);
+const char* vx_inside_last_error(void);
+
 /* Host-side element operator (the kernels' own code compiled for the CPU):
  * y24 = KE(nu) * u24 for one unit element in the reference's local DOF order
  * (core/fea.py:33-36), E = 1. */

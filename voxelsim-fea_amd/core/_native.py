@@ -117,6 +117,10 @@ SIGNATURES = {
     "vx_corrector_prepare": (ctypes.c_int, [_P, _I32, _P, _P, _P, _P]),
     "vx_corrector_train": (ctypes.c_int, [_P, _I32, _P, _P, _P, _P, _I32, _I32, ctypes.c_float,
                                           ctypes.c_uint64, _P]),
+    "vx_inside_mask": (ctypes.c_int, [_P, _P, _I64, _P, _I64, _P, _I64, _P, _P]),
+    "vx_inside_grid": (ctypes.c_int, [_P, _P, _I64, _P, _I64, _I32, _I32, _I32, _P, _D, _I32,
+                                      _P, _P]),
+    "vx_inside_last_error": (ctypes.c_char_p, []),
     "vxn_create": (ctypes.c_int, [_I32, _I32, _I32, ctypes.POINTER(_P)]),
     "vxn_destroy": (None, [_P]),
     "vxn_last_error": (ctypes.c_char_p, [_P]),

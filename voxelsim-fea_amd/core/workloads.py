@@ -333,3 +333,46 @@ WORKLOADS = {
     "uneven_parts": uneven_parts,
     "two_material": two_material,
 }
+
+
+# ---------------------------------------------------------------------------
+# triangle meshes for the voxelisation inside test
+# ---------------------------------------------------------------------------
+def box_mesh(lo, hi):
+    """Closed axis-aligned box: 8 vertices, 12 triangles."""
+    lo, hi = np.asarray(lo, float), np.asarray(hi, float)
+    v = np.array([[x, y, z] for z in (lo[2], hi[2]) for y in (lo[1], hi[1])
+                  for x in (lo[0], hi[0])])
+    f = np.array([[0, 2, 1], [1, 2, 3], [4, 5, 6], [5, 7, 6], [0, 1, 4], [1, 5, 4],
+                  [2, 6, 3], [3, 6, 7], [0, 4, 2], [2, 4, 6], [1, 3, 5], [3, 7, 5]],
+                 dtype=np.int64)
+    return v, f
+
+
+def icosphere(level=2, center=(0.0, 0.0, 0.0), radius=1.0):
+    """Closed sphere: an icosahedron subdivided ``level`` times
+    (20 * 4^level triangles), vertices on the sphere."""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    v = [[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t],
+         [0, 1, -t], [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]]
+    f = [[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4],
+         [11, 10, 2], [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8],
+         [3, 8, 9], [4, 9, 5], [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]]
+    verts = [np.asarray(p, float) / np.linalg.norm(p) for p in v]
+    for _ in range(level):
+        mid = {}
+        nf = []
+
+        def m(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in mid:
+                p = verts[a] + verts[b]
+                verts.append(p / np.linalg.norm(p))
+                mid[key] = len(verts) - 1
+            return mid[key]
+        for a, b, c in f:
+            ab, bc, ca = m(a, b), m(b, c), m(c, a)
+            nf += [[a, ab, ca], [b, bc, ab], [c, ca, bc], [ab, bc, ca]]
+        f = nf
+    V = np.asarray(verts) * float(radius) + np.asarray(center, float)
+    return V, np.asarray(f, dtype=np.int64)
