@@ -1639,7 +1639,8 @@ int vx_time_kernel(vx_handle s, int32_t kind, int32_t l, int32_t prec, int32_t r
     sc.cr = 0.1;
     if (kind == 0) return launch_tile<T, M_MATVEC, false>(s, l, x, nullptr, y, nullptr, nullptr);
     if (kind == 1) return launch_tile<T, M_JACOBI, false>(s, l, x, bb, y, nullptr, nullptr, &sc);
-    return launch_tile<T, M_CHEB, false>(s, l, x, bb, y, nullptr, nullptr, &sc);
+    // as inside the smoother, the new iterate overwrites the previous one
+    return launch_tile<T, M_CHEB, false>(s, l, x, bb, up, nullptr, nullptr, &sc);
   };
   if (prec == VX_FP64) {
     CHK(alloc_pcg64(s));
