@@ -148,7 +148,8 @@ int vx_solve_jacobi(vx_handle h, const double* E_dev, const uint8_t* active_dev,
 
 /* Single-operator entry points for parity tests.  All float64, level-l sized.
  * vx_matvec: MGSolver._apply  (masked K u, core/multigrid.py:318-341)
- * vx_smooth: MGSolver._smooth (iters damped-Jacobi sweeps, :367-402)
+ * vx_smooth: MGSolver._smooth (iters damped-Jacobi sweeps, :367-402; with
+ *            VX_SMOOTH_CHEBYSHEV: a Chebyshev smoothing step of degree iters)
  * vx_restrict: MGSolver._restrict (R = P^T then coarse mask, :404-416)
  * vx_prolong: MGSolver._prolongate (P e then fine mask, :418-424)
  * vx_vcycle: MGSolver._vcycle(0, b) (:426-436)

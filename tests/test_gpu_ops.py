@@ -93,3 +93,52 @@ def test_vcycle(dims, kw):
     assert _rel(g._vcycle(0, b), z) < 1e-11
     assert _rel(g._vcycle(0, b, precision="mixed"), z) < 1e-4
     g.close()
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4])
+@pytest.mark.parametrize("start", ["zero", "u"])
+def test_chebyshev_smoother_closed_form(k, start):
+    """k Chebyshev steps on D^-1 A over [ratio*lam, lam] leave the error
+    P_k(D^-1 A) e0 with P_k(x) = T_k((theta - x)/delta) / T_k(theta/delta):
+    checked against that matrix polynomial built densely on a tiny grid."""
+    from core.multigrid import MGSolver
+    from oracle.fea_ref import hex8_KE
+    from oracle.multigrid_ref import MGSolverRef
+    print(f"chebyshev closed form k={k} start={start}", flush=True)
+    rng = np.random.default_rng(10 + k)
+    nx, ny, nz = 3, 2, 3
+    fixed = np.arange(3 * (nx + 1) * (ny + 1))          # z = 0 plane
+    E = rng.uniform(0.5, 2.0, nx * ny * nz)
+    ratio = 0.3
+    ref = MGSolverRef(nx, ny, nz, fixed, max_levels=1)
+    ref.set_density(E)
+    n = 3 * (nx + 1) * (ny + 1) * (nz + 1)
+    A = np.stack([ref._Ku(0, np.eye(n)[j]) for j in range(n)], axis=1)
+    free = ref.levels[0]["free"]
+    f = np.flatnonzero(free)
+    Af = A[np.ix_(f, f)]
+    Dinv = 1.0 / np.diag(Af)
+    KE = hex8_KE(1.0, 0.3)
+    lam = np.linalg.eigvalsh(KE).max() / KE[0, 0]
+    a_, b_ = ratio * lam, lam
+    theta, delta = 0.5 * (b_ + a_), 0.5 * (b_ - a_)
+    M = Dinv[:, None] * Af
+    Y = (theta * np.eye(len(f)) - M) / delta
+    Tm, T = np.eye(len(f)), Y.copy()
+    tm, t = 1.0, theta / delta
+    for _ in range(k - 1):
+        Tm, T = T, 2 * Y @ T - Tm
+        tm, t = t, 2 * (theta / delta) * t - tm
+    P = T / t
+    b = rng.standard_normal(n)
+    b[np.setdiff1d(np.arange(n), f)] = 0.0
+    u0 = np.zeros(n) if start == "zero" else rng.standard_normal(n) * (free > 0)
+    xs = np.linalg.solve(Af, b[f])
+    want = np.zeros(n)
+    want[f] = xs - P @ (xs - u0[f])
+    mg = MGSolver(nx, ny, nz, fixed, compute_mode="GPU", smoother="chebyshev", cheb_ratio=ratio,
+                  max_levels=1)
+    mg.set_density(E)
+    got = mg._smooth(0, None if start == "zero" else u0, b, k, "fp64")
+    assert np.abs(got - want).max() <= 1e-10 * np.abs(want).max()
+    mg.close()

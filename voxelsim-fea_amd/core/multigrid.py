@@ -401,7 +401,8 @@ class MGSolver:
         return yd.cpu().numpy()
 
     def _smooth(self, li, u, b, iters, precision="fp64"):
-        """``iters`` damped-Jacobi sweeps (core/multigrid.py:367-402); u=None
+        """``iters`` damped-Jacobi sweeps (core/multigrid.py:367-402), or a
+        Chebyshev step of degree ``iters`` with smoother="chebyshev"; u=None
         starts from zero."""
         self._unpadded_only('_smooth')
         bd = _dev(b)

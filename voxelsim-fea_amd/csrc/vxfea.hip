@@ -1486,7 +1486,7 @@ int vx_smooth(vx_handle s, int32_t l, int32_t prec, const double* u, const doubl
   const long long n = 3 * L.Nn;
   if (prec == VX_FP64) {
     CHK(alloc_level_vectors<double>(s));
-    CHK(smooth_n<double>(s, l, u, b, iters, L.x64, L.t64, false, nullptr));
+    CHK(smooth_step<double>(s, l, u, b, iters, L.x64, L.t64, L.w64, false, nullptr));
     HIPCHK(hipMemcpyAsync(out, L.x64, sizeof(double) * n, hipMemcpyDeviceToDevice, s->stream));
   } else {
     CHK(alloc_pcg32(s));
@@ -1494,7 +1494,8 @@ int vx_smooth(vx_handle s, int32_t l, int32_t prec, const double* u, const doubl
     float* uu = s->r32;
     CHK(to_T<float>(s, b, bb, n));
     if (u) CHK(to_T<float>(s, u, uu, n));
-    CHK(smooth_n<float>(s, l, u ? uu : nullptr, bb, iters, L.x32, L.t32, false, nullptr));
+    CHK(smooth_step<float>(s, l, u ? uu : nullptr, bb, iters, L.x32, L.t32, L.w32, false,
+                           nullptr));
     CHK(from_T<float>(s, L.x32, out, n));
   }
   HIPCHK(hipStreamSynchronize(s->stream));
