@@ -32,7 +32,7 @@ sys.path.insert(0, os.path.join(ROOT, "voxelsim-fea_amd"))
 HBM_PEAK_GBS = 8000.0
 
 # the MI355X solver configuration measured as `value`
-OPT = dict(precision="mixed", smoother="chebyshev", cheb_ratio=0.1, system="active",
+OPT = dict(precision="mixed", smoother="chebyshev", cheb_ratio=0.1, system="active", autotune=True,
            inner_tol=1e-6)
 
 
@@ -407,6 +407,7 @@ def main():
     dev_mem_gb = (free0 - torch.cuda.mem_get_info()[0]) / 1e9
     rel_err = float(((u_dev - u64).abs().max() / u64.abs().max()).item())
     mixed_iters, mixed_outer = opt.last_iters, opt.last_outer_iters
+    tuned, tune_s = opt.tuned, opt.autotune_s
     mixed_true, mixed_conv = opt.last_true_resid_ratio, opt.last_converged
 
     # the same solve from host numpy arrays (the reference's calling form):
@@ -490,7 +491,9 @@ def main():
                         "learned corrector is measured beside it ('learned')",
         "mixed": {"iters": mixed_iters, "outer": mixed_outer,
                   "true_resid": mixed_true, "converged": mixed_conv,
-                  "rel_disp_diff_vs_fp64": rel_err},
+                  "rel_disp_diff_vs_fp64": rel_err,
+                  "autotuned_zchunk": {str(k): v for k, v in (tuned or {}).items()},
+                  "autotune_s": tune_s},
         "learned": learned,
         "pcie_inclusive_s": t_pcie,
         "device_mem_gb": dev_mem_gb,

@@ -313,6 +313,16 @@ int vx_inside_grid(void* stream, const double* verts_dev, int64_t nv, const int6
                    double vsize, int32_t target, const double* dir_host, uint8_t* out_dev);
 const char* vx_inside_last_error(void);
 
+/* Autotuner (no counterpart in the reference): for every level, time the
+ * tile kernels' z-chunk (node planes per work-group) over a candidate set,
+ * for the operator (matvec forms, class 0) and the smoother / residual forms
+ * (class 1), in the given precision, and keep the fastest for the handle's
+ * later launches.  Needs vx_set_fixed and vx_set_density; overwrites the
+ * handle's work vectors.  opt.z_chunk > 0 overrides it. */
+int vx_autotune(vx_handle h, int32_t precision, int32_t reps);
+/* The z-chunk the handle uses for a level, precision and class. */
+int vx_tuned_zchunk(vx_handle h, int32_t level, int32_t precision, int32_t cls);
+
 /* Host-side element operator (the kernels' own code compiled for the CPU):
  * y24 = KE(nu) * u24 for one unit element in the reference's local DOF order
  * (core/fea.py:33-36), E = 1. */

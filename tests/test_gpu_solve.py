@@ -243,3 +243,31 @@ def test_batched_solve_equals_single_solves(precision):
             assert _rel(Ub[q], ref[q][0]) < 1e-5
     single.close()
     batch.close()
+
+
+def test_autotuned_solver_same_solution():
+    """The z-chunk autotuner changes only the tiling: the solve is the same."""
+    import torch
+    from core import workloads as W
+    from core.multigrid import MGSolver
+    print("autotune: voids_block n=32", flush=True)
+    p = W.voids_block(n=32)
+    E = torch.tensor(p.E_stiffness, device="cuda")
+    A = torch.tensor(p.active.astype(np.uint8), device="cuda")
+    f = torch.tensor(p.force, device="cuda")
+    out = {}
+    for tune in (False, True):
+        mg = MGSolver(p.nx, p.ny, p.nz, p.fixed_dofs, nu=p.nu, compute_mode="GPU",
+                      precision="mixed", smoother="chebyshev", cheb_ratio=0.1, system="active",
+                      autotune=tune)
+        mg.set_density(E, active=A)
+        u = torch.empty_like(f)
+        mg.solve_device(f, u, tol=1e-8)
+        assert mg.last_converged
+        out[tune] = u.cpu().numpy()
+        if tune:
+            assert set(mg.tuned) == set(range(len(mg.levels)))
+            for zo, zs in mg.tuned.values():
+                assert 1 <= zo <= 65 and 1 <= zs <= 65
+        mg.close()
+    assert _rel(out[True], out[False]) < 1e-9

@@ -110,6 +110,8 @@ SIGNATURES = {
     "vx_drop_islands": (ctypes.c_int, [_P, _P, _P, _I64, _P, ctypes.POINTER(_I64)]),
     "vx_time_kernel": (ctypes.c_int, [_P, _I32, _I32, _I32, _I32, ctypes.POINTER(_D)]),
     "vx_element_apply_host": (None, [_D, _P, _P]),
+    "vx_autotune": (ctypes.c_int, [_P, _I32, _I32]),
+    "vx_tuned_zchunk": (ctypes.c_int, [_P, _I32, _I32, _I32]),
     "vx_set_corrector": (ctypes.c_int, [_P, _P, _I32, ctypes.c_float, _I32, _I32]),
     "vx_corrector_eval": (ctypes.c_int, [_P, _I32, _P, _P, _P]),
     "vx_capture": (ctypes.c_int, [_P, _I32, _P, _I32, _I32]),

@@ -67,7 +67,8 @@ class MGSolver:
                  min_elems=4, max_levels=6, compute_mode="AUTO", cpu_threads=0,
                  verbose=False, precision="fp64", coarse_sweeps=30, inner_tol=1e-5,
                  max_outer=20, z_chunk=0, smoother="jacobi", cheb_ratio=0.3,
-                 system="full", e_coarsen="mean", max_batch=1, flexible=False, pad=False):
+                 system="full", e_coarsen="mean", max_batch=1, flexible=False, pad=False,
+                 autotune=False):
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
         self.nelem = self.nx * self.ny * self.nz
         self.ndof = 3 * (self.nx + 1) * (self.ny + 1) * (self.nz + 1)
@@ -120,6 +121,9 @@ class MGSolver:
         self.last_outer_iters = 0
         self.last_solve_ms = float("nan")
         self.resid_history = []
+        self._autotune = bool(autotune)
+        self.tuned = None          # {level: (operator zc, smoother zc)} once tuned
+        self.autotune_s = 0.0
 
     # -- padding ------------------------------------------------------------
     def _pad_dof_ids(self, ids):
@@ -205,6 +209,12 @@ class MGSolver:
         em = -1.0 if self.system == "active" else float(e_min)
         self._check(self._lib.vx_set_density(self._h, _native.ptr(self._E_dev),
                                               _native.ptr(self._active_dev), em))
+        if self._autotune and self.tuned is None:
+            # once per solver: the kernels' timing does not depend on the values
+            import time
+            t0 = time.perf_counter()
+            self.tuned = self.autotune()
+            self.autotune_s = time.perf_counter() - t0
 
     # -- solve -------------------------------------------------------------
     def solve_device(self, f_dev, u_dev, tol=1e-4, max_cg=None, x0_dev=None,
@@ -315,6 +325,17 @@ class MGSolver:
             print(f"[VoxelSim FEA] MG solve: {self.last_iters} CG iters "
                   f"(converged={self.last_converged}), {self.last_solve_ms:.1f} ms on device")
         return u_dev.cpu().numpy()
+
+    # -- autotuner ------------------------------------------------------------
+    def autotune(self, precision=None, reps=5):
+        """Pick each level's tile z-chunk by timing (needs the density set);
+        returns {level: (operator zc, smoother zc)}."""
+        prec = _PRECISION[precision or self.precision]
+        self._stream_bind()
+        self._check(self._lib.vx_autotune(self._h, prec, int(reps)))
+        return {li: (self._lib.vx_tuned_zchunk(self._h, li, prec, 0),
+                     self._lib.vx_tuned_zchunk(self._h, li, prec, 1))
+                for li in range(len(self.levels))}
 
     # -- learned coarse-grid corrector --------------------------------------
     def level_nodes(self, li):

@@ -41,6 +41,9 @@ struct Level {
   uint8_t* mask_fixed = nullptr;   // injected fixed-DOF mask (set_fixed)
   double* mi64 = nullptr;          // inverse Jacobi diagonal per node
   float* mi32 = nullptr;
+  // autotuned node planes per work-group: [precision 0 FP64 / 1 FP32][class
+  // 0 operator (matvec forms) / 1 smoother and residual forms]; 0 = rule
+  int zc_tuned[2][2] = {{0, 0}, {0, 0}};
 };
 
 template <typename T> struct Sel;
@@ -159,8 +162,9 @@ int ew_grid(long long n) {
   return (int)std::max<long long>(1, std::min<long long>(g, EW_BLOCKS));
 }
 
-int zchunk(const vx_solver* s, const Level& L) {
+int zchunk(const vx_solver* s, const Level& L, int prec = -1, int cls = 0) {
   if (s->opt.z_chunk > 0) return s->opt.z_chunk;
+  if (prec >= 0 && L.zc_tuned[prec][cls] > 0) return L.zc_tuned[prec][cls];
   // enough work-groups to fill 256 CUs several times, long enough columns
   // to amortise the halo layer.
   const long long cols = (long long)((L.Lx + OWN - 1) / OWN) * ((L.Ly + OWN - 1) / OWN);
@@ -197,7 +201,7 @@ int launch_tile_L(vx_solver* s, Level& L, const T* u, const T* b, T* out, double
   TileArgs<T> a;
   a.nx = L.nx; a.ny = L.ny; a.nz = L.nz;
   a.Lx = L.Lx; a.Ly = L.Ly; a.Lz = L.Lz;
-  a.zc = zchunk(s, L);
+  a.zc = zchunk(s, L, std::is_same<T, float>::value ? 1 : 0, MODE == M_MATVEC ? 0 : 1);
   a.nb = s->nb;
   a.vstride = 3 * L.Nn;
   a.u = u; a.E = Sel<T>::E(L); a.mask = L.mask; a.b = b; a.out = out;
@@ -1680,6 +1684,105 @@ int vx_time_kernel(vx_handle s, int32_t kind, int32_t l, int32_t prec, int32_t r
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// autotuner of the tile kernels' z-chunk
+// ===========================================================================
+extern "C" int vx_autotune(vx_handle s, int32_t precision, int32_t reps) {
+  if (!s) return VX_ERR_ARG;
+  CHK(require_ready(s));
+  if (reps < 1) reps = 5;
+  const int pi = precision == VX_FP64 ? 0 : 1;
+  if (pi == 0) CHK(alloc_pcg64(s));
+  else CHK(alloc_pcg32(s));
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  const int cands[] = {2, 3, 4, 6, 8, 10, 12, 14, 16, 18, 20, 24, 29, 33, 43, 65};
+  // grow the partial buffer to the largest work-group count a candidate gives
+  long long need = s->n_partial;
+  for (auto& L : s->lv)
+    for (int c : cands) {
+      const dim3 g = tile_grid(s, L, std::min(c, L.Lz));
+      need = std::max(need, (long long)g.x * g.y * g.z * s->opt.max_batch);
+    }
+  if (need > s->n_partial) {
+    s->partial = nullptr;   // the smaller buffer stays owned by s->allocs
+    CHK(dalloc(s, &s->partial, (size_t)need));
+    s->n_partial = (int)need;
+  }
+  int rc = VX_OK;
+  s->nb = 1;
+  for (size_t li = 0; li < s->lv.size() && rc == VX_OK; ++li) {
+    Level& L = s->lv[li];
+    for (int cls = 0; cls < 2 && rc == VX_OK; ++cls) {
+      L.zc_tuned[pi][cls] = 0;
+      std::vector<int> cs;
+      cs.push_back(zchunk(s, L));   // the rule's choice competes too
+      for (int c : cands)
+        if (c < L.Lz && std::find(cs.begin(), cs.end(), c) == cs.end()) cs.push_back(c);
+      float best = INFINITY;
+      int best_zc = 0;
+      for (int zc : cs) {
+        // the partial buffer must hold this launch's work-groups
+        const dim3 g = tile_grid(s, L, zc);
+        if ((long long)g.x * g.y * g.z * s->opt.max_batch > s->n_partial) continue;
+        L.zc_tuned[pi][cls] = zc;
+        auto launch = [&]() -> int {
+          SmoothCoef sc;
+          sc.omega = s->opt.omega;
+          sc.cu = 0.5;
+          sc.cr = 0.1;
+          if (pi == 0) {
+            double* x = li == 0 ? s->p64 : L.x64;
+            double* y = li == 0 ? s->Ap64 : L.t64;
+            double* bb = li == 0 ? s->r64 : L.b64;
+            sc.up = L.w64;
+            if (cls == 0)
+              return launch_tile<double, M_MATVEC, false>(s, (int)li, x, nullptr, y, nullptr,
+                                                          nullptr);
+            return launch_tile<double, M_CHEB, false>(s, (int)li, x, bb, L.w64, nullptr, nullptr,
+                                                      &sc);
+          }
+          float* x = li == 0 ? s->p32 : L.x32;
+          float* y = li == 0 ? s->Ap32 : L.t32;
+          float* bb = li == 0 ? s->r32 : L.b32;
+          sc.up = L.w32;
+          if (cls == 0)
+            return launch_tile<float, M_MATVEC, false>(s, (int)li, x, nullptr, y, nullptr,
+                                                       nullptr);
+          return launch_tile<float, M_CHEB, false>(s, (int)li, x, bb, L.w32, nullptr, nullptr,
+                                                   &sc);
+        };
+        rc = launch();   // warm-up
+        if (rc != VX_OK) break;
+        if (hipEventRecord(a, s->stream) != hipSuccess) { rc = fail(s, VX_ERR_HIP, "event"); break; }
+        for (int r = 0; r < reps && rc == VX_OK; ++r) rc = launch();
+        if (rc != VX_OK) break;
+        float ms = 0.f;
+        if (hipEventRecord(b, s->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+            hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+          rc = fail(s, VX_ERR_HIP, "event timing");
+          break;
+        }
+        if (ms < best) {
+          best = ms;
+          best_zc = zc;
+        }
+      }
+      L.zc_tuned[pi][cls] = best_zc;
+    }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return rc;
+}
+
+extern "C" int vx_tuned_zchunk(vx_handle s, int32_t level, int32_t precision, int32_t cls) {
+  if (!s || level < 0 || level >= (int)s->lv.size() || cls < 0 || cls > 1) return VX_ERR_ARG;
+  const Level& L = s->lv[level];
+  return zchunk(s, L, precision == VX_FP64 ? 0 : 1, cls);
+}
 
 // ===========================================================================
 // learned coarse-grid corrector: attach, capture, targets, training
