@@ -267,7 +267,7 @@ def test_autotuned_solver_same_solution():
         out[tune] = u.cpu().numpy()
         if tune:
             assert set(mg.tuned) == set(range(len(mg.levels)))
-            for zo, zs in mg.tuned.values():
-                assert 1 <= zo <= 65 and 1 <= zs <= 65
+            for zcs in mg.tuned.values():
+                assert len(zcs) == 3 and all(1 <= z <= 65 for z in zcs)
         mg.close()
     assert _rel(out[True], out[False]) < 1e-9

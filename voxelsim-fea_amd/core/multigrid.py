@@ -329,12 +329,11 @@ class MGSolver:
     # -- autotuner ------------------------------------------------------------
     def autotune(self, precision=None, reps=5):
         """Pick each level's tile z-chunk by timing (needs the density set);
-        returns {level: (operator zc, smoother zc)}."""
+        returns {level: (operator zc, smoother zc, smoother-with-dot zc)}."""
         prec = _PRECISION[precision or self.precision]
         self._stream_bind()
         self._check(self._lib.vx_autotune(self._h, prec, int(reps)))
-        return {li: (self._lib.vx_tuned_zchunk(self._h, li, prec, 0),
-                     self._lib.vx_tuned_zchunk(self._h, li, prec, 1))
+        return {li: tuple(self._lib.vx_tuned_zchunk(self._h, li, prec, c) for c in range(3))
                 for li in range(len(self.levels))}
 
     # -- learned coarse-grid corrector --------------------------------------

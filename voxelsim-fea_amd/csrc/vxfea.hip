@@ -42,8 +42,9 @@ struct Level {
   double* mi64 = nullptr;          // inverse Jacobi diagonal per node
   float* mi32 = nullptr;
   // autotuned node planes per work-group: [precision 0 FP64 / 1 FP32][class
-  // 0 operator (matvec forms) / 1 smoother and residual forms]; 0 = rule
-  int zc_tuned[2][2] = {{0, 0}, {0, 0}};
+  // 0 operator (matvec forms), 1 smoother / residual forms, 2 smoother forms
+  // with the dot-product partials]; 0 = rule
+  int zc_tuned[2][3] = {{0, 0, 0}, {0, 0, 0}};
 };
 
 template <typename T> struct Sel;
@@ -201,7 +202,8 @@ int launch_tile_L(vx_solver* s, Level& L, const T* u, const T* b, T* out, double
   TileArgs<T> a;
   a.nx = L.nx; a.ny = L.ny; a.nz = L.nz;
   a.Lx = L.Lx; a.Ly = L.Ly; a.Lz = L.Lz;
-  a.zc = zchunk(s, L, std::is_same<T, float>::value ? 1 : 0, MODE == M_MATVEC ? 0 : 1);
+  a.zc = zchunk(s, L, std::is_same<T, float>::value ? 1 : 0,
+                MODE == M_MATVEC ? 0 : (DOT ? 2 : 1));
   a.nb = s->nb;
   a.vstride = 3 * L.Nn;
   a.u = u; a.E = Sel<T>::E(L); a.mask = L.mask; a.b = b; a.out = out;
@@ -257,8 +259,15 @@ int sweep(vx_solver* s, int li, const T* u, const T* b, T* out, bool dot, int* n
           double omega) {
   Level& L = s->lv[li];
   if (!u) {
+    // float4 path: every right-hand side's vector and the per-node arrays
+    // start 16-byte aligned (allocation alignment; rhs stride 3 Nn floats)
+    const bool aligned =
+        ((reinterpret_cast<uintptr_t>(b) | reinterpret_cast<uintptr_t>(out) |
+          reinterpret_cast<uintptr_t>(Sel<T>::mi(L)) | reinterpret_cast<uintptr_t>(L.mask)) & 15) ==
+            0 &&
+        (s->nb == 1 || (3 * L.Nn) % 4 == 0);
     hipLaunchKernelGGL(k_zero_step<T>, dim3(ew_grid(L.Nn), s->nb), dim3(EW_THREADS), 0, s->stream,
-                       L.Nn, L.mask, Sel<T>::mi(L), b, out, (T)omega);
+                       L.Nn, L.mask, Sel<T>::mi(L), b, out, (T)omega, aligned ? 1 : 0);
     LAUNCHCHK();
     if (dot) CHK(dot_partials<T>(s, 3 * L.Nn, out, b, n_part));
     return VX_OK;
@@ -1698,7 +1707,7 @@ extern "C" int vx_autotune(vx_handle s, int32_t precision, int32_t reps) {
   hipEvent_t a, b;
   HIPCHK(hipEventCreate(&a));
   HIPCHK(hipEventCreate(&b));
-  const int cands[] = {2, 3, 4, 6, 8, 10, 12, 14, 16, 18, 20, 24, 29, 33, 43, 65};
+  const int cands[] = {1, 2, 3, 4, 6, 8, 10, 12, 14, 16, 18, 20, 24, 29, 33, 43, 65};
   // grow the partial buffer to the largest work-group count a candidate gives
   long long need = s->n_partial;
   for (auto& L : s->lv)
@@ -1715,7 +1724,7 @@ extern "C" int vx_autotune(vx_handle s, int32_t precision, int32_t reps) {
   s->nb = 1;
   for (size_t li = 0; li < s->lv.size() && rc == VX_OK; ++li) {
     Level& L = s->lv[li];
-    for (int cls = 0; cls < 2 && rc == VX_OK; ++cls) {
+    for (int cls = 0; cls < 3 && rc == VX_OK; ++cls) {
       L.zc_tuned[pi][cls] = 0;
       std::vector<int> cs;
       cs.push_back(zchunk(s, L));   // the rule's choice competes too
@@ -1728,31 +1737,34 @@ extern "C" int vx_autotune(vx_handle s, int32_t precision, int32_t reps) {
         const dim3 g = tile_grid(s, L, zc);
         if ((long long)g.x * g.y * g.z * s->opt.max_batch > s->n_partial) continue;
         L.zc_tuned[pi][cls] = zc;
-        auto launch = [&]() -> int {
+        // the variants the solve launches: class 0 the PCG operator (direction
+        // update fused, p.Ap partials), 1 a smoothing step, 2 the last
+        // smoothing step of the V-cycle (with its dot-product partials)
+        auto variant = [&](auto* x, auto* y, auto* bb, auto* w, auto* q) -> int {
+          using T = std::remove_pointer_t<decltype(x)>;
           SmoothCoef sc;
           sc.omega = s->opt.omega;
           sc.cu = 0.5;
           sc.cr = 0.1;
-          if (pi == 0) {
-            double* x = li == 0 ? s->p64 : L.x64;
-            double* y = li == 0 ? s->Ap64 : L.t64;
-            double* bb = li == 0 ? s->r64 : L.b64;
-            sc.up = L.w64;
-            if (cls == 0)
-              return launch_tile<double, M_MATVEC, false>(s, (int)li, x, nullptr, y, nullptr,
-                                                          nullptr);
-            return launch_tile<double, M_CHEB, false>(s, (int)li, x, bb, L.w64, nullptr, nullptr,
-                                                      &sc);
+          sc.up = w;
+          int np = 0;
+          if (cls == 0) {
+            PUpd pu;
+            pu.pold = q;
+            pu.pnew = w;
+            return launch_tile_L<T, M_MATVEC, true, true>(s, L, x, nullptr, y, s->partial, &np,
+                                                          nullptr, &pu);
           }
-          float* x = li == 0 ? s->p32 : L.x32;
-          float* y = li == 0 ? s->Ap32 : L.t32;
-          float* bb = li == 0 ? s->r32 : L.b32;
-          sc.up = L.w32;
-          if (cls == 0)
-            return launch_tile<float, M_MATVEC, false>(s, (int)li, x, nullptr, y, nullptr,
-                                                       nullptr);
-          return launch_tile<float, M_CHEB, false>(s, (int)li, x, bb, L.w32, nullptr, nullptr,
-                                                   &sc);
+          if (cls == 1)
+            return launch_tile<T, M_CHEB, false>(s, (int)li, x, bb, w, nullptr, nullptr, &sc);
+          return launch_tile<T, M_CHEB, true>(s, (int)li, x, bb, w, s->partial, &np, &sc);
+        };
+        auto launch = [&]() -> int {
+          if (pi == 0)
+            return variant(li == 0 ? s->p64 : L.x64, li == 0 ? s->Ap64 : L.t64,
+                           li == 0 ? s->r64 : L.b64, L.w64, li == 0 ? s->q64 : L.t64);
+          return variant(li == 0 ? s->p32 : L.x32, li == 0 ? s->Ap32 : L.t32,
+                         li == 0 ? s->r32 : L.b32, L.w32, li == 0 ? s->q32 : L.t32);
         };
         rc = launch();   // warm-up
         if (rc != VX_OK) break;
@@ -1779,7 +1791,7 @@ extern "C" int vx_autotune(vx_handle s, int32_t precision, int32_t reps) {
 }
 
 extern "C" int vx_tuned_zchunk(vx_handle s, int32_t level, int32_t precision, int32_t cls) {
-  if (!s || level < 0 || level >= (int)s->lv.size() || cls < 0 || cls > 1) return VX_ERR_ARG;
+  if (!s || level < 0 || level >= (int)s->lv.size() || cls < 0 || cls > 2) return VX_ERR_ARG;
   const Level& L = s->lv[level];
   return zchunk(s, L, precision == VX_FP64 ? 0 : 1, cls);
 }
