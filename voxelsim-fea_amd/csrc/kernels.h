@@ -622,6 +622,74 @@ __global__ void k_prolong_add3(int Lfx, int Lfy, int Lfz, const T* __restrict__ 
   if (mk & 4u) x[3 * i + 2] += s2;
 }
 
+// (P e)[3i+c] at fine node i = (ix, iy, iz) of a fine grid (Lfx, Lfy, Lfz)
+template <typename T>
+__device__ __forceinline__ void prolong_at(int ix, int iy, int iz, int Lcx, int Lcy,
+                                           const T* __restrict__ ec, T& s0, T& s1, T& s2) {
+  const int cx = ix >> 1, cy = iy >> 1, cz = iz >> 1;
+  const int ox = ix & 1, oy = iy & 1, oz = iz & 1;
+  const T w = (ox ? T(0.5) : T(1)) * (oy ? T(0.5) : T(1)) * (oz ? T(0.5) : T(1));
+  s0 = s1 = s2 = T(0);
+  for (int kz = 0; kz <= oz; ++kz)
+    for (int ky = 0; ky <= oy; ++ky) {
+      const T* row = ec + ((long long)(cz + kz) * Lcy + (cy + ky)) * Lcx * 3;
+      for (int kx = 0; kx <= ox; ++kx) {
+        const T* q = row + 3 * (cx + kx);
+        s0 += w * q[0];
+        s1 += w * q[1];
+        s2 += w * q[2];
+      }
+    }
+}
+
+// k_prolong_add3 over the flat node order, four nodes per thread through
+// float4 / uint32 accesses of x and the mask (x 16-byte aligned per right-
+// hand side; blockIdx.y = rhs); the < 4 tail nodes one by one.
+__global__ void k_prolong_add_v4(int Lfx, int Lfy, int Lfz, const float* __restrict__ ec,
+                                 const uint8_t* __restrict__ fmask, float* __restrict__ x) {
+  const int Lcx = (Lfx + 1) / 2, Lcy = (Lfy + 1) / 2, Lcz = (Lfz + 1) / 2;
+  const unsigned Nn = (unsigned)Lfx * Lfy * Lfz;
+  ec += (long long)blockIdx.y * 3 * Lcx * Lcy * Lcz;
+  x += (long long)blockIdx.y * 3 * (long long)Nn;
+  const unsigned ng = Nn >> 2;
+  const unsigned stride = gridDim.x * blockDim.x;
+  for (unsigned g = blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += stride) {
+    const unsigned mk4 = reinterpret_cast<const unsigned*>(fmask)[g];
+    float4* x4 = reinterpret_cast<float4*>(x) + 3 * (long long)g;
+    const float4 q0 = x4[0], q1 = x4[1], q2 = x4[2];
+    float v[12] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const unsigned n = 4 * g + k;
+      const unsigned row = n / (unsigned)Lfx;
+      const int ix = (int)(n - row * (unsigned)Lfx);
+      const int iz = (int)(row / (unsigned)Lfy);
+      const int iy = (int)(row - (unsigned)iz * (unsigned)Lfy);
+      float s0, s1, s2;
+      prolong_at<float>(ix, iy, iz, Lcx, Lcy, ec, s0, s1, s2);
+      const unsigned mk = (mk4 >> (8 * k)) & 0xFFu;
+      if (mk & 1u) v[3 * k] += s0;
+      if (mk & 2u) v[3 * k + 1] += s1;
+      if (mk & 4u) v[3 * k + 2] += s2;
+    }
+    x4[0] = make_float4(v[0], v[1], v[2], v[3]);
+    x4[1] = make_float4(v[4], v[5], v[6], v[7]);
+    x4[2] = make_float4(v[8], v[9], v[10], v[11]);
+  }
+  for (unsigned n = 4 * ng + blockIdx.x * blockDim.x + threadIdx.x; n < Nn; n += stride) {
+    const unsigned row = n / (unsigned)Lfx;
+    const int ix = (int)(n - row * (unsigned)Lfx);
+    const int iz = (int)(row / (unsigned)Lfy);
+    const int iy = (int)(row - (unsigned)iz * (unsigned)Lfy);
+    float s0, s1, s2;
+    prolong_at<float>(ix, iy, iz, Lcx, Lcy, ec, s0, s1, s2);
+    const unsigned mk = fmask[n];
+    if (mk & 1u) x[3 * n] += s0;
+    if (mk & 2u) x[3 * n + 1] += s1;
+    if (mk & 4u) x[3 * n + 2] += s2;
+  }
+}
+
 // E of level l+1 = 2 * mean of the 8 children (scaled E carries 2^l;
 // core/multigrid.py:287-292).  x-fastest layout on both levels.
 // mode 0: arithmetic mean (the reference rule); 1: geometric mean; 2:
