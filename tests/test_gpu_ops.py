@@ -83,7 +83,8 @@ def test_smooth_restrict_prolong(dims):
 
 
 @pytest.mark.parametrize("dims,kw", [((16, 8, 8), {}), ((8, 8, 8), {"max_levels": 3}),
-                                     ((6, 10, 4), {"n_smooth": 3}), ((5, 7, 3), {})])
+                                     ((6, 10, 4), {"n_smooth": 3}), ((5, 7, 3), {}),
+                                     ((160, 12, 4), {"max_levels": 2})])
 def test_vcycle(dims, kw):
     print("case vcycle", dims, kw, flush=True)
     E, fixed, ndof, rng = _problem(*dims, seed=5)
