@@ -110,11 +110,14 @@ def copy_bandwidth(nbytes=2 << 30, reps=10):
     return 2 * nbytes / dt / 1e9
 
 
-def kernel_only(p, E_dev, act_dev, MGSolver):
+def kernel_only(p, E_dev, act_dev, MGSolver, z_chunk):
     """The level-0 FP32 Chebyshev tile kernel 20 times, then a plain 1 GiB
-    FP32 copy 5 times (the reference dispatch for counter scaling)."""
+    FP32 copy 5 times (the reference dispatch for counter scaling).  The
+    z-chunk is fixed (the autotuner's usual pick at 256^3) so that every
+    counter pass sees the same launch."""
     import torch
-    opt = MGSolver(p.nx, p.ny, p.nz, p.fixed_dofs, nu=p.nu, compute_mode="GPU", **OPT)
+    opt = MGSolver(p.nx, p.ny, p.nz, p.fixed_dofs, nu=p.nu, compute_mode="GPU",
+                   **dict(OPT, autotune=False, z_chunk=z_chunk))
     opt.set_density(E_dev, active=act_dev)
     ms = opt.time_kernel("chebyshev", 0, "mixed", 20)
     a = torch.ones((1 << 30) // 4, dtype=torch.float32, device="cuda")
@@ -293,6 +296,8 @@ def main():
                              "uneven_parts", "inside"],
                     help="two_material is the headline line; the others print their own "
                          "per-configuration measurement")
+    ap.add_argument("--z-chunk", type=int, default=10,
+                    help="--kernel-only: node planes per work-group of the timed kernel")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip training / measuring the learned corrector")
     ap.add_argument("--oracle-full", action="store_true",
@@ -327,7 +332,7 @@ def main():
     tol = 1e-6
 
     if args.kernel_only:
-        kernel_only(p, E_dev, act_dev, MGSolver)
+        kernel_only(p, E_dev, act_dev, MGSolver, args.z_chunk)
         return
 
     def timed(fn):
