@@ -318,7 +318,8 @@ const char* vx_inside_last_error(void);
  * for the operator (matvec forms, class 0; timed in the PCG's fused
  * direction-update form), the smoother / residual forms (class 1) and the
  * smoother forms with dot-product partials (class 2), in the given
- * precision, and keep the fastest for the handle's later launches.  Needs vx_set_fixed and vx_set_density; overwrites the
+ * precision, timed with the handle's max_batch right-hand sides per launch,
+ * and keep the fastest for the handle's later launches.  Needs vx_set_fixed and vx_set_density; overwrites the
  * handle's work vectors.  opt.z_chunk > 0 overrides it. */
 int vx_autotune(vx_handle h, int32_t precision, int32_t reps);
 /* The z-chunk the handle uses for a level, precision and class. */

@@ -1737,7 +1737,9 @@ extern "C" int vx_autotune(vx_handle s, int32_t precision, int32_t reps) {
     s->n_partial = (int)need;
   }
   int rc = VX_OK;
-  s->nb = 1;
+  // timed at the handle's full batch: the work-group count that the z-chunk
+  // trades against scales with the right-hand sides of a launch
+  s->nb = s->opt.max_batch;
   for (size_t li = 0; li < s->lv.size() && rc == VX_OK; ++li) {
     Level& L = s->lv[li];
     for (int cls = 0; cls < 3 && rc == VX_OK; ++cls) {
@@ -1803,6 +1805,7 @@ extern "C" int vx_autotune(vx_handle s, int32_t precision, int32_t reps) {
   }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
+  s->nb = 1;
   return rc;
 }
 
