@@ -271,3 +271,33 @@ def test_autotuned_solver_same_solution():
                 assert len(zcs) == 3 and all(1 <= z <= 65 for z in zcs)
         mg.close()
     assert _rel(out[True], out[False]) < 1e-9
+
+
+def test_final_payload_keys_and_round_trip(tmp_path):
+    """The worker's final.pkl dict (core/solve_worker.py:411-431): every key,
+    the dtypes the UI reads, and the atomic write."""
+    import pickle
+    from core.postsolve import final_payload, write_final
+    print("case final payload", flush=True)
+    g = load_golden("voids16")
+    vf = _fea(g)
+    u = vf.solve(g["E_pa"] * float(g["vsize"]), g["force"], tol=float(g["tol"]))
+    dims = (int(g["nx"]), int(g["ny"]), int(g["nz"]))
+    pay = final_payload(vf, u, g["E_pa"], float(g["vsize"]), 250e6, dims, (0.0, 0.0, 0.0),
+                        g["active"], n_islands_removed=0, tol=float(g["tol"]))
+    keys = {"dims", "origin", "vsize", "active3d", "stress3d", "stress_vmax", "max_stress_pa",
+            "safety3d", "safety_cap", "min_safety_factor", "yield_strength_pa",
+            "displacement3d", "displacement_vmax_m", "max_displacement_m", "cg_iters",
+            "cg_converged", "cg_resid_history", "cg_tol", "reliable", "n_islands_removed",
+            "compute_label"}
+    assert set(pay) == keys
+    for k in ("stress3d", "safety3d", "displacement3d"):
+        assert pay[k].dtype == np.float32 and pay[k].shape == dims
+    assert pay["active3d"].shape == dims and pay["cg_iters"] == vf.last_iters
+    assert len(pay["cg_resid_history"]) == vf.last_iters
+    path = str(tmp_path / "final.pkl")
+    write_final(path, pay)
+    with open(path, "rb") as fh:        # a file this test wrote itself
+        back = pickle.load(fh)
+    assert set(back) == keys and np.array_equal(back["stress3d"], pay["stress3d"])
+    vf.close()

@@ -51,3 +51,46 @@ def final_fields(vfea, u_full, E_pa, vsize, yield_pa, converged=True):
         "max_displacement_m": float(sc[2]),
         "reliable": bool(converged) and float(sc[2]) <= 0.5 * bbox,
     }
+
+
+def final_payload(vfea, u_full, E_pa, vsize, yield_pa, dims, origin, active3d,
+                  n_islands_removed=0, tol=1e-6):
+    """The solve worker's complete ``final.pkl`` dict (core/solve_worker.py:411-431):
+    the device fields of ``final_fields`` plus the grid, the CG record of the
+    solve and the compute label."""
+    f = final_fields(vfea, u_full, E_pa, vsize, yield_pa, converged=bool(vfea.last_converged))
+    if float(yield_pa) <= 0:
+        f["min_safety_factor"] = None
+    return {
+        "dims": tuple(int(d) for d in dims), "origin": tuple(float(o) for o in origin),
+        "vsize": float(vsize),
+        "active3d": np.asarray(active3d, dtype=bool).reshape(tuple(int(d) for d in dims)),
+        "stress3d": f["stress3d"].astype(np.float32),
+        "stress_vmax": f["stress_vmax"],
+        "max_stress_pa": f["max_stress_pa"],
+        "safety3d": f["safety3d"].astype(np.float32),
+        "safety_cap": f["safety_cap"],
+        "min_safety_factor": f["min_safety_factor"],
+        "yield_strength_pa": float(yield_pa),
+        "displacement3d": f["displacement3d"].astype(np.float32),
+        "displacement_vmax_m": f["displacement_vmax_m"],
+        "max_displacement_m": f["max_displacement_m"],
+        "cg_iters": int(vfea.last_iters),
+        "cg_converged": bool(vfea.last_converged),
+        "cg_resid_history": list(vfea.resid_history),
+        "cg_tol": float(tol),
+        "reliable": f["reliable"],
+        "n_islands_removed": int(n_islands_removed),
+        "compute_label": vfea.plan.label,
+    }
+
+
+def write_final(path, payload):
+    """Write ``payload`` the way the worker writes final.pkl: to a temporary
+    file, then renamed over the target (core/solve_worker.py:189-193)."""
+    import os
+    import pickle
+    tmp = path + ".tmp"
+    with open(tmp, "wb") as fh:
+        pickle.dump(payload, fh)
+    os.replace(tmp, path)
