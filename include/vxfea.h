@@ -39,7 +39,12 @@ enum {
 enum { VX_FP64 = 0, VX_MIXED = 1 };
 /* Smoother inside the V-cycle. */
 enum { VX_SMOOTH_JACOBI = 0, VX_SMOOTH_CHEBYSHEV = 1 };
-/* Coarsest-level treatment. */
+/* Coarsest-level treatment: coarse_sweeps smoothing steps from zero
+ * (core/multigrid.py:428-429), or an FP64 direct solve: the inverse of the
+ * coarsest operator on its free DOFs, factored on the host at
+ * vx_set_density and applied as a dense FP64 product in every V-cycle (at
+ * most 1500 free DOFs, and only when that block is positive definite;
+ * otherwise the sweeps are used and vx_info.coarse_direct is 0). */
 enum { VX_COARSE_SWEEPS = 0, VX_COARSE_DIRECT = 1 };
 
 typedef struct vx_solver* vx_handle;
@@ -81,7 +86,7 @@ typedef struct {
   double solve_ms;            /* wall time of the solve on the handle's stream */
   int32_t corrector;          /* learned corrector: 0 not attached, 1 used throughout,
                                  2 switched off by the FP64 true-residual check */
-  int32_t reserved;
+  int32_t coarse_direct;      /* 1: the coarsest level was solved directly (VX_COARSE_DIRECT) */
 } vx_info;
 
 /* Called synchronously on the caller's thread after every PCG iteration

@@ -8,6 +8,15 @@ its element stiffness by ``2**l`` (:161-162); coarse E is the 2x2x2 mean
 (:255-278); R = P^T unscaled (:404-416); V(n_smooth, n_smooth) with damped
 Jacobi and a 30-sweep coarsest level (:426-436); Fletcher-Reeves PCG with the
 stall guard (:439-583).
+
+``coarse="direct"`` replaces the coarsest level's 30 sweeps by the exact
+solve of that level's operator on its free DOFs (no counterpart in the
+reference, which only smooths there).  The dense matrix is assembled from the
+same scaled element stiffness as ``_apply``; ``coarse_direct_factor`` keeps
+the sweeps when the free block has more than 1500 DOFs or a Cholesky pivot
+is not above 1e-13 times the largest diagonal entry.  Pinned by
+tests/test_oracle_mg.py: the dense matrix against ``_apply`` column by
+column, and K x = b on the free DOFs for the solve.
 """
 
 import numpy as np
@@ -54,6 +63,34 @@ def level_dims(nx, ny, nz, max_levels=6, min_elems=4):
     return dims
 
 
+def coarse_direct_factor(K, free, max_free=1500):
+    """Cholesky factor of K restricted to the free DOFs, as ``(free, L)``, or
+    None when there are more than ``max_free`` free DOFs or a pivot is not
+    above 1e-13 times the largest diagonal entry."""
+    idx = np.flatnonzero(free)
+    nf = idx.size
+    if nf == 0 or nf > max_free:
+        return None
+    A = K[np.ix_(idx, idx)].copy()
+    dmax = np.diag(A).max()
+    L = np.zeros_like(A)
+    for j in range(nf):
+        d = A[j, j] - L[j, :j] @ L[j, :j]
+        if not d > 1e-13 * dmax:
+            return None
+        L[j, j] = np.sqrt(d)
+        L[j + 1:, j] = (A[j + 1:, j] - L[j + 1:, :j] @ L[j, :j]) / L[j, j]
+    return idx, L
+
+
+def _forward(L, b):
+    """Solve L y = b for lower-triangular L."""
+    y = np.zeros_like(b)
+    for i in range(b.size):
+        y[i] = (b[i] - L[i, :i] @ y[:i]) / L[i, i]
+    return y
+
+
 class StallError(RuntimeError):
     """The MG-PCG stall/divergence guard fired (core/multigrid.py:524-531)."""
 
@@ -62,7 +99,10 @@ class MGSolverRef:
     """Restatement of ``core.multigrid.MGSolver`` on numpy."""
 
     def __init__(self, nx, ny, nz, fixed_dofs, nu=0.3, n_smooth=2, omega=0.6,
-                 min_elems=4, max_levels=6, pool_workers=0, verbose=False):
+                 min_elems=4, max_levels=6, pool_workers=0, verbose=False,
+                 coarse="sweeps"):
+        self.coarse = coarse
+        self.coarse_chol = None
         self.nx, self.ny, self.nz = nx, ny, nz
         self.ndof = 3 * (nx + 1) * (ny + 1) * (nz + 1)
         self.n_smooth, self.omega = n_smooth, omega
@@ -130,6 +170,28 @@ class MGSolverRef:
                             weights=(scaled[:, None] * self.diagKE[None, :]).ravel(),
                             minlength=lv["ndof"])
             lv["Minv"] = 1.0 / np.where(d == 0, 1.0, d)
+        self.coarse_chol = None
+        if (self.coarse == "direct" and len(self.levels) > 1
+                and 0 < self.levels[-1]["free"].sum() <= 1500):
+            self.coarse_chol = coarse_direct_factor(self.dense_operator(len(self.levels) - 1),
+                                                    self.levels[-1]["free"])
+
+    def dense_operator(self, li):
+        """The level's K (unmasked), assembled densely from its scaled E."""
+        lv = self.levels[li]
+        n = lv["ndof"]
+        K = np.zeros((n, n))
+        for e, dofs in enumerate(lv["edof"]):
+            K[np.ix_(dofs, dofs)] += lv["_scaled"][e] * self.KE
+        return K
+
+    def _coarse_solve(self, b):
+        """Exact solve on the coarsest level's free DOFs, 0 elsewhere."""
+        free, L = self.coarse_chol
+        x = np.zeros_like(b)
+        y = _forward(L, b[free])
+        x[free] = _forward(L.T[::-1, ::-1], y[::-1])[::-1]
+        return x
 
     def _Ku(self, li, u):
         lv = self.levels[li]
@@ -177,6 +239,8 @@ class MGSolverRef:
     def _vcycle(self, li, b):
         """core/multigrid.py:426-436."""
         if li == len(self.levels) - 1:
+            if self.coarse_chol is not None and li > 0:
+                return self._coarse_solve(b)
             return self._smooth(li, np.zeros_like(b), b, 30)
         u = self._smooth(li, np.zeros_like(b), b, self.n_smooth)
         rc = self._restrict(li, b - self._apply(li, u))

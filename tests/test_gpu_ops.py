@@ -96,6 +96,49 @@ def test_vcycle(dims, kw):
     g.close()
 
 
+@pytest.mark.parametrize("dims,kw,direct", [
+    ((16, 8, 8), {}, True), ((8, 8, 8), {"max_levels": 3}, True),
+    ((6, 10, 4), {"n_smooth": 3}, True), ((24, 16, 16), {"max_levels": 3}, True),
+    ((40, 34, 20), {"max_levels": 2}, False),    # 12474 coarse DOFs: sweeps kept
+])
+def test_vcycle_direct_coarse(dims, kw, direct):
+    """coarse="direct": the coarsest level solved exactly (FP64 inverse)."""
+    print("case vcycle direct", dims, kw, flush=True)
+    from core.multigrid import MGSolver
+    nx, ny, nz = dims
+    E, _, ndof, rng = _problem(*dims, seed=9)
+    face = np.arange(3 * (nx + 1) * (ny + 1))
+    fixed = np.unique(np.concatenate([face, rng.choice(ndof, 25, replace=False)]))
+    g = MGSolver(nx, ny, nz, fixed, nu=0.3, compute_mode="GPU", coarse="direct", **kw)
+    g.set_density(E)
+    o = MGSolverRef(nx, ny, nz, fixed, nu=0.3, coarse="direct", **kw)
+    o.set_density(E)
+    assert (o.coarse_chol is not None) == direct
+    b = np.where(o.levels[0]["free"], rng.standard_normal(ndof), 0.0)
+    z = o._vcycle(0, b)
+    assert _rel(g._vcycle(0, b), z) < 1e-10
+    assert _rel(g._vcycle(0, b, precision="mixed"), z) < 1e-4
+    g.close()
+
+
+def test_vcycle_direct_coarse_singular_keeps_sweeps():
+    """Supports only on nodes that are not coarse nodes: the coarse free block
+    is singular, the library keeps the 30 sweeps."""
+    print("case vcycle direct singular", flush=True)
+    from core.multigrid import MGSolver
+    dims = (6, 6, 6)
+    E, _, ndof, rng = _problem(*dims, seed=3)
+    node = 1 + 7 * (1 + 7 * 1)
+    fixed = np.array([3 * node, 3 * node + 1, 3 * node + 2])
+    g = MGSolver(*dims, fixed, nu=0.3, compute_mode="GPU", coarse="direct", max_levels=2)
+    g.set_density(E)
+    o = MGSolverRef(*dims, fixed, nu=0.3, max_levels=2)
+    o.set_density(E)
+    b = np.where(o.levels[0]["free"], rng.standard_normal(ndof), 0.0)
+    assert _rel(g._vcycle(0, b), o._vcycle(0, b)) < 1e-11
+    g.close()
+
+
 @pytest.mark.parametrize("k", [1, 2, 3, 4])
 @pytest.mark.parametrize("start", ["zero", "u"])
 def test_chebyshev_smoother_closed_form(k, start):

@@ -273,6 +273,35 @@ def test_autotuned_solver_same_solution():
     assert _rel(out[True], out[False]) < 1e-9
 
 
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_direct_coarse_same_solution(precision):
+    """coarse="direct" changes only the preconditioner: both solves reach the
+    same u (iteration counts are reported, not asserted: on this small grid
+    the 30 coarsest sweeps are already close to the exact solve)."""
+    import torch
+    from core import workloads as W
+    from core.multigrid import MGSolver
+    print("direct coarse: two_material n=32", precision, flush=True)
+    p = W.two_material(n=32)
+    E = torch.tensor(p.E_stiffness, device="cuda")
+    A = torch.tensor(p.active.astype(np.uint8), device="cuda")
+    f = torch.tensor(p.force, device="cuda")
+    out, its = {}, {}
+    for coarse in ("sweeps", "direct"):
+        mg = MGSolver(p.nx, p.ny, p.nz, p.fixed_dofs, nu=p.nu, compute_mode="GPU",
+                      precision=precision, smoother="chebyshev", cheb_ratio=0.1,
+                      system="active", max_levels=4, coarse=coarse)
+        mg.set_density(E, active=A)
+        u = torch.empty_like(f)
+        mg.solve_device(f, u, tol=1e-10)
+        assert mg.last_converged
+        assert mg.last_coarse_direct == (coarse == "direct")
+        out[coarse], its[coarse] = u.cpu().numpy(), mg.last_iters
+        mg.close()
+    print("iterations", its, flush=True)
+    assert _rel(out["direct"], out["sweeps"]) < 1e-7
+
+
 def test_final_payload_keys_and_round_trip(tmp_path):
     """The worker's final.pkl dict (core/solve_worker.py:411-431): every key,
     the dtypes the UI reads, and the atomic write."""

@@ -125,6 +125,53 @@ def test_vcycle_equals_dense_operator_product():
     assert np.allclose(mg._vcycle(0, b), z, rtol=1e-12, atol=1e-14 * np.abs(z).max())
 
 
+def test_direct_coarse_vcycle():
+    """coarse="direct": the dense operator is the sparse assembly, the coarse
+    solve satisfies K x = b on the free DOFs, and the two-grid cycle is the
+    dense product with K_ff^-1 (solved by scipy) in place of the 30 sweeps."""
+    dims = (8, 4, 4)
+    rng, n, E, _ = _prob(*dims, seed=4)
+    fixed = np.concatenate([np.arange(3 * 9 * 5), rng.choice(n, 20, replace=False)])
+    mg = MGSolverRef(*dims, fixed, nu=0.3, max_levels=2, coarse="direct")
+    mg.set_density(E)
+    L = mg.levels
+    assert mg.coarse_chol is not None
+    Kc = _assemble(*L[1]["dims"], L[1]["_scaled"], 0.3).toarray()
+    assert np.allclose(mg.dense_operator(1), Kc, rtol=1e-14, atol=1e-14 * np.abs(Kc).max())
+    fc = L[1]["free"]
+    bc = np.where(fc, rng.standard_normal(L[1]["ndof"]), 0.0)
+    xc = mg._coarse_solve(bc)
+    assert np.all(xc[~fc] == 0.0)
+    assert np.abs(mg._apply(1, xc) - bc).max() < 1e-10 * np.abs(bc).max()
+    ref = np.zeros_like(bc)
+    ref[fc] = np.linalg.solve(Kc[np.ix_(fc, fc)], bc[fc])
+    assert np.abs(xc - ref).max() < 1e-10 * np.abs(ref).max()
+    # two-grid cycle
+    ms = MGSolverRef(*dims, fixed, nu=0.3, max_levels=2)
+    ms.set_density(E)
+    b = np.where(L[0]["free"], rng.standard_normal(n), 0.0)
+    v = ms._smooth(0, np.zeros(n), b, 2)
+    ec = np.zeros(L[1]["ndof"])
+    rc = ms._restrict(0, b - ms._apply(0, v))
+    ec[fc] = np.linalg.solve(Kc[np.ix_(fc, fc)], rc[fc])
+    z = ms._smooth(0, v + ms._prolongate(0, ec), b, 2)
+    assert np.abs(mg._vcycle(0, b) - z).max() < 1e-10 * np.abs(z).max()
+    assert np.abs(ms._vcycle(0, b) - z).max() > 1e-6 * np.abs(z).max()
+
+
+def test_direct_coarse_falls_back_when_singular():
+    """No fixed DOF survives injection (all supports on odd nodes): the free
+    block is singular and the coarsest level keeps its sweeps."""
+    dims = (4, 4, 4)
+    n = 3 * 5 * 5 * 5
+    node = 1 + 5 * (1 + 5 * 1)                       # (1, 1, 1): not a coarse node
+    fixed = np.array([3 * node, 3 * node + 1, 3 * node + 2])
+    mg = MGSolverRef(*dims, fixed, nu=0.3, max_levels=2, coarse="direct")
+    mg.set_density(np.ones(64))
+    assert mg.coarse_chol is None
+    assert mg.levels[1]["free"].all() and n == mg.levels[0]["ndof"]
+
+
 @pytest.mark.parametrize("dims", [(6, 4, 4), (8, 8, 4)])
 def test_solve_equals_direct_solve(dims):
     rng, n, E, _ = _prob(*dims, seed=3)

@@ -66,7 +66,7 @@ class vx_info(ctypes.Structure):
         ("true_resid_ratio", ctypes.c_double),
         ("solve_ms", ctypes.c_double),
         ("corrector", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("coarse_direct", ctypes.c_int32),
     ]
 
 

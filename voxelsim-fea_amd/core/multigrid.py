@@ -68,7 +68,7 @@ class MGSolver:
                  verbose=False, precision="fp64", coarse_sweeps=30, inner_tol=1e-5,
                  max_outer=20, z_chunk=0, smoother="jacobi", cheb_ratio=0.3,
                  system="full", e_coarsen="mean", max_batch=1, flexible=False, pad=False,
-                 autotune=False):
+                 autotune=False, coarse="sweeps"):
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
         self.nelem = self.nx * self.ny * self.nz
         self.ndof = 3 * (self.nx + 1) * (self.ny + 1) * (self.nz + 1)
@@ -96,7 +96,9 @@ class MGSolver:
             inner_tol=inner_tol, max_outer=max_outer, z_chunk=z_chunk,
             smoother=_SMOOTHER[smoother], cheb_ratio=cheb_ratio,
             e_coarsen={"mean": 0, "geometric": 1, "harmonic": 2}[e_coarsen],
-            max_batch=max_batch, flexible=1 if flexible else 0)
+            max_batch=max_batch, flexible=1 if flexible else 0,
+            coarse_mode={"sweeps": _native.VX_COARSE_SWEEPS,
+                         "direct": _native.VX_COARSE_DIRECT}[coarse])
         self.max_batch = max_batch
         h = ctypes.c_void_p()
         px, py, pz = self._pdims
@@ -246,6 +248,7 @@ class MGSolver:
         self.last_outer_iters = info.outer_iters
         self.last_solve_ms = info.solve_ms
         self.last_corrector = {0: "none", 1: "used", 2: "fell back"}[info.corrector]
+        self.last_coarse_direct = bool(info.coarse_direct)
         self.resid_history = [float(v) for v in hist[:min(info.iters, hist_cap)]]
         self._check(rc, allow=(_native.VX_ERR_NOT_CONVERGED,))
         return rc

@@ -110,6 +110,14 @@ struct vx_solver {
   float* corr_rms = nullptr;  // [max_batch]
   unsigned int* corr_emax = nullptr;
   long long corr_cap_nn = 0;  // nodes the corrector buffers were sized for
+  // coarsest-level direct solve (coarse_mode VX_COARSE_DIRECT): the inverse
+  // of the coarsest operator on its free DOFs, FP64, applied in the V-cycle
+  bool cd_ok = false;
+  int cd_nf = 0;
+  double* cd_inv = nullptr;   // [nf][nf], symmetric
+  int* cd_pos = nullptr;      // per coarsest DOF: its row in the free list, or -1
+  int* cd_idx = nullptr;      // free DOFs of the coarsest level
+  long long cd_cap_inv = 0, cd_cap_dof = 0;
   // capture of corrector-level residuals during solves (vx_capture)
   float* cap_buf = nullptr;
   int cap_level = -1, cap_cap = 0, cap_count = 0, cap_every = 1;
@@ -431,6 +439,40 @@ int coarse_fused(vx_solver* s, Level& L, const T* b, T* x) {
   return VX_OK;
 }
 
+// x = K_ff^-1 b on the free DOFs of the coarsest level (0 elsewhere), FP64
+// accumulation; the free part of b is staged in LDS.  blockIdx.y = rhs.
+template <typename T>
+__global__ __launch_bounds__(256) void k_coarse_direct(long long ndof, int nf,
+                                                       const int* __restrict__ pos,
+                                                       const int* __restrict__ idx,
+                                                       const double* __restrict__ inv,
+                                                       const T* __restrict__ b, T* __restrict__ x) {
+  extern __shared__ double sb[];
+  b += (long long)blockIdx.y * ndof;
+  x += (long long)blockIdx.y * ndof;
+  for (int j = threadIdx.x; j < nf; j += blockDim.x) sb[j] = (double)b[idx[j]];
+  __syncthreads();
+  const long long d = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (d >= ndof) return;
+  const int p = pos[d];
+  double acc = 0.0;
+  if (p >= 0)
+    for (int j = 0; j < nf; ++j) acc += inv[(long long)j * nf + p] * sb[j];
+  x[d] = (T)acc;
+}
+
+// the coarsest level's correction by the direct solve
+template <typename T>
+int coarse_direct(vx_solver* s, const T* b, T* x) {
+  Level& C = s->lv.back();
+  const long long ndof = 3 * C.Nn;
+  hipLaunchKernelGGL(k_coarse_direct<T>, dim3((unsigned)((ndof + 255) / 256), s->nb), dim3(256),
+                     sizeof(double) * s->cd_nf, s->stream, ndof, s->cd_nf, s->cd_pos, s->cd_idx,
+                     s->cd_inv, b, x);
+  LAUNCHCHK();
+  return VX_OK;
+}
+
 template <typename T>
 int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, T* w0, bool dot, int* n_part, int l0);
 
@@ -444,6 +486,7 @@ int plain_coarse(vx_solver* s, int li, const T* t, T* v, T* tmp, T* tmp2) {
   const int nb = s->nb;
   if (li == nl - 1) {
     const int k = s->opt.coarse_sweeps;
+    if (s->cd_ok) return coarse_direct<T>(s, t, v);
     if (li > 0 && L.Nn <= coarse_max_nodes<T>() && L.Ne <= coarse_max_elems<T>() &&
         k <= COARSE_MAX_STEPS)
       return coarse_fused<T>(s, L, t, v);
@@ -556,7 +599,9 @@ int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, T* w0, bool dot, int* n_part
     }
     if (li == nl - 1) {
       const int k = s->opt.coarse_sweeps;
-      if (li > 0 && L.Nn <= coarse_max_nodes<T>() && L.Ne <= coarse_max_elems<T>() &&
+      if (s->cd_ok && li > 0 && !(dot && li == l0)) {
+        CHK(coarse_direct<T>(s, b, x));
+      } else if (li > 0 && L.Nn <= coarse_max_nodes<T>() && L.Ne <= coarse_max_elems<T>() &&
           k <= COARSE_MAX_STEPS) {
         CHK(coarse_fused<T>(s, L, b, x));
       } else {
@@ -806,6 +851,114 @@ int corr_features(vx_solver* s, int level) {
 
 static int alloc_pcg64(vx_solver* s);
 
+// Direct solve of the coarsest level: assemble its operator densely on the
+// host from the level's E (sum of E_e KE over elements, the same operator
+// the tile kernel applies), keep the free DOFs, factor (Cholesky, FP64) and
+// invert.  Leaves cd_ok false (the V-cycle then smooths as configured) when
+// the level has more than CD_MAX_FREE free DOFs or the free block is not
+// positive definite (a support that injection lost on the coarse grid).
+constexpr int CD_MAX_FREE = 1500;
+static int build_coarse_direct(vx_solver* s) {
+  s->cd_ok = false;
+  Level& C = s->lv.back();
+  if (s->lv.size() < 2) return VX_OK;
+  const long long ndof = 3 * C.Nn;
+  std::vector<double> E(C.Ne);
+  std::vector<uint8_t> m(C.Nn);
+  HIPCHK(hipMemcpyAsync(E.data(), C.E64, sizeof(double) * C.Ne, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipMemcpyAsync(m.data(), C.mask, C.Nn, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  std::vector<int> pos(ndof, -1), idx;
+  for (long long d = 0; d < ndof; ++d)
+    if ((m[d / 3] >> (d % 3)) & 1u) {
+      pos[d] = (int)idx.size();
+      idx.push_back((int)d);
+    }
+  const int nf = (int)idx.size();
+  if (nf == 0 || nf > CD_MAX_FREE) return VX_OK;
+  // unit-E element matrix in binary corner order
+  double KE[24][24];
+  for (int j = 0; j < 24; ++j) {
+    double u[8][3] = {}, y[8][3];
+    u[j / 3][j % 3] = 1.0;
+    elem_apply<double>(u, 1.0, s->K64, y);
+    for (int i = 0; i < 24; ++i) KE[i][j] = y[i / 3][i % 3];
+  }
+  std::vector<double> A((size_t)nf * nf, 0.0);
+  for (int ez = 0; ez < C.nz; ++ez)
+    for (int ey = 0; ey < C.ny; ++ey)
+      for (int ex = 0; ex < C.nx; ++ex) {
+        const double Ee = E[((long long)ez * C.ny + ey) * C.nx + ex];
+        if (Ee == 0.0) continue;
+        int row[24];
+        for (int q = 0; q < 8; ++q) {
+          const long long n = (ex + (q & 1)) + (long long)C.Lx * ((ey + ((q >> 1) & 1)) +
+                                                                 (long long)C.Ly * (ez + (q >> 2)));
+          for (int c = 0; c < 3; ++c) row[3 * q + c] = pos[3 * n + c];
+        }
+        for (int i = 0; i < 24; ++i) {
+          if (row[i] < 0) continue;
+          for (int j = 0; j < 24; ++j)
+            if (row[j] >= 0) A[(size_t)row[i] * nf + row[j]] += Ee * KE[i][j];
+        }
+      }
+  // Cholesky A = L L^T (lower triangle in place)
+  double dmax = 0.0;
+  for (int i = 0; i < nf; ++i) dmax = std::max(dmax, A[(size_t)i * nf + i]);
+  for (int j = 0; j < nf; ++j) {
+    double d = A[(size_t)j * nf + j];
+    for (int k = 0; k < j; ++k) d -= A[(size_t)j * nf + k] * A[(size_t)j * nf + k];
+    if (!(d > 1e-13 * dmax)) return VX_OK;   // not positive definite
+    const double ljj = std::sqrt(d);
+    A[(size_t)j * nf + j] = ljj;
+    for (int i = j + 1; i < nf; ++i) {
+      double v = A[(size_t)i * nf + j];
+      const double* ri = &A[(size_t)i * nf];
+      const double* rj = &A[(size_t)j * nf];
+      for (int k = 0; k < j; ++k) v -= ri[k] * rj[k];
+      A[(size_t)i * nf + j] = v / ljj;
+    }
+  }
+  // inverse: solve L L^T X = I column by column (X symmetric)
+  std::vector<double> inv((size_t)nf * nf), col(nf);
+  for (int c = 0; c < nf; ++c) {
+    for (int i = 0; i < nf; ++i) {
+      double v = (i == c) ? 1.0 : 0.0;
+      const double* ri = &A[(size_t)i * nf];
+      for (int k = c; k < i; ++k) v -= ri[k] * col[k];
+      col[i] = (i < c) ? 0.0 : v / ri[i];
+    }
+    for (int i = nf - 1; i >= 0; --i) {
+      double v = col[i];
+      for (int k = i + 1; k < nf; ++k) v -= A[(size_t)k * nf + i] * col[k];
+      col[i] = v / A[(size_t)i * nf + i];
+    }
+    for (int i = 0; i < nf; ++i) inv[(size_t)c * nf + i] = col[i];
+  }
+  if ((long long)nf * nf > s->cd_cap_inv) {
+    s->cd_inv = nullptr;   // smaller buffers stay owned by s->allocs
+    CHK(dalloc(s, &s->cd_inv, (size_t)nf * nf));
+    s->cd_cap_inv = (long long)nf * nf;
+  }
+  if (ndof > s->cd_cap_dof) {
+    s->cd_pos = nullptr;
+    s->cd_idx = nullptr;
+    CHK(dalloc(s, &s->cd_pos, (size_t)ndof));
+    CHK(dalloc(s, &s->cd_idx, (size_t)ndof));
+    s->cd_cap_dof = ndof;
+  }
+  HIPCHK(hipMemcpyAsync(s->cd_inv, inv.data(), sizeof(double) * nf * nf, hipMemcpyHostToDevice,
+                        s->stream));
+  HIPCHK(hipMemcpyAsync(s->cd_pos, pos.data(), sizeof(int) * ndof, hipMemcpyHostToDevice,
+                        s->stream));
+  HIPCHK(hipMemcpyAsync(s->cd_idx, idx.data(), sizeof(int) * nf, hipMemcpyHostToDevice,
+                        s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  s->cd_nf = nf;
+  s->cd_ok = true;
+  return VX_OK;
+}
+
 // PCG on level l's own system (V-cycle from l down): x = A_l^-1 b to tol.
 static int level_solve(vx_solver* s, int l, const double* b, double* x, double tol, int max_iter,
                        PcgResult* out) {
@@ -899,7 +1052,8 @@ int vx_create(int32_t nx, int32_t ny, int32_t nz, double nu, const vx_options* o
     vx_default_options(&s->opt);
   }
   if (s->opt.max_batch < 1) s->opt.max_batch = 1;
-  if (s->opt.n_smooth < 1 || s->opt.coarse_sweeps < 1 || s->opt.max_levels < 1) {
+  if (s->opt.n_smooth < 1 || s->opt.coarse_sweeps < 1 || s->opt.max_levels < 1 ||
+      (s->opt.coarse_mode != VX_COARSE_SWEEPS && s->opt.coarse_mode != VX_COARSE_DIRECT)) {
     delete s;
     return VX_ERR_ARG;
   }
@@ -1002,6 +1156,7 @@ int vx_set_fixed(vx_handle s, const int64_t* fixed, int64_t n_fixed) {
   for (auto& L : s->lv)
     HIPCHK(hipMemcpyAsync(L.mask, L.mask_fixed, L.Nn, hipMemcpyDeviceToDevice, s->stream));
   s->density_set = false;   // the effective masks are rebuilt by set_density
+  s->cd_ok = false;
   HIPCHK(hipStreamSynchronize(s->stream));
   s->fixed_set = true;
   return VX_OK;
@@ -1041,6 +1196,8 @@ int vx_set_density(vx_handle s, const double* E, const uint8_t* active, double e
     LAUNCHCHK();
   }
   s->density_set = true;
+  s->cd_ok = false;
+  if (s->opt.coarse_mode == VX_COARSE_DIRECT) CHK(build_coarse_direct(s));
   if (s->corr) return corr_features(s, s->corr_level);
   return VX_OK;
 }
@@ -1267,7 +1424,10 @@ static int solve_impl(vx_solver* s, int nb, const double* F, double* U, const do
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  for (int q = 0; q < nb; ++q) inf[q].corrector = corr_state;
+  for (int q = 0; q < nb; ++q) {
+    inf[q].corrector = corr_state;
+    inf[q].coarse_direct = s->cd_ok ? 1 : 0;
+  }
   s->corr_on = false;
   if (infos)
     for (int q = 0; q < nb; ++q) infos[q] = inf[q];
