@@ -55,9 +55,6 @@ struct TileArgs {
   int nb;                // right-hand sides in the launch (vectors [nb][vstride])
   long long vstride;     // elements between consecutive right-hand sides
   const T* __restrict__ u;
-  const T* __restrict__ pold;   // PUPD: previous direction; the kernel applies
-  T* __restrict__ pnew;         //   K to p = u + beta * pold and stores p
-  const Scalars* __restrict__ scal;   // PUPD: beta of each right-hand side
   const T* __restrict__ E;
   const uint8_t* __restrict__ mask;
   const T* __restrict__ b;
@@ -132,11 +129,12 @@ __device__ inline double block_sum(double v, double* sred) {
 //                 the Chebyshev iteration on D^-1 A (three-term recurrence)
 // DOT   adds the work-group partial of  sum out*u (MATVEC) or sum out*b
 //       (the smoothers) to partial[block]; RESID: sum out*out.
-// PUPD  (M_MATVEC) the input is p = u + beta*pold, formed while staging,
-//       and stored to pnew by the owners.
 // ---------------------------------------------------------------------------
-template <typename T, int MODE, bool DOT, bool PUPD>
-__global__ __launch_bounds__(NT) void k_tile(TileArgs<T> a) {
+// FP32: at least 5 waves per SIMD (the register budget that allows it costs
+// no spills); FP64 stays at what its registers give (forcing 4 spills)
+template <typename T, int MODE, bool DOT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 1)))
+void k_tile(TileArgs<T> a) {
   __shared__ T sP[2][3 * FOOT * FOOT];
   __shared__ T sQ[2][4][TILE][TILE + 1];
   __shared__ double sred[NT / 64];
@@ -151,8 +149,6 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs<T> a) {
   const int vt = v - rhs * G;
   const long long voff = (long long)rhs * a.vstride;
   const T* __restrict__ au = a.u + voff;
-  const T* __restrict__ apold = PUPD ? a.pold + voff : nullptr;
-  T* __restrict__ apnew = PUPD ? a.pnew + voff : nullptr;
   const T* __restrict__ ab = a.b ? a.b + voff : nullptr;
   const T* __restrict__ aup = a.up ? a.up + voff : nullptr;
   T* __restrict__ aout = a.out + voff;
@@ -193,7 +189,6 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs<T> a) {
       if (x >= 0 && x < a.Lx && y >= 0 && y < a.Ly) goff[k] = (y * a.Lx + x) * 3 + c;
     }
   }
-  const T beta = PUPD ? (T)a.scal[rhs].beta : T(0);
 
   T pre[PRE];
   auto fetch = [&](int pz) {
@@ -202,10 +197,7 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs<T> a) {
 #pragma unroll
     for (int k = 0; k < PRE; ++k) {
       T val = T(0);
-      if (zin && goff[k] >= 0) {
-        val = au[base + goff[k]];
-        if (PUPD) val += beta * apold[base + goff[k]];
-      }
+      if (zin && goff[k] >= 0) val = au[base + goff[k]];
       pre[k] = val;
     }
   };
@@ -323,10 +315,6 @@ __global__ __launch_bounds__(NT) void k_tile(TileArgs<T> a) {
       T y[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) y[c] = A0[c] + sQ[q][c][ty - 1][tx];
-      if (PUPD) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) apnew[3 * n + c] = ub[c];
-      }
       if (MODE == M_MATVEC) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -545,148 +533,243 @@ __global__ void k_zero_step(long long Nn, const uint8_t* __restrict__ mask,
   }
 }
 
-// R = P^T (27-point full weighting, unscaled) followed by the coarse mask
-// (core/multigrid.py:404-416).  3D launch: one thread per coarse node of an
-// x-row, block rows (y, z) from the grid.
+// Grid transfers marching in z.  One thread per column of the 2-D plane
+// grid, z split into chunks so that about TR_TARGET_GROUPS work-groups run;
+// the 1-D launch is numbered XCD-aware (blocks are dealt to the 8 XCDs
+// round-robin; each XCD receives a contiguous run of columns, whose shared
+// rows then stay in its L2).  Work-groups: groups_xy * chunks * nb, the grid
+// rounded up to a multiple of 8 (the extra blocks exit at once).
+constexpr int TR_THREADS = 256;
+constexpr int TR_TARGET_GROUPS = 2048;
+
+struct TrGrid {
+  int gxy, nchunk, per_chunk;   // per_chunk: planes (coarse / fine) per chunk
+};
+
+__device__ __forceinline__ bool tr_block(int total, int& v) {
+  const int per = (total + 7) >> 3;
+  v = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+  return v < total;
+}
+
+// R = P^T (27-point full weighting, unscaled) then the coarse mask
+// (core/multigrid.py:404-416), z-marching: the
+// thread of coarse column (jx, jy) reads each fine plane once.  Each coarse
+// value is the tap-by-tap sum over (z, y, x) in that order, the odd plane
+// below carried from the previous coarse plane as its own partial sum (the
+// weights are powers of 2, so that partial is exact to reuse).
 template <typename T>
-__global__ void k_restrict3(int Lcx, int Lcy, int Lcz, const T* __restrict__ rf,
-                            const uint8_t* __restrict__ cmask, T* __restrict__ rc) {
-  const int jx = blockIdx.x * blockDim.x + threadIdx.x;
-  const int rhs = blockIdx.z / Lcz;
-  const int jy = blockIdx.y, jz = blockIdx.z - rhs * Lcz;
-  if (jx >= Lcx) return;
+__global__ __launch_bounds__(TR_THREADS) void k_restrict_z(int Lcx, int Lcy, int Lcz, TrGrid g,
+                                                           int nb, const T* __restrict__ rf,
+                                                           const uint8_t* __restrict__ cmask,
+                                                           T* __restrict__ rc) {
+  int v;
+  if (!tr_block(g.gxy * g.nchunk * nb, v)) return;
+  const int bxy = v % g.gxy;
+  const int rest = v / g.gxy;
+  const int chunk = rest % g.nchunk;
+  const int rhs = rest / g.nchunk;
+  const int t = bxy * TR_THREADS + (int)threadIdx.x;
+  const int jx = t % Lcx, jy = t / Lcx;
+  if (jy >= Lcy) return;
   const int Lfx = 2 * Lcx - 1, Lfy = 2 * Lcy - 1, Lfz = 2 * Lcz - 1;
-  rf += (long long)rhs * 3 * Lfx * Lfy * Lfz;
-  rc += (long long)rhs * 3 * Lcx * Lcy * Lcz;
-  T s0 = T(0), s1 = T(0), s2 = T(0);
+  const long long fplane = 3LL * Lfx * Lfy;
+  rf += (long long)rhs * fplane * Lfz;
+  rc += (long long)rhs * 3LL * Lcx * Lcy * Lcz;
+  const int jz0 = chunk * g.per_chunk;
+  const int jz1 = min(jz0 + g.per_chunk, Lcz);
+  // x-y taps of this column (y outer, x inner): fine offsets and weights,
+  // weight 0 for taps outside the grid (skipped)
+  int off[9];
+  T w[9];
 #pragma unroll
-  for (int dz = -1; dz <= 1; ++dz) {
-    const int z = 2 * jz + dz;
-    if (z < 0 || z >= Lfz) continue;
-    const T wz = dz == 0 ? T(1) : T(0.5);
+  for (int dy = -1; dy <= 1; ++dy)
 #pragma unroll
-    for (int dy = -1; dy <= 1; ++dy) {
-      const int y = 2 * jy + dy;
-      if (y < 0 || y >= Lfy) continue;
-      const T wzy = wz * (dy == 0 ? T(1) : T(0.5));
-      const T* row = rf + ((long long)z * Lfy + y) * Lfx * 3;
+    for (int dx = -1; dx <= 1; ++dx) {
+      const int k = (dy + 1) * 3 + (dx + 1);
+      const int x = 2 * jx + dx, y = 2 * jy + dy;
+      const bool in = x >= 0 && x < Lfx && y >= 0 && y < Lfy;
+      off[k] = in ? 3 * (y * Lfx + x) : 0;
+      w[k] = in ? (dx == 0 ? T(1) : T(0.5)) * (dy == 0 ? T(1) : T(0.5)) : T(0);
+    }
+  // acc += sum over the taps of plane z, weight scale * w
+  auto add_plane = [&](int z, T sc, T acc[3]) {
+    const T* P = rf + (long long)z * fplane;
 #pragma unroll
-      for (int dx = -1; dx <= 1; ++dx) {
-        const int x = 2 * jx + dx;
-        if (x < 0 || x >= Lfx) continue;
-        const T w = wzy * (dx == 0 ? T(1) : T(0.5));
-        s0 += w * row[3 * x];
-        s1 += w * row[3 * x + 1];
-        s2 += w * row[3 * x + 2];
+    for (int k = 0; k < 9; ++k) {
+      if (w[k] == T(0)) continue;
+      const T wk = sc * w[k];
+      const T* q = P + off[k];
+      acc[0] += wk * q[0];
+      acc[1] += wk * q[1];
+      acc[2] += wk * q[2];
+    }
+  };
+  T h[3] = {T(0), T(0), T(0)};   // partial sum of plane 2 jz - 1 (weight 1/2)
+  if (jz0 > 0) add_plane(2 * jz0 - 1, T(0.5), h);
+  for (int jz = jz0; jz < jz1; ++jz) {
+    T a[3] = {h[0], h[1], h[2]};
+    add_plane(2 * jz, T(1), a);
+    if (2 * jz + 1 < Lfz) {
+      const T* P = rf + (long long)(2 * jz + 1) * fplane;
+      h[0] = h[1] = h[2] = T(0);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) {
+        if (w[k] == T(0)) continue;
+        const T wk = T(0.5) * w[k];
+        const T* q = P + off[k];
+        const T q0 = q[0], q1 = q[1], q2 = q[2];
+        a[0] += wk * q0;
+        a[1] += wk * q1;
+        a[2] += wk * q2;
+        h[0] += wk * q0;
+        h[1] += wk * q1;
+        h[2] += wk * q2;
       }
     }
+    const long long J = ((long long)jz * Lcy + jy) * Lcx + jx;
+    const unsigned mk = cmask[J];
+    rc[3 * J] = (mk & 1u) ? a[0] : T(0);
+    rc[3 * J + 1] = (mk & 2u) ? a[1] : T(0);
+    rc[3 * J + 2] = (mk & 4u) ? a[2] : T(0);
   }
-  const long long J = ((long long)jz * Lcy + jy) * Lcx + jx;
-  const unsigned mk = cmask[J];
-  rc[3 * J] = (mk & 1u) ? s0 : T(0);
-  rc[3 * J + 1] = (mk & 2u) ? s1 : T(0);
-  rc[3 * J + 2] = (mk & 4u) ? s2 : T(0);
 }
 
-// x += mask * (P e) with trilinear P (core/multigrid.py:418-424, 434).
-// 3D launch: one thread per fine node of an x-row.
+// x += fine mask * (P e) (core/multigrid.py:418-424, 434), z-marching: the
+// thread of coarse column cx and fine row iy updates fine nodes 2cx and
+// 2cx+1 of that row in every fine plane of its chunk.  Each interpolant is
+// the sum of its coarse values in (z, y, x) order times the product of the
+// 1/2 weights (a power of 2, so applying it last is exact); the sum over a
+// coarse plane is formed once and reused by the fine planes it feeds.
 template <typename T>
-__global__ void k_prolong_add3(int Lfx, int Lfy, int Lfz, const T* __restrict__ ec,
-                               const uint8_t* __restrict__ fmask, T* __restrict__ x) {
-  const int ix = blockIdx.x * blockDim.x + threadIdx.x;
-  const int rhs = blockIdx.z / Lfz;
-  const int iy = blockIdx.y, iz = blockIdx.z - rhs * Lfz;
-  if (ix >= Lfx) return;
-  const int Lcx = (Lfx + 1) / 2, Lcy = (Lfy + 1) / 2, Lcz = (Lfz + 1) / 2;
-  ec += (long long)rhs * 3 * Lcx * Lcy * Lcz;
-  x += (long long)rhs * 3 * Lfx * Lfy * Lfz;
-  const int cx = ix >> 1, cy = iy >> 1, cz = iz >> 1;
-  const int ox = ix & 1, oy = iy & 1, oz = iz & 1;
-  const T wx = ox ? T(0.5) : T(1), wy = oy ? T(0.5) : T(1), wz = oz ? T(0.5) : T(1);
-  T s0 = T(0), s1 = T(0), s2 = T(0);
-  for (int kz = 0; kz <= oz; ++kz)
-    for (int ky = 0; ky <= oy; ++ky) {
-      const T* row = ec + ((long long)(cz + kz) * Lcy + (cy + ky)) * Lcx * 3;
-      for (int kx = 0; kx <= ox; ++kx) {
-        const T w = wx * wy * wz;
-        const T* q = row + 3 * (cx + kx);
-        s0 += w * q[0];
-        s1 += w * q[1];
-        s2 += w * q[2];
-      }
-    }
-  const long long i = ((long long)iz * Lfy + iy) * Lfx + ix;
-  const unsigned mk = fmask[i];
-  if (mk & 1u) x[3 * i] += s0;
-  if (mk & 2u) x[3 * i + 1] += s1;
-  if (mk & 4u) x[3 * i + 2] += s2;
-}
-
-// (P e)[3i+c] at fine node i = (ix, iy, iz) of a fine grid (Lfx, Lfy, Lfz)
-template <typename T>
-__device__ __forceinline__ void prolong_at(int ix, int iy, int iz, int Lcx, int Lcy,
-                                           const T* __restrict__ ec, T& s0, T& s1, T& s2) {
-  const int cx = ix >> 1, cy = iy >> 1, cz = iz >> 1;
-  const int ox = ix & 1, oy = iy & 1, oz = iz & 1;
-  const T w = (ox ? T(0.5) : T(1)) * (oy ? T(0.5) : T(1)) * (oz ? T(0.5) : T(1));
-  s0 = s1 = s2 = T(0);
-  for (int kz = 0; kz <= oz; ++kz)
-    for (int ky = 0; ky <= oy; ++ky) {
-      const T* row = ec + ((long long)(cz + kz) * Lcy + (cy + ky)) * Lcx * 3;
-      for (int kx = 0; kx <= ox; ++kx) {
-        const T* q = row + 3 * (cx + kx);
-        s0 += w * q[0];
-        s1 += w * q[1];
-        s2 += w * q[2];
-      }
-    }
-}
-
-// k_prolong_add3 over the flat node order, four nodes per thread through
-// float4 / uint32 accesses of x and the mask (x 16-byte aligned per right-
-// hand side; blockIdx.y = rhs); the < 4 tail nodes one by one.
-__global__ void k_prolong_add_v4(int Lfx, int Lfy, int Lfz, const float* __restrict__ ec,
-                                 const uint8_t* __restrict__ fmask, float* __restrict__ x) {
-  const int Lcx = (Lfx + 1) / 2, Lcy = (Lfy + 1) / 2, Lcz = (Lfz + 1) / 2;
-  const unsigned Nn = (unsigned)Lfx * Lfy * Lfz;
-  ec += (long long)blockIdx.y * 3 * Lcx * Lcy * Lcz;
-  x += (long long)blockIdx.y * 3 * (long long)Nn;
-  const unsigned ng = Nn >> 2;
-  const unsigned stride = gridDim.x * blockDim.x;
-  for (unsigned g = blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += stride) {
-    const unsigned mk4 = reinterpret_cast<const unsigned*>(fmask)[g];
-    float4* x4 = reinterpret_cast<float4*>(x) + 3 * (long long)g;
-    const float4 q0 = x4[0], q1 = x4[1], q2 = x4[2];
-    float v[12] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+__global__ __launch_bounds__(TR_THREADS) void k_prolong_z(int Lcx, int Lcy, int Lcz, TrGrid g,
+                                                          int nb, const T* __restrict__ ec,
+                                                          const uint8_t* __restrict__ fmask,
+                                                          T* __restrict__ x) {
+  int v;
+  if (!tr_block(g.gxy * g.nchunk * nb, v)) return;
+  const int bxy = v % g.gxy;
+  const int rest = v / g.gxy;
+  const int chunk = rest % g.nchunk;
+  const int rhs = rest / g.nchunk;
+  const int Lfx = 2 * Lcx - 1, Lfy = 2 * Lcy - 1, Lfz = 2 * Lcz - 1;
+  const int t = bxy * TR_THREADS + (int)threadIdx.x;
+  const int cx = t % Lcx, iy = t / Lcx;
+  if (iy >= Lfy) return;
+  const long long cplane = 3LL * Lcx * Lcy;
+  ec += (long long)rhs * cplane * Lcz;
+  x += (long long)rhs * 3LL * Lfx * Lfy * Lfz;
+  const int cy = iy >> 1;
+  const bool oy = iy & 1;
+  const bool two = cx + 1 < Lcx;   // fine node 2cx+1 exists
+  const int iz0 = chunk * g.per_chunk;
+  const int iz1 = min(iz0 + g.per_chunk, Lfz);
+  // coarse values of plane cz feeding this row: [ky][kx][c]
+  auto load = [&](int cz, T q[2][2][3]) {
+    const T* row = ec + (long long)cz * cplane + 3LL * cy * Lcx + 3 * cx;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const unsigned n = 4 * g + k;
-      const unsigned row = n / (unsigned)Lfx;
-      const int ix = (int)(n - row * (unsigned)Lfx);
-      const int iz = (int)(row / (unsigned)Lfy);
-      const int iy = (int)(row - (unsigned)iz * (unsigned)Lfy);
-      float s0, s1, s2;
-      prolong_at<float>(ix, iy, iz, Lcx, Lcy, ec, s0, s1, s2);
-      const unsigned mk = (mk4 >> (8 * k)) & 0xFFu;
-      if (mk & 1u) v[3 * k] += s0;
-      if (mk & 2u) v[3 * k + 1] += s1;
-      if (mk & 4u) v[3 * k + 2] += s2;
+    for (int ky = 0; ky < 2; ++ky)
+#pragma unroll
+      for (int kx = 0; kx < 2; ++kx)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          q[ky][kx][c] = ((ky == 0 || oy) && (kx == 0 || two)) ? row[ky * 3 * Lcx + kx * 3 + c]
+                                                                : T(0);
+  };
+  // s[0]: even node (x taps kx = 0), s[1]: odd node (kx = 0, 1), y taps
+  // ky = 0 (and 1 on odd rows); sums continue from s
+  auto accum = [&](const T q[2][2][3], T s[2][3]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      s[0][c] += q[0][0][c];
+      s[1][c] += q[0][0][c];
+      s[1][c] += q[0][1][c];
+      if (oy) {
+        s[0][c] += q[1][0][c];
+        s[1][c] += q[1][0][c];
+        s[1][c] += q[1][1][c];
+      }
     }
-    x4[0] = make_float4(v[0], v[1], v[2], v[3]);
-    x4[1] = make_float4(v[4], v[5], v[6], v[7]);
-    x4[2] = make_float4(v[8], v[9], v[10], v[11]);
+  };
+  const T wy = oy ? T(0.5) : T(1);
+  // fine planes in pairs (2cz, 2cz+1): the chunk starts on an even plane;
+  // both planes' loads are issued before either is updated
+  const long long fplane = (long long)Lfx * Lfy;
+  T lo[2][3];   // sums of coarse plane cz
+  {
+    T q[2][2][3];
+    load(iz0 >> 1, q);
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) lo[k][c] = T(0);
+    accum(q, lo);
   }
-  for (unsigned n = 4 * ng + blockIdx.x * blockDim.x + threadIdx.x; n < Nn; n += stride) {
-    const unsigned row = n / (unsigned)Lfx;
-    const int ix = (int)(n - row * (unsigned)Lfx);
-    const int iz = (int)(row / (unsigned)Lfy);
-    const int iy = (int)(row - (unsigned)iz * (unsigned)Lfy);
-    float s0, s1, s2;
-    prolong_at<float>(ix, iy, iz, Lcx, Lcy, ec, s0, s1, s2);
-    const unsigned mk = fmask[n];
-    if (mk & 1u) x[3 * n] += s0;
-    if (mk & 2u) x[3 * n + 1] += s1;
-    if (mk & 4u) x[3 * n + 2] += s2;
+  for (int iz = iz0; iz < iz1; iz += 2) {
+    const int cz = iz >> 1;
+    const bool odd = iz + 1 < iz1;
+    const long long nA = ((long long)iz * Lfy + iy) * Lfx + 2 * cx;
+    const long long nB = nA + fplane;
+    T* xa = x + 3 * nA;
+    T* xb = x + 3 * nB;
+    T va[6], vb[6], q[2][2][3];
+    unsigned ma0 = fmask[nA], ma1 = two ? fmask[nA + 1] : 0u, mb0 = 0u, mb1 = 0u;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      va[c] = xa[c];
+      va[3 + c] = two ? xa[3 + c] : T(0);
+    }
+    if (odd) {
+      mb0 = fmask[nB];
+      mb1 = two ? fmask[nB + 1] : 0u;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        vb[c] = xb[c];
+        vb[3 + c] = two ? xb[3 + c] : T(0);
+      }
+      load(cz + 1, q);
+    }
+    // even plane: plane cz's sums
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if ((ma0 >> c) & 1u) va[c] += lo[0][c] * wy;
+      if ((ma1 >> c) & 1u) va[3 + c] += lo[1][c] * (T(0.5) * wy);
+    }
+    // every value is stored (unchanged where masked): whole cache lines are
+    // written, not a byte-masked subset
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      xa[c] = va[c];
+      if (two) xa[3 + c] = va[3 + c];
+    }
+    if (odd) {
+      // plane cz's sums continued with plane cz+1's values; plane cz+1's own
+      // sums become the next pair's lo
+      T val[2][3], hs[2][3];
+#pragma unroll
+      for (int k = 0; k < 2; ++k)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          val[k][c] = lo[k][c];
+          hs[k][c] = T(0);
+        }
+      accum(q, val);
+      accum(q, hs);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        if ((mb0 >> c) & 1u) vb[c] += val[0][c] * (T(0.5) * wy);
+        if ((mb1 >> c) & 1u) vb[3 + c] += val[1][c] * (T(0.25) * wy);
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        xb[c] = vb[c];
+        if (two) xb[3 + c] = vb[3 + c];
+      }
+#pragma unroll
+      for (int k = 0; k < 2; ++k)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) lo[k][c] = hs[k][c];
+    }
   }
 }
 
@@ -922,6 +1005,30 @@ __global__ void k_cg_update(long long n, const Scalars* __restrict__ s, T* __res
   }
   acc = block_sum(acc, sred);
   if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = acc;
+}
+
+// p = z + beta p per right-hand side (blockIdx.y = rhs, n values each)
+template <typename T>
+__global__ void k_p_update_b(long long n, const Scalars* __restrict__ s, const T* __restrict__ z,
+                             T* __restrict__ p) {
+  const long long off = (long long)blockIdx.y * n;
+  z += off;
+  p += off;
+  const T beta = (T)s[blockIdx.y].beta;
+  const long long n4 = n >> 2;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n4;
+       i += (long long)gridDim.x * blockDim.x) {
+    T zz[4], pp[4];
+    ld4(z + 4 * i, zz);
+    ld4(p + 4 * i, pp);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) pp[k] = zz[k] + beta * pp[k];
+    st4(p + 4 * i, pp);
+  }
+  if (blockIdx.x == 0 && threadIdx.x < (n & 3)) {
+    const long long i = 4 * n4 + threadIdx.x;
+    p[i] = z[i] + beta * p[i];
+  }
 }
 
 // p = z + beta p

@@ -81,9 +81,9 @@ struct vx_solver {
   bool fixed_set = false, density_set = false;
   bool active_system = false;
   // level-0 PCG vectors
-  double *u64 = nullptr, *r64 = nullptr, *z64 = nullptr, *p64 = nullptr, *q64 = nullptr,
+  double *u64 = nullptr, *r64 = nullptr, *z64 = nullptr, *p64 = nullptr,
          *Ap64 = nullptr, *fm64 = nullptr;
-  float *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *q32 = nullptr, *Ap32 = nullptr,
+  float *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *Ap32 = nullptr,
         *d32 = nullptr;
   Level jac;                  // active-DOF system of the Jacobi-PCG fallback
   double* jac_minv = nullptr; // per node
@@ -196,17 +196,12 @@ struct SmoothCoef {
   double cu = 0, cr = 0;
 };
 
-struct PUpd {
-  const void* pold = nullptr;
-  void* pnew = nullptr;
-};
-
 // Every launcher below covers the s->nb right-hand sides of the current
 // operation (vectors [nb][3 Nn] of the level, contiguous).
 
-template <typename T, int MODE, bool DOT, bool PUPD = false>
+template <typename T, int MODE, bool DOT>
 int launch_tile_L(vx_solver* s, Level& L, const T* u, const T* b, T* out, double* partial,
-                  int* n_part, const SmoothCoef* sc = nullptr, const PUpd* pu = nullptr) {
+                  int* n_part, const SmoothCoef* sc = nullptr) {
   TileArgs<T> a;
   a.nx = L.nx; a.ny = L.ny; a.nz = L.nz;
   a.Lx = L.Lx; a.Ly = L.Ly; a.Lz = L.Lz;
@@ -215,9 +210,6 @@ int launch_tile_L(vx_solver* s, Level& L, const T* u, const T* b, T* out, double
   a.nb = s->nb;
   a.vstride = 3 * L.Nn;
   a.u = u; a.E = Sel<T>::E(L); a.mask = L.mask; a.b = b; a.out = out;
-  a.pold = pu ? static_cast<const T*>(pu->pold) : nullptr;
-  a.pnew = pu ? static_cast<T*>(pu->pnew) : nullptr;
-  a.scal = s->dscal;
   a.omega = (T)(sc ? sc->omega : s->opt.omega);
   a.up = sc ? static_cast<const T*>(sc->up) : nullptr;
   a.cu = (T)(sc ? sc->cu : 0.0);
@@ -231,7 +223,7 @@ int launch_tile_L(vx_solver* s, Level& L, const T* u, const T* b, T* out, double
   if (n_part) *n_part = (int)G;
   if (DOT && G * s->nb > s->n_partial) return fail(s, VX_ERR_ARG, "partial buffer too small");
   const long long launch = ((G * s->nb + 7) / 8) * 8;   // XCD-ordered 1D grid
-  hipLaunchKernelGGL((k_tile<T, MODE, DOT, PUPD>), dim3((unsigned)launch), dim3(NT), 0, s->stream,
+  hipLaunchKernelGGL((k_tile<T, MODE, DOT>), dim3((unsigned)launch), dim3(NT), 0, s->stream,
                      a);
   LAUNCHCHK();
   return VX_OK;
@@ -261,22 +253,47 @@ int dot_partials(vx_solver* s, long long n, const T* a, const T* b, int* n_part)
   return VX_OK;
 }
 
-// x += mask * (P e) from level li+1 into level li for the s->nb right-hand
-// sides (FP32: the float4 kernel when every vector is 16-byte aligned)
+// launch shape of the z-marching transfers: cols threads per plane, planes
+// split into chunks so that about TR_TARGET_GROUPS work-groups run
+TrGrid tr_grid(long long cols, int planes, bool even_chunks);
+// the XCD-ordered 1-D grid covers every work-group index only when its size
+// is a multiple of 8 (tr_block maps block b to (b mod 8) * per + b / 8)
+unsigned tr_blocks(const TrGrid& g, int nb) {
+  const long long total = (long long)g.gxy * g.nchunk * nb;
+  return (unsigned)(((total + 7) / 8) * 8);
+}
+TrGrid tr_grid(long long cols, int planes, bool even_chunks) {
+  TrGrid g;
+  g.gxy = (int)((cols + TR_THREADS - 1) / TR_THREADS);
+  int nchunk = (TR_TARGET_GROUPS + g.gxy - 1) / g.gxy;
+  nchunk = std::max(1, std::min(nchunk, planes));
+  g.per_chunk = (planes + nchunk - 1) / nchunk;
+  if (even_chunks) g.per_chunk += g.per_chunk & 1;
+  g.nchunk = (planes + g.per_chunk - 1) / g.per_chunk;
+  return g;
+}
+
+// rc = coarse mask * (R rf) into level C (= level li+1) for nb right-hand sides
 template <typename T>
-int prolong_add(vx_solver* s, int li, const T* ec, T* x) {
+int restrict_to(vx_solver* s, Level& C, const T* rf, T* rc, int nb) {
+  const TrGrid g = tr_grid((long long)C.Lx * C.Ly, C.Lz, false);
+  hipLaunchKernelGGL(k_restrict_z<T>, dim3(tr_blocks(g, nb)), dim3(TR_THREADS), 0, s->stream,
+                     C.Lx, C.Ly, C.Lz, g, nb, rf, C.mask, rc);
+  LAUNCHCHK();
+  return VX_OK;
+}
+
+// x += mask * (P e) from level li+1 into level li for nb right-hand sides
+// (default: the s->nb of the current operation)
+template <typename T>
+int prolong_add(vx_solver* s, int li, const T* ec, T* x, int nb = 0) {
   Level& L = s->lv[li];
-  const int nb = s->nb;
-  if (std::is_same<T, float>::value &&
-      ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(L.mask)) & 15) == 0 &&
-      (nb == 1 || (3 * L.Nn) % 4 == 0) && L.Nn < (1LL << 31)) {
-    hipLaunchKernelGGL(k_prolong_add_v4, dim3(ew_grid((L.Nn + 3) / 4), nb), dim3(EW_THREADS), 0,
-                       s->stream, L.Lx, L.Ly, L.Lz, reinterpret_cast<const float*>(ec), L.mask,
-                       reinterpret_cast<float*>(x));
-  } else {
-    hipLaunchKernelGGL(k_prolong_add3<T>, dim3((L.Lx + 63) / 64, L.Ly, L.Lz * nb), dim3(64), 0,
-                       s->stream, L.Lx, L.Ly, L.Lz, ec, L.mask, x);
-  }
+  const Level& C = s->lv[li + 1];
+  if (nb <= 0) nb = s->nb;
+  // fine planes in chunks of an even count (the kernel pairs planes)
+  const TrGrid g = tr_grid((long long)C.Lx * L.Ly, L.Lz, true);
+  hipLaunchKernelGGL(k_prolong_z<T>, dim3(tr_blocks(g, nb)), dim3(TR_THREADS), 0, s->stream,
+                     C.Lx, C.Ly, C.Lz, g, nb, ec, L.mask, x);
   LAUNCHCHK();
   return VX_OK;
 }
@@ -493,9 +510,7 @@ int plain_coarse(vx_solver* s, int li, const T* t, T* v, T* tmp, T* tmp2) {
     return smooth_step<T>(s, li, nullptr, t, k, v, tmp, tmp2, false, nullptr);
   }
   Level& C = s->lv[li + 1];
-  hipLaunchKernelGGL(k_restrict3<T>, dim3((C.Lx + 63) / 64, C.Ly, C.Lz * nb), dim3(64), 0,
-                     s->stream, C.Lx, C.Ly, C.Lz, t, C.mask, Sel<T>::b(C));
-  LAUNCHCHK();
+  CHK(restrict_to<T>(s, C, t, Sel<T>::b(C), nb));
   CHK(vcycle<T>(s, Sel<T>::b(C), Sel<T>::x(C), Sel<T>::t(C), Sel<T>::w(C), false, nullptr,
                 li + 1));
   HIPCHK(hipMemsetAsync(v, 0, sizeof(T) * 3 * L.Nn * nb, s->stream));
@@ -614,9 +629,7 @@ int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, T* w0, bool dot, int* n_part
     CHK((launch_tile<T, M_RESID, false>(s, li, x, b, t, nullptr, nullptr)));
     if (l0 == 0) CHK(capture_residual<T>(s, li, t));
     Level& C = s->lv[li + 1];
-    hipLaunchKernelGGL(k_restrict3<T>, dim3((C.Lx + 63) / 64, C.Ly, C.Lz * nb), dim3(64), 0,
-                       s->stream, C.Lx, C.Ly, C.Lz, t, C.mask, Sel<T>::b(C));
-    LAUNCHCHK();
+    CHK(restrict_to<T>(s, C, t, Sel<T>::b(C), nb));
   }
   // ascend
   for (int li = bottom - 1; li >= l0; --li) {
@@ -685,7 +698,7 @@ struct PcgResult {
 // beta = -z_new.A p / p.A p (A-orthogonal to the previous direction), which
 // equals Fletcher-Reeves for a fixed symmetric preconditioner.
 template <typename T>
-int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* p2, T* Ap, T* t0, double tol_rel, int max_iter,
+int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* Ap, T* t0, double tol_rel, int max_iter,
         bool stall_guard, const std::vector<double>& ratio_scale, double cb_tol,
         const std::vector<int>& iter_offset, vx_progress_cb cb, void* user, double* hist,
         int hist_cap, std::vector<PcgResult>& res, const std::vector<int>& skip, int l0 = 0) {
@@ -705,27 +718,24 @@ int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* p2, T* Ap, T* t0, double tol_re
   if (rc == VX_OK && std::any_of(frozen.begin(), frozen.end(), [](int f) { return f; }))
     rc = push_frozen(s, frozen);
   if (rc != VX_OK) { cleanup(); return rc; }
-  // both direction buffers start at 0 so that p = z + 0 * p_prev is exact
-  if (hipMemsetAsync(p, 0, sizeof(T) * n * nb, s->stream) != hipSuccess ||
-      hipMemsetAsync(p2, 0, sizeof(T) * n * nb, s->stream) != hipSuccess) {
+  // p starts at 0 so that the first p = z + 0 * p is exact
+  if (hipMemsetAsync(p, 0, sizeof(T) * n * nb, s->stream) != hipSuccess) {
     cleanup();
     return fail(s, VX_ERR_HIP, "memset p");
   }
-  T* pb[2] = {p, p2};
   std::vector<double> best(nb, INFINITY), prev(nb, NAN), prev2(nb, NAN);
   for (int it = 0; it < max_iter; ++it) {
-    T* pcur = pb[it & 1];
-    T* pprev = pb[(it + 1) & 1];
-    // p = z + beta p_prev formed inside the operator kernel; Ap; p.Ap partials
-    PUpd pu;
-    pu.pold = pprev;
-    pu.pnew = pcur;
-    rc = launch_tile_L<T, M_MATVEC, true, true>(s, L0, z, nullptr, Ap, s->partial, &np, nullptr,
-                                                 &pu);
+    // p = z + beta p, then Ap with the p.Ap partials.  (Forming p inside the
+    // operator kernel instead costs more than this extra streaming pass: that
+    // kernel then stages two halo streams and writes two outputs.)
+    hipLaunchKernelGGL(k_p_update_b<T>, dim3(EW_BLOCKS, nb), dim3(EW_THREADS), 0, s->stream, n,
+                       s->dscal, z, p);
+    if (hipGetLastError() != hipSuccess) { rc = fail(s, VX_ERR_HIP, "p update launch"); break; }
+    rc = launch_tile_L<T, M_MATVEC, true>(s, L0, p, nullptr, Ap, s->partial, &np);
     if (rc == VX_OK) rc = finalize(s, np, F_ALPHA);
     if (rc != VX_OK) break;
     hipLaunchKernelGGL(k_cg_update<T>, dim3(EW_BLOCKS, nb), dim3(EW_THREADS), 0, s->stream, n,
-                       s->dscal, u, r, pcur, Ap, s->partial);
+                       s->dscal, u, r, p, Ap, s->partial);
     if (hipGetLastError() != hipSuccess) { rc = fail(s, VX_ERR_HIP, "cg_update launch"); break; }
     rc = finalize(s, EW_BLOCKS, F_RATIO);
     if (rc == VX_OK) rc = read_scalars(s, ev);
@@ -981,7 +991,7 @@ static int level_solve(vx_solver* s, int l, const double* b, double* x, double t
   std::vector<double> one(1, 1.0);
   std::vector<int> zero(1, 0);
   std::vector<PcgResult> pr;
-  CHK(pcg<double>(s, x, s->r64, s->z64, s->p64, s->q64, s->Ap64, L.t64, tol, max_iter, false, one,
+  CHK(pcg<double>(s, x, s->r64, s->z64, s->p64, s->Ap64, L.t64, tol, max_iter, false, one,
                   tol, zero, nullptr, nullptr, nullptr, 0, pr, zero, l));
   *out = pr[0];
   return VX_OK;
@@ -1209,7 +1219,6 @@ static int alloc_pcg64(vx_solver* s) {
   CHK(dalloc(s, &s->r64, n));
   CHK(dalloc(s, &s->z64, n));
   CHK(dalloc(s, &s->p64, n));
-  CHK(dalloc(s, &s->q64, n));
   CHK(dalloc(s, &s->Ap64, n));
   CHK(dalloc(s, &s->fm64, n));
   return alloc_level_vectors<double>(s);
@@ -1220,7 +1229,6 @@ static int alloc_pcg32(vx_solver* s) {
   CHK(dalloc(s, &s->r32, n));
   CHK(dalloc(s, &s->z32, n));
   CHK(dalloc(s, &s->p32, n));
-  CHK(dalloc(s, &s->q32, n));
   CHK(dalloc(s, &s->Ap32, n));
   CHK(dalloc(s, &s->d32, n));
   return alloc_level_vectors<float>(s);
@@ -1294,7 +1302,7 @@ static int solve_impl(vx_solver* s, int nb, const double* F, double* U, const do
       std::vector<double> one(nb, 1.0);
       std::vector<int> off(nb, 0);
       if (rc == VX_OK)
-        rc = pcg<double>(s, U, s->r64, s->z64, s->p64, s->q64, s->Ap64, L0.t64, tol, max_iter,
+        rc = pcg<double>(s, U, s->r64, s->z64, s->p64, s->Ap64, L0.t64, tol, max_iter,
                          true, one, tol, off, cb, user, hist, hist_cap, pr, zero);
       for (int q = 0; q < nb; ++q) {
         if (zero[q]) continue;
@@ -1357,7 +1365,7 @@ static int solve_impl(vx_solver* s, int nb, const double* F, double* U, const do
         learned_step = s->corr_on;
         int cap = max_iter - most;
         if (learned_step && s->corr_budget > 0) cap = std::min(cap, s->corr_budget);
-        rc = pcg<float>(s, s->d32, s->r32, s->z32, s->p32, s->q32, s->Ap32, L0.t32, want, cap,
+        rc = pcg<float>(s, s->d32, s->r32, s->z32, s->p32, s->Ap32, L0.t32, want, cap,
                         false, true_ratio, tol, total, cb, user, hist, hist_cap, pr, done);
         if (rc != VX_OK) break;
         if (learned_step && s->corr_budget > 0) {
@@ -1694,10 +1702,7 @@ int vx_smooth(vx_handle s, int32_t l, int32_t prec, const double* u, const doubl
 int vx_restrict(vx_handle s, int32_t l, const double* rf, double* rc) {
   CHK(check_level(s, l));
   if (l + 1 >= (int)s->lv.size()) return fail(s, VX_ERR_ARG, "no coarser level");
-  Level& C = s->lv[l + 1];
-  hipLaunchKernelGGL(k_restrict3<double>, dim3((C.Lx + 63) / 64, C.Ly, C.Lz), dim3(64), 0,
-                     s->stream, C.Lx, C.Ly, C.Lz, rf, C.mask, rc);
-  LAUNCHCHK();
+  CHK(restrict_to<double>(s, s->lv[l + 1], rf, rc, 1));
   HIPCHK(hipStreamSynchronize(s->stream));
   return VX_OK;
 }
@@ -1707,9 +1712,7 @@ int vx_prolong(vx_handle s, int32_t l, const double* ec, double* ef) {
   if (l + 1 >= (int)s->lv.size()) return fail(s, VX_ERR_ARG, "no coarser level");
   Level& L = s->lv[l];
   HIPCHK(hipMemsetAsync(ef, 0, sizeof(double) * 3 * L.Nn, s->stream));
-  hipLaunchKernelGGL(k_prolong_add3<double>, dim3((L.Lx + 63) / 64, L.Ly, L.Lz), dim3(64), 0,
-                     s->stream, L.Lx, L.Ly, L.Lz, ec, L.mask, ef);
-  LAUNCHCHK();
+  CHK(prolong_add<double>(s, l, ec, ef, 1));
   HIPCHK(hipStreamSynchronize(s->stream));
   return VX_OK;
 }
@@ -1915,10 +1918,10 @@ extern "C" int vx_autotune(vx_handle s, int32_t precision, int32_t reps) {
         const dim3 g = tile_grid(s, L, zc);
         if ((long long)g.x * g.y * g.z * s->opt.max_batch > s->n_partial) continue;
         L.zc_tuned[pi][cls] = zc;
-        // the variants the solve launches: class 0 the PCG operator (direction
-        // update fused, p.Ap partials), 1 a smoothing step, 2 the last
-        // smoothing step of the V-cycle (with its dot-product partials)
-        auto variant = [&](auto* x, auto* y, auto* bb, auto* w, auto* q) -> int {
+        // the variants the solve launches: class 0 the PCG operator (with the
+        // p.Ap partials), 1 a smoothing step, 2 the last smoothing step of the
+        // V-cycle (with its dot-product partials)
+        auto variant = [&](auto* x, auto* y, auto* bb, auto* w) -> int {
           using T = std::remove_pointer_t<decltype(x)>;
           SmoothCoef sc;
           sc.omega = s->opt.omega;
@@ -1926,13 +1929,7 @@ extern "C" int vx_autotune(vx_handle s, int32_t precision, int32_t reps) {
           sc.cr = 0.1;
           sc.up = w;
           int np = 0;
-          if (cls == 0) {
-            PUpd pu;
-            pu.pold = q;
-            pu.pnew = w;
-            return launch_tile_L<T, M_MATVEC, true, true>(s, L, x, nullptr, y, s->partial, &np,
-                                                          nullptr, &pu);
-          }
+          if (cls == 0) return launch_tile_L<T, M_MATVEC, true>(s, L, x, nullptr, y, s->partial, &np);
           if (cls == 1)
             return launch_tile<T, M_CHEB, false>(s, (int)li, x, bb, w, nullptr, nullptr, &sc);
           return launch_tile<T, M_CHEB, true>(s, (int)li, x, bb, w, s->partial, &np, &sc);
@@ -1940,9 +1937,9 @@ extern "C" int vx_autotune(vx_handle s, int32_t precision, int32_t reps) {
         auto launch = [&]() -> int {
           if (pi == 0)
             return variant(li == 0 ? s->p64 : L.x64, li == 0 ? s->Ap64 : L.t64,
-                           li == 0 ? s->r64 : L.b64, L.w64, li == 0 ? s->q64 : L.t64);
+                           li == 0 ? s->r64 : L.b64, L.w64);
           return variant(li == 0 ? s->p32 : L.x32, li == 0 ? s->Ap32 : L.t32,
-                         li == 0 ? s->r32 : L.b32, L.w32, li == 0 ? s->q32 : L.t32);
+                         li == 0 ? s->r32 : L.b32, L.w32);
         };
         rc = launch();   // warm-up
         if (rc != VX_OK) break;
