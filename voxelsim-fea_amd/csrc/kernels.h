@@ -1007,6 +1007,75 @@ __global__ void k_cg_update(long long n, const Scalars* __restrict__ s, T* __res
   if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = acc;
 }
 
+// k_cg_update fused with the first smoothing step from zero of the next
+// V-cycle on the same level: u += alpha p ; r -= alpha Ap ; partial r.r ;
+// x1 = mask ? c * minv * r : 0 (the output k_zero_step would write from the
+// new r).  FP32, four nodes per thread through float4 / uint32 accesses
+// (every vector 16-byte aligned per right-hand side), the < 4 tail nodes one
+// by one.  Grid (EW blocks, rhs); partial[rhs * gridDim.x + block].
+__global__ void k_cg_update_x1(long long Nn, const Scalars* __restrict__ s, float* __restrict__ u,
+                               float* __restrict__ r, const float* __restrict__ p,
+                               const float* __restrict__ Ap, double* __restrict__ partial,
+                               const uint8_t* __restrict__ mask, const float* __restrict__ minv,
+                               float c, float* __restrict__ x1) {
+  __shared__ double sred[4];
+  const long long off = (long long)blockIdx.y * 3 * Nn;
+  u += off;
+  r += off;
+  p += off;
+  Ap += off;
+  x1 += off;
+  const float a = (float)s[blockIdx.y].alpha;
+  double acc = 0.0;
+  const long long ng = Nn >> 2;
+  for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < ng;
+       g += (long long)gridDim.x * blockDim.x) {
+    const unsigned mk4 = reinterpret_cast<const unsigned*>(mask)[g];
+    const float4 mv = reinterpret_cast<const float4*>(minv)[g];
+    float uu[12], rr[12], pp[12], aa[12];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      ld4(u + 12 * g + 4 * k, uu + 4 * k);
+      ld4(r + 12 * g + 4 * k, rr + 4 * k);
+      ld4(p + 12 * g + 4 * k, pp + 4 * k);
+      ld4(Ap + 12 * g + 4 * k, aa + 4 * k);
+    }
+    const float w4[4] = {c * mv.x, c * mv.y, c * mv.z, c * mv.w};
+    float xx[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) {
+      uu[i] = uu[i] + a * pp[i];
+      rr[i] = rr[i] - a * aa[i];
+      acc += (double)rr[i] * (double)rr[i];
+      const int nd = i / 3, k = i - 3 * (i / 3);
+      const unsigned mk = (mk4 >> (8 * nd)) & 0xFFu;
+      xx[i] = ((mk >> k) & 1u) ? w4[nd] * rr[i] : 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      st4(u + 12 * g + 4 * k, uu + 4 * k);
+      st4(r + 12 * g + 4 * k, rr + 4 * k);
+      st4(x1 + 12 * g + 4 * k, xx + 4 * k);
+    }
+  }
+  for (long long n = 4 * ng + blockIdx.x * (long long)blockDim.x + threadIdx.x; n < Nn;
+       n += (long long)gridDim.x * blockDim.x) {
+    const unsigned mk = mask[n];
+    const float w = c * minv[n];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const long long i = 3 * n + k;
+      u[i] = u[i] + a * p[i];
+      const float rn = r[i] - a * Ap[i];
+      r[i] = rn;
+      acc += (double)rn * (double)rn;
+      x1[i] = ((mk >> k) & 1u) ? w * rn : 0.f;
+    }
+  }
+  acc = block_sum(acc, sred);
+  if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = acc;
+}
+
 // p = z + beta p per right-hand side (blockIdx.y = rhs, n values each)
 template <typename T>
 __global__ void k_p_update_b(long long n, const Scalars* __restrict__ s, const T* __restrict__ z,

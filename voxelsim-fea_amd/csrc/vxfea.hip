@@ -329,7 +329,7 @@ int sweep(vx_solver* s, int li, const T* u, const T* b, T* out, bool dot, int* n
 // dst/tmp/tmp2 rotate; the k-th iterate lands in dst.
 template <typename T>
 int cheb_n(vx_solver* s, int li, const T* u_in, const T* b, int k, T* dst, T* tmp, T* tmp2,
-           bool dot, int* n_part) {
+           bool dot, int* n_part, bool first_done = false) {
   const double beta = s->lam_max, alpha = s->opt.cheb_ratio * beta;
   const double theta = 0.5 * (beta + alpha), delta = 0.5 * (beta - alpha);
   const double sigma = theta / delta;
@@ -343,7 +343,7 @@ int cheb_n(vx_solver* s, int li, const T* u_in, const T* b, int k, T* dst, T* tm
     const bool last = (j == k - 1);
     if (j == 0) {
       if (o == xcur) return fail(s, VX_ERR_ARG, "chebyshev aliasing");
-      CHK(sweep<T>(s, li, xcur, b, o, dot && last, n_part, 1.0 / theta));
+      if (!first_done) CHK(sweep<T>(s, li, xcur, b, o, dot && last, n_part, 1.0 / theta));
     } else {
       rho = 1.0 / (2.0 * sigma - rho_prev);
       SmoothCoef sc;
@@ -370,14 +370,15 @@ int cheb_n(vx_solver* s, int li, const T* u_in, const T* b, int k, T* dst, T* tm
 // partials of out.b.
 template <typename T>
 int smooth_n(vx_solver* s, int li, const T* u_in, const T* b, int k, T* dst, T* tmp, bool dot,
-             int* n_part) {
+             int* n_part, bool first_done = false) {
   T* bufs[2] = {dst, tmp};
   int cur = (k % 2 == 1) ? 0 : 1;   // first write goes to bufs[cur]
   const T* src = u_in;
   for (int i = 0; i < k; ++i) {
     T* o = bufs[cur];
     if (src == o) return fail(s, VX_ERR_ARG, "smoother aliasing");
-    CHK(sweep<T>(s, li, src, b, o, dot && i == k - 1, n_part, s->opt.omega));
+    if (!(first_done && i == 0))
+      CHK(sweep<T>(s, li, src, b, o, dot && i == k - 1, n_part, s->opt.omega));
     src = o;
     cur ^= 1;
   }
@@ -407,8 +408,16 @@ int alloc_level_vectors(vx_solver* s) {
 // smoothing step of the configured kind; result in dst (tmp, tmp2 scratch)
 template <typename T>
 int smooth_step(vx_solver* s, int li, const T* u_in, const T* b, int k, T* dst, T* tmp, T* tmp2,
-                bool dot, int* n_part) {
+                bool dot, int* n_part, bool first_done = false) {
   const long long n = 3 * s->lv[li].Nn * s->nb;
+  if (first_done) {
+    // the first step from zero (k_cg_update_x1) is already in tmp: both
+    // smoothers write their first step of k = 2 there
+    if (u_in || k != 2) return fail(s, VX_ERR_ARG, "precomputed first step needs k = 2 from zero");
+    if (s->opt.smoother == VX_SMOOTH_CHEBYSHEV)
+      return cheb_n<T>(s, li, u_in, b, k, dst, tmp, tmp2, dot, n_part, true);
+    return smooth_n<T>(s, li, u_in, b, k, dst, tmp, dot, n_part, true);
+  }
   if (s->opt.smoother == VX_SMOOTH_CHEBYSHEV) {
     if (u_in == dst && k % 3 == 1) {
       // the first write would land on the start vector: go through tmp
@@ -491,7 +500,8 @@ int coarse_direct(vx_solver* s, const T* b, T* x) {
 }
 
 template <typename T>
-int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, T* w0, bool dot, int* n_part, int l0);
+int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, T* w0, bool dot, int* n_part, int l0,
+           bool first_done = false);
 
 // The plain V-cycle's coarse-grid correction of level li for residual t:
 // restrict, V-cycle from level li+1, prolong (v = 0 + P x); at the coarsest
@@ -590,7 +600,8 @@ int capture_residual(vx_solver* s, int li, const T* t) {
 // coarse-grid correction (restrict, coarser levels, prolong) is replaced by
 // the network's correction between its pre- and post-smoothing.
 template <typename T>
-int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, T* w0, bool dot, int* n_part, int l0) {
+int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, T* w0, bool dot, int* n_part, int l0,
+           bool first_done) {
   const int nl = (int)s->lv.size();
   const int ns = s->opt.n_smooth;
   const int nb = s->nb;
@@ -604,8 +615,9 @@ int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, T* w0, bool dot, int* n_part
     T* x = li == l0 ? z0 : Sel<T>::x(L);
     T* t = li == l0 ? t0 : Sel<T>::t(L);
     T* w = li == l0 ? w0 : Sel<T>::w(L);
+    const bool pre_done = first_done && li == l0;
     if (li == bottom && learned) {
-      CHK(smooth_step<T>(s, li, nullptr, b, ns, x, t, w, false, nullptr));
+      CHK(smooth_step<T>(s, li, nullptr, b, ns, x, t, w, false, nullptr, pre_done));
       CHK((launch_tile<T, M_RESID, false>(s, li, x, b, t, nullptr, nullptr)));
       if (l0 == 0) CHK(capture_residual<T>(s, li, t));
       CHK(corrector_apply(s, li, reinterpret_cast<const float*>(t), reinterpret_cast<float*>(x)));
@@ -624,7 +636,7 @@ int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, T* w0, bool dot, int* n_part
       }
       break;
     }
-    CHK(smooth_step<T>(s, li, nullptr, b, ns, x, t, w, false, nullptr));
+    CHK(smooth_step<T>(s, li, nullptr, b, ns, x, t, w, false, nullptr, pre_done));
     // residual into t, then restrict into b of level li+1
     CHK((launch_tile<T, M_RESID, false>(s, li, x, b, t, nullptr, nullptr)));
     if (l0 == 0) CHK(capture_residual<T>(s, li, t));
@@ -718,6 +730,19 @@ int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* Ap, T* t0, double tol_rel, int 
   if (rc == VX_OK && std::any_of(frozen.begin(), frozen.end(), [](int f) { return f; }))
     rc = push_frozen(s, frozen);
   if (rc != VX_OK) { cleanup(); return rc; }
+  // FP32 with two smoothing steps from zero below the coarsest level: the
+  // update kernel also writes the next V-cycle's first step (its output
+  // buffer for k = 2 is t0 for both smoothers)
+  const auto al16 = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; };
+  const bool fuse_x1 = std::is_same<T, float>::value && s->opt.n_smooth == 2 &&
+                       l0 < (int)s->lv.size() - 1 && (nb == 1 || (3 * L0.Nn) % 4 == 0) &&
+                       al16(u) && al16(r) && al16(p) && al16(Ap) && al16(t0) && al16(L0.mask) &&
+                       al16(L0.mi32);
+  double x1_coef = s->opt.omega;
+  if (s->opt.smoother == VX_SMOOTH_CHEBYSHEV) {
+    const double beta = s->lam_max, alpha = s->opt.cheb_ratio * beta;
+    x1_coef = 1.0 / (0.5 * (beta + alpha));
+  }
   // p starts at 0 so that the first p = z + 0 * p is exact
   if (hipMemsetAsync(p, 0, sizeof(T) * n * nb, s->stream) != hipSuccess) {
     cleanup();
@@ -734,8 +759,17 @@ int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* Ap, T* t0, double tol_rel, int 
     rc = launch_tile_L<T, M_MATVEC, true>(s, L0, p, nullptr, Ap, s->partial, &np);
     if (rc == VX_OK) rc = finalize(s, np, F_ALPHA);
     if (rc != VX_OK) break;
-    hipLaunchKernelGGL(k_cg_update<T>, dim3(EW_BLOCKS, nb), dim3(EW_THREADS), 0, s->stream, n,
-                       s->dscal, u, r, p, Ap, s->partial);
+    if (fuse_x1) {
+      // also the next V-cycle's first smoothing step from zero, into t0
+      hipLaunchKernelGGL(k_cg_update_x1, dim3(EW_BLOCKS, nb), dim3(EW_THREADS), 0, s->stream,
+                         L0.Nn, s->dscal, reinterpret_cast<float*>(u),
+                         reinterpret_cast<float*>(r), reinterpret_cast<const float*>(p),
+                         reinterpret_cast<const float*>(Ap), s->partial, L0.mask, L0.mi32,
+                         (float)x1_coef, reinterpret_cast<float*>(t0));
+    } else {
+      hipLaunchKernelGGL(k_cg_update<T>, dim3(EW_BLOCKS, nb), dim3(EW_THREADS), 0, s->stream, n,
+                         s->dscal, u, r, p, Ap, s->partial);
+    }
     if (hipGetLastError() != hipSuccess) { rc = fail(s, VX_ERR_HIP, "cg_update launch"); break; }
     rc = finalize(s, EW_BLOCKS, F_RATIO);
     if (rc == VX_OK) rc = read_scalars(s, ev);
@@ -753,7 +787,7 @@ int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* Ap, T* t0, double tol_rel, int 
     }
     auto queue_next = [&]() -> int {
       int np2 = 0;
-      CHK(vcycle<T>(s, r, z, t0, Sel<T>::w(L0), true, &np2, l0));
+      CHK(vcycle<T>(s, r, z, t0, Sel<T>::w(L0), true, &np2, l0, fuse_x1));
       if (!flexible) return finalize(s, np2, F_BETA);
       CHK(finalize(s, np2, F_RZNEW));
       CHK(dot_partials<T>(s, n, z, Ap, &np2));
