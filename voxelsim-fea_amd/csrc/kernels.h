@@ -67,6 +67,27 @@ struct TileArgs {
   double* __restrict__ partial;   // one double per work-group (DOT variants)
 };
 
+// Raw buffer loads (gfx9 buffer resource, stride 0): a load whose byte
+// offset is at or beyond the resource's size returns 0, so the tile kernel
+// stages halo values outside the grid, and skips non-owner epilogue loads,
+// with no branch and no per-load 64-bit address arithmetic.  BUF_OOB is the
+// offset used for "no value" (as composable_kernel does).
+constexpr int BUF_WORD3 = 0x00020000;
+constexpr int BUF_OOB = (int)0x80000000u;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, bytes, BUF_WORD3);
+}
+__device__ __forceinline__ float buf_ld(__amdgpu_buffer_rsrc_t r, int off, float) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ __forceinline__ double buf_ld(__amdgpu_buffer_rsrc_t r, int off, double) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ unsigned buf_ld_u8(__amdgpu_buffer_rsrc_t r, int off) {
+  return (unsigned)__builtin_amdgcn_raw_buffer_load_b8(r, off, 0, 0);
+}
+
 // lane i <- lane i-1 inside each row of 16 lanes (DPP row_shr:1); the first
 // lane of a row receives 0.  One tile row of element columns is one DPP row.
 __device__ __forceinline__ float dpp_shr1(float v) {
@@ -130,10 +151,11 @@ __device__ inline double block_sum(double v, double* sred) {
 // DOT   adds the work-group partial of  sum out*u (MATVEC) or sum out*b
 //       (the smoothers) to partial[block]; RESID: sum out*out.
 // ---------------------------------------------------------------------------
-// FP32: at least 5 waves per SIMD (the register budget that allows it costs
-// no spills); FP64 stays at what its registers give (forcing 4 spills)
+// FP32: at least 4 waves per SIMD (at most 128 registers: no spills with the
+// layer loop unrolled by two; forcing 5 spills and is slower); FP64 keeps
+// what its registers give (forcing 3 spills and is slower)
 template <typename T, int MODE, bool DOT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 1)))
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
 void k_tile(TileArgs<T> a) {
   __shared__ T sP[2][3 * FOOT * FOOT];
   __shared__ T sQ[2][4][TILE][TILE + 1];
@@ -171,13 +193,13 @@ void k_tile(TileArgs<T> a) {
   const int plane = a.Lx * a.Ly;
   const int plane3 = 3 * plane;
 
-  // staging descriptors: global offset inside a plane (-1: outside the grid)
-  // and LDS slot of the (up to) PRE values this thread moves per plane
+  // staging descriptors: byte offset inside a plane (BUF_OOB: outside the
+  // grid) and LDS slot of the (up to) PRE values this thread moves per plane
   int goff[PRE], loff[PRE];
 #pragma unroll
   for (int k = 0; k < PRE; ++k) {
     const int i = tid + k * NT;
-    goff[k] = -1;
+    goff[k] = BUF_OOB;
     loff[k] = -1;
     if (i < FOOTVALS) {
       const int row = i / (FOOT * 3);
@@ -186,20 +208,21 @@ void k_tile(TileArgs<T> a) {
       const int c = col - xn * 3;
       const int x = X0 - 1 + xn, y = Y0 - 1 + row;
       loff[k] = (c * FOOT + row) * FOOT + xn;
-      if (x >= 0 && x < a.Lx && y >= 0 && y < a.Ly) goff[k] = (y * a.Lx + x) * 3 + c;
+      if (x >= 0 && x < a.Lx && y >= 0 && y < a.Ly)
+        goff[k] = ((y * a.Lx + x) * 3 + c) * (int)sizeof(T);
     }
   }
 
+  // one plane's buffer resource (size 0 outside the grid: every load is 0)
+  auto plane_rsrc = [&](const T* v, int pz) {
+    const bool zin = pz >= 0 && pz < a.Lz;
+    return buf_rsrc(v + (long long)(zin ? pz : 0) * plane3, zin ? plane3 * (int)sizeof(T) : 0);
+  };
   T pre[PRE];
   auto fetch = [&](int pz) {
-    const bool zin = pz >= 0 && pz < a.Lz;
-    const int base = pz * plane3;
+    const auto r = plane_rsrc(au, pz);
 #pragma unroll
-    for (int k = 0; k < PRE; ++k) {
-      T val = T(0);
-      if (zin && goff[k] >= 0) val = au[base + goff[k]];
-      pre[k] = val;
-    }
+    for (int k = 0; k < PRE; ++k) pre[k] = buf_ld(r, goff[k], T(0));
   };
   auto store = [&](int buf) {
 #pragma unroll
@@ -243,30 +266,45 @@ void k_tile(TileArgs<T> a) {
   constexpr bool NEEDB = MODE != M_MATVEC;
   constexpr bool NEEDUP = MODE == M_CHEB;
   const bool has_up = NEEDUP && aup != nullptr;
+  const int eoff = col_in ? e0 * (int)sizeof(T) : BUF_OOB;
   auto loadE = [&](int iz) -> T {
-    return (col_in && iz >= 0 && iz < a.nz) ? a.E[iz * nxy + e0] : T(0);
+    const bool zin = iz >= 0 && iz < a.nz;
+    const auto r = buf_rsrc(a.E + (long long)(zin ? iz : 0) * nxy, zin ? nxy * (int)sizeof(T) : 0);
+    return buf_ld(r, eoff, T(0));
   };
+  // epilogue operands of the owners (node n0 of plane pz; 0 for others)
+  const int moff = owner ? n0 : BUF_OOB;
+  const int voffb = owner ? 3 * n0 * (int)sizeof(T) : BUF_OOB;
   unsigned mk_n = 0;
   T b_n[3] = {T(0), T(0), T(0)}, up_n[3] = {T(0), T(0), T(0)};
   auto loadEpi = [&](int pz) {
-    if (owner && pz >= Z0 && pz < Z1) {
-      const int n = pz * plane + n0;
-      mk_n = a.mask[n];
+    const bool zin = pz >= Z0 && pz < Z1;
+    const int pzc = zin ? pz : 0;
+    const auto rm = buf_rsrc(a.mask + (long long)pzc * plane, zin ? plane : 0);
+    mk_n = buf_ld_u8(rm, moff);
+    if (NEEDB) {
+      const auto rb = plane_rsrc(ab, zin ? pz : -1);
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        if (NEEDB) b_n[c] = ab[3 * n + c];
-        if (NEEDUP) up_n[c] = has_up ? aup[3 * n + c] : T(0);
-      }
+      for (int c = 0; c < 3; ++c) b_n[c] = buf_ld(rb, voffb + c * (int)sizeof(T), T(0));
+    }
+    if (NEEDUP && has_up) {
+      const auto ru = plane_rsrc(aup, zin ? pz : -1);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) up_n[c] = buf_ld(ru, voffb + c * (int)sizeof(T), T(0));
     }
   };
   T E_n = loadE(Z0 - 1);
   loadEpi(Z0);
-  for (int j = 0; j < J; ++j) {
+  // one element layer: the bottom-plane modes / force halves / u come in
+  // (Pb, acc, ub), the top plane's go out (Pt, Yt, ut).  The layer loop is
+  // unrolled by two with the two state sets swapped, so that no register
+  // copies carry the state from one layer to the next.
+  auto layer = [&](int j, const T (&Pb)[3][4], const T (&acc)[3][4], const T (&ub)[3],
+                   T (&Pt)[3][4], T (&Yt)[3][4], T (&ut)[3]) {
     const int iz = Z0 - 1 + j;
     const int top = (j + 1) & 1;
     const T Ee = E_n;
     E_n = loadE(iz + 1);
-    T Pt[3][4], ut[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       T u00, u10, u01, u11;
@@ -274,7 +312,7 @@ void k_tile(TileArgs<T> a) {
       had2(u00, u10, u01, u11, Pt[c]);
       ut[c] = u00;
     }
-    T Yb[3][4], Yt[3][4];
+    T Yb[3][4];
     elem_modes(Pb, Pt, Ee, a.K, Yb, Yt);
     const T Es = Eprev + Ee;
     Eprev = Ee;
@@ -284,11 +322,7 @@ void k_tile(TileArgs<T> a) {
     for (int c = 0; c < 3; ++c) {
       T Y[4];
 #pragma unroll
-      for (int m = 0; m < 4; ++m) {
-        Y[m] = acc[c][m] + Yb[c][m];
-        acc[c][m] = Yt[c][m];
-        Pb[c][m] = Pt[c][m];
-      }
+      for (int m = 0; m < 4; ++m) Y[m] = acc[c][m] + Yb[c][m];
       T y00, y10, y01, y11;
       ihad2(Y, y00, y10, y01, y11);
       A0[c] = y00 + dpp_shr1(y10);
@@ -346,9 +380,14 @@ void k_tile(TileArgs<T> a) {
         }
       }
     }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) ub[c] = ut[c];
+  };
+  T Pb2[3][4], acc2[3][4], ub2[3];
+  int j = 0;
+  for (; j + 1 < J; j += 2) {
+    layer(j, Pb, acc, ub, Pb2, acc2, ub2);
+    layer(j + 1, Pb2, acc2, ub2, Pb, acc, ub);
   }
+  if (j < J) layer(j, Pb, acc, ub, Pb2, acc2, ub2);
 
   if (DOT) {
     const double s = block_sum(dacc, sred);
