@@ -345,7 +345,8 @@ void k_tile(TileArgs<T> a) {
     __syncthreads();
 
     if (iz >= Z0 && owner) {
-      const int n = iz * plane + n0;
+      // output plane base (uniform, 64-bit) + this node's 32-bit offset
+      T* const op = aout + (long long)iz * plane3 + 3 * n0;
       T y[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) y[c] = A0[c] + sQ[q][c][ty - 1][tx];
@@ -353,14 +354,14 @@ void k_tile(TileArgs<T> a) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           const T o = ((mk >> c) & 1u) ? y[c] : T(0);
-          aout[3 * n + c] = o;
+          op[c] = o;
           if (DOT) dacc += (double)o * (double)ub[c];
         }
       } else if (MODE == M_RESID) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           const T o = bb[c] - (((mk >> c) & 1u) ? y[c] : T(0));
-          aout[3 * n + c] = o;
+          op[c] = o;
           if (DOT) dacc += (double)o * (double)o;
         }
       } else {
@@ -375,7 +376,7 @@ void k_tile(TileArgs<T> a) {
             const T du = ub[c] - upv[c];
             o = ub[c] + (((mk >> c) & 1u) ? a.cu * du + a.cr * (minv * (bb[c] - y[c])) : T(0));
           }
-          aout[3 * n + c] = o;
+          op[c] = o;
           if (DOT) dacc += (double)o * (double)bb[c];
         }
       }
