@@ -32,7 +32,7 @@ sys.path.insert(0, os.path.join(ROOT, "voxelsim-fea_amd"))
 HBM_PEAK_GBS = 8000.0
 
 # the MI355X solver configuration measured as `value`
-OPT = dict(precision="mixed", smoother="chebyshev", cheb_ratio=0.1, system="active", autotune=True,
+OPT = dict(precision="mixed", smoother="chebyshev", cheb_ratio=0.05, system="active", autotune=True,
            max_levels=7,
            inner_tol=1e-6)
 
