@@ -186,3 +186,33 @@ def test_chebyshev_smoother_closed_form(k, start):
     got = mg._smooth(0, None if start == "zero" else u0, b, k, "fp64")
     assert np.abs(got - want).max() <= 1e-10 * np.abs(want).max()
     mg.close()
+
+
+def test_tile_results_independent_of_z_chunk():
+    """The z-chunk only changes how planes are split over work-groups: every
+    FP32/FP64 smoothing step, the operator and the V-cycle are bitwise the
+    same for chunks that start on even and on odd planes (the layer loop is
+    unrolled by two, and both copies must round alike)."""
+    print("case z-chunk independence 40x24x24", flush=True)
+    from core.multigrid import MGSolver
+    dims = (40, 24, 24)
+    E, fixed, ndof, rng = _problem(*dims, seed=21)
+    ms = {zc: MGSolver(*dims, fixed, nu=0.3, compute_mode="GPU", smoother="chebyshev",
+                       cheb_ratio=0.05, z_chunk=zc) for zc in (10, 5, 7, 25)}
+    for m in ms.values():
+        m.set_density(E)
+    u = rng.standard_normal(ndof)
+    b = rng.standard_normal(ndof)
+    ops = [lambda m: m._smooth(0, None, b, 2, precision="mixed"),
+           lambda m: m._smooth(0, u, b, 2, precision="mixed"),
+           lambda m: m._smooth(0, u, b, 3, precision="mixed"),
+           lambda m: m._smooth(0, u, b, 2),
+           lambda m: m._apply(0, u, precision="mixed"),
+           lambda m: m._vcycle(0, b, precision="mixed"),
+           lambda m: m._vcycle(0, b)]
+    for op in ops:
+        ref = op(ms[10])
+        for zc in (5, 7, 25):
+            assert np.array_equal(op(ms[zc]), ref), zc
+    for m in ms.values():
+        m.close()

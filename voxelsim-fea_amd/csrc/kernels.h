@@ -67,6 +67,9 @@ struct TileArgs {
   double* __restrict__ partial;   // one double per work-group (DOT variants)
 };
 
+__device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 // Raw buffer loads (gfx9 buffer resource, stride 0): a load whose byte
 // offset is at or beyond the resource's size returns 0, so the tile kernel
 // stages halo values outside the grid, and skips non-owner epilogue loads,
@@ -355,29 +358,32 @@ void k_tile(TileArgs<T> a) {
         for (int c = 0; c < 3; ++c) {
           const T o = ((mk >> c) & 1u) ? y[c] : T(0);
           op[c] = o;
-          if (DOT) dacc += (double)o * (double)ub[c];
+          if (DOT) dacc = fmaT((double)o, (double)ub[c], dacc);
         }
       } else if (MODE == M_RESID) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           const T o = bb[c] - (((mk >> c) & 1u) ? y[c] : T(0));
           op[c] = o;
-          if (DOT) dacc += (double)o * (double)o;
+          if (DOT) dacc = fmaT((double)o, (double)o, dacc);
         }
       } else {
         const T d = a.kd * (EX + sQ[q][3][ty - 1][tx]);
         const T minv = inv_diag(d);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
+          // explicit fused multiply-adds and no other contractible mul+add:
+          // the two unrolled copies of the layer body must round alike
           T o;
+          const bool fr = (mk >> c) & 1u;
           if (MODE == M_JACOBI) {
-            o = ub[c] + a.omega * (((mk >> c) & 1u) ? minv * (bb[c] - y[c]) : T(0));
+            o = fr ? fmaT(a.omega, minv * (bb[c] - y[c]), ub[c]) : ub[c];
           } else {
             const T du = ub[c] - upv[c];
-            o = ub[c] + (((mk >> c) & 1u) ? a.cu * du + a.cr * (minv * (bb[c] - y[c])) : T(0));
+            o = fr ? ub[c] + fmaT(a.cu, du, a.cr * (minv * (bb[c] - y[c]))) : ub[c];
           }
           op[c] = o;
-          if (DOT) dacc += (double)o * (double)bb[c];
+          if (DOT) dacc = fmaT((double)o, (double)bb[c], dacc);
         }
       }
     }
