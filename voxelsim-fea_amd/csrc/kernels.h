@@ -437,10 +437,12 @@ __global__ void k_node_minv(int nx, int ny, int nz, const T* __restrict__ E, T k
 //   Chebyshev j > 0: x  = x + cu_j (x - x_prev) + cr_j D^-1 (b - K x)
 // (masked DOFs stay 0).
 // ---------------------------------------------------------------------------
+// LDS budget (160 KiB): 24 forces per element + E per element, x, the previous
+// x and b per DOF, D^-1 per node, the mask byte per node.
 template <typename T>
 constexpr int coarse_max_elems() { return sizeof(T) == 8 ? 512 : 1024; }
 template <typename T>
-constexpr int coarse_max_nodes() { return sizeof(T) == 8 ? 800 : 1400; }
+constexpr int coarse_max_nodes() { return sizeof(T) == 8 ? 736 : 1400; }
 constexpr int COARSE_MAX_STEPS = 64;
 
 struct CoarseCoef {
@@ -463,6 +465,7 @@ __global__ __launch_bounds__(1024) void k_coarse_smooth(int nx, int ny, int nz,
   __shared__ T sye[MAXE * 24];
   __shared__ T sx[3 * MAXN], sxp[3 * MAXN], sb[3 * MAXN];
   __shared__ T smi[MAXN];
+  __shared__ T sE[MAXE];
   __shared__ uint8_t smk[MAXN];
   const int Lx = nx + 1, Ly = ny + 1;
   const int Nn = Lx * Ly * (nz + 1);
@@ -474,6 +477,7 @@ __global__ __launch_bounds__(1024) void k_coarse_smooth(int nx, int ny, int nz,
     smk[n] = mask[n];
     smi[n] = minv[n];
   }
+  for (int e = tid; e < Ne; e += nt) sE[e] = E[e];   // read once, used every step
   for (int i = tid; i < 3 * Nn; i += nt) sb[i] = b[i];
   __syncthreads();
   for (int i = tid; i < 3 * Nn; i += nt) {
@@ -482,6 +486,68 @@ __global__ __launch_bounds__(1024) void k_coarse_smooth(int nx, int ny, int nz,
     sxp[i] = T(0);
   }
   __syncthreads();
+  if (Ne <= nt && 3 * Nn <= nt) {
+    // one element and one DOF per thread: their LDS indices are fixed, so
+    // they are computed once instead of once per step
+    int en[8];
+    T Ee = T(0);
+    if (tid < Ne) {
+      const int ex = tid % nx, ey = (tid / nx) % ny, ez = tid / (nx * ny);
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        en[q] = 3 * (((ez + (q >> 2)) * Ly + (ey + ((q >> 1) & 1))) * Lx + (ex + (q & 1)));
+      Ee = sE[tid];
+    }
+    int slot[8];
+    bool freed = false;
+    T mi = T(0), bi = T(0);
+    if (tid < 3 * Nn) {
+      const int n = tid / 3, c = tid - 3 * n;
+      const int ix = n % Lx, iy = (n / Lx) % Ly, iz = n / (Lx * Ly);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int ex = ix - (q & 1), ey = iy - ((q >> 1) & 1), ez = iz - (q >> 2);
+        slot[q] = (ex >= 0 && ex < nx && ey >= 0 && ey < ny && ez >= 0 && ez < nz)
+                      ? ((ez * ny + ey) * nx + ex) * 24 + 3 * q + c
+                      : -1;
+      }
+      freed = (smk[n] >> c) & 1u;
+      mi = smi[n];
+      bi = sb[tid];
+    }
+    for (int st = 1; st < cc.steps; ++st) {
+      if (tid < Ne) {
+        T u[8][3], y[8][3];
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) u[q][c] = sx[en[q] + c];
+        elem_apply<T>(u, Ee, K, y);
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) sye[tid * 24 + 3 * q + c] = y[q][c];
+      }
+      __syncthreads();
+      if (tid < 3 * Nn) {
+        T kx = T(0);
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+          if (slot[q] >= 0) kx += sye[slot[q]];
+        const T x = sx[tid];
+        T xn = x;
+        if (freed) {
+          const T r = mi * (bi - kx);
+          xn = cc.cheb ? x + (T)cc.cu[st] * (x - sxp[tid]) + (T)cc.cr[st] * r : x + (T)cc.w0 * r;
+        }
+        sxp[tid] = x;
+        sx[tid] = xn;
+      }
+      __syncthreads();
+    }
+    for (int i = tid; i < 3 * Nn; i += nt) out[i] = sx[i];
+    return;
+  }
   for (int st = 1; st < cc.steps; ++st) {
     for (int e = tid; e < Ne; e += nt) {
       const int ex = e % nx, ey = (e / nx) % ny, ez = e / (nx * ny);
@@ -492,7 +558,7 @@ __global__ __launch_bounds__(1024) void k_coarse_smooth(int nx, int ny, int nz,
 #pragma unroll
         for (int c = 0; c < 3; ++c) u[q][c] = sx[3 * n + c];
       }
-      elem_apply<T>(u, E[e], K, y);
+      elem_apply<T>(u, sE[e], K, y);
 #pragma unroll
       for (int q = 0; q < 8; ++q)
 #pragma unroll

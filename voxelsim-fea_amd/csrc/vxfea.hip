@@ -459,7 +459,11 @@ int coarse_fused(vx_solver* s, Level& L, const T* b, T* x) {
     cc.cheb = 0;
     cc.w0 = s->opt.omega;
   }
-  hipLaunchKernelGGL(k_coarse_smooth<T>, dim3(s->nb), dim3(1024), 0, s->stream, L.nx, L.ny, L.nz,
+  // as many threads as the larger of the element and DOF loops needs (whole
+  // waves, at most 1024): fewer waves make each step's two barriers cheaper
+  const long long work = std::max<long long>(L.Ne, 3 * L.Nn);
+  const int threads = (int)std::min<long long>(1024, std::max<long long>(64, (work + 63) / 64 * 64));
+  hipLaunchKernelGGL(k_coarse_smooth<T>, dim3(s->nb), dim3(threads), 0, s->stream, L.nx, L.ny, L.nz,
                      Sel<T>::E(L), L.mask, Sel<T>::mi(L), b, x, kconst<T>(s), cc);
   LAUNCHCHK();
   return VX_OK;
