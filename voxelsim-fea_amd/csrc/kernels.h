@@ -1014,13 +1014,24 @@ enum {
   F_BETA_FLEX = 8   // sum = z.Ap: s.beta = -sum / pAp (flexible CG); s.rz = s.rz_new
 };
 
-__global__ void k_finalize(const double* __restrict__ partial, int n, int op, Scalars* s) {
-  __shared__ double sred[4];
+constexpr int FIN_THREADS = 1024;
+__global__ __launch_bounds__(FIN_THREADS) void k_finalize(const double* __restrict__ partial, int n,
+                                                          int op, Scalars* s) {
+  __shared__ double sred[FIN_THREADS / 64];
   partial += (long long)blockIdx.x * n;
   s += blockIdx.x;
-  double v = 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) v += partial[i];
-  v = block_sum(v, sred);
+  // four independent chains per thread keep four loads in flight
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+  const int bd = blockDim.x;
+  int i = threadIdx.x;
+  for (; i + 3 * bd < n; i += 4 * bd) {
+    v0 += partial[i];
+    v1 += partial[i + bd];
+    v2 += partial[i + 2 * bd];
+    v3 += partial[i + 3 * bd];
+  }
+  for (; i < n; i += bd) v0 += partial[i];
+  double v = block_sum((v0 + v1) + (v2 + v3), sred);
   if (threadIdx.x == 0) {
     switch (op) {
       case F_SET_TMP: s->tmp = v; break;

@@ -237,7 +237,7 @@ int launch_tile(vx_solver* s, int li, const T* u, const T* b, T* out, double* pa
 
 // per right-hand side: sum its n partials and apply op to its scalars
 int finalize(vx_solver* s, int n, int op) {
-  hipLaunchKernelGGL(k_finalize, dim3(s->nb), dim3(256), 0, s->stream, s->partial, n, op,
+  hipLaunchKernelGGL(k_finalize, dim3(s->nb), dim3(FIN_THREADS), 0, s->stream, s->partial, n, op,
                      s->dscal);
   LAUNCHCHK();
   return VX_OK;
