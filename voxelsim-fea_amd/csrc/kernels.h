@@ -747,14 +747,16 @@ __global__ __launch_bounds__(TR_THREADS) void k_restrict_z(int Lcx, int Lcy, int
   }
 }
 
-// x += fine mask * (P e) (core/multigrid.py:418-424, 434), z-marching: the
-// thread of coarse column cx and fine row iy updates fine nodes 2cx and
-// 2cx+1 of that row in every fine plane of its chunk.  Each interpolant is
-// the sum of its coarse values in (z, y, x) order times the product of the
-// 1/2 weights (a power of 2, so applying it last is exact); the sum over a
-// coarse plane is formed once and reused by the fine planes it feeds.
+// x += fine mask * (P e) (core/multigrid.py:418-424, 434) with one thread per
+// fine value (node, component) of a plane: a wave's fine loads and stores
+// cover one contiguous span (whole cache lines), its coarse loads a
+// contiguous span about half as long.  The thread marches in z over its
+// chunk's fine planes; per coarse plane it forms its interpolant's x-y sum
+// once.  Each interpolant is the sum of its coarse values in (z, y, x) order
+// times the product of the 1/2 weights (a power of 2, so applying it last is
+// exact).
 template <typename T>
-__global__ __launch_bounds__(TR_THREADS) void k_prolong_z(int Lcx, int Lcy, int Lcz, TrGrid g,
+__global__ __launch_bounds__(TR_THREADS) void k_prolong_f(int Lcx, int Lcy, int Lcz, TrGrid g,
                                                           int nb, const T* __restrict__ ec,
                                                           const uint8_t* __restrict__ fmask,
                                                           T* __restrict__ x) {
@@ -765,122 +767,86 @@ __global__ __launch_bounds__(TR_THREADS) void k_prolong_z(int Lcx, int Lcy, int 
   const int chunk = rest % g.nchunk;
   const int rhs = rest / g.nchunk;
   const int Lfx = 2 * Lcx - 1, Lfy = 2 * Lcy - 1, Lfz = 2 * Lcz - 1;
-  const int t = bxy * TR_THREADS + (int)threadIdx.x;
-  const int cx = t % Lcx, iy = t / Lcx;
-  if (iy >= Lfy) return;
+  const int fplane = Lfx * Lfy;   // nodes per fine plane
+  const int f = bxy * TR_THREADS + (int)threadIdx.x;   // value index in a plane
+  if (f >= 3 * fplane) return;
+  const int node = f / 3, c = f - 3 * node;
+  const int iy = node / Lfx, ix = node - iy * Lfx;
+  const int cx = ix >> 1, cy = iy >> 1;
+  const bool ox = ix & 1, oy = iy & 1;
   const long long cplane = 3LL * Lcx * Lcy;
   ec += (long long)rhs * cplane * Lcz;
-  x += (long long)rhs * 3LL * Lfx * Lfy * Lfz;
-  const int cy = iy >> 1;
-  const bool oy = iy & 1;
-  const bool two = cx + 1 < Lcx;   // fine node 2cx+1 exists
+  x += (long long)rhs * 3LL * fplane * Lfz;
+  // the coarse taps of this value inside a coarse plane, (ky, kx) order
+  const int t00 = 3 * (cy * Lcx + cx) + c;
+  const int t01 = t00 + 3, t10 = t00 + 3 * Lcx, t11 = t10 + 3;
+  auto csum = [&](int cz) -> T {
+    const T* P = ec + (long long)cz * cplane;
+    T sacc = P[t00];
+    if (ox) sacc += P[t01];
+    if (oy) {
+      sacc += P[t10];
+      if (ox) sacc += P[t11];
+    }
+    return sacc;
+  };
+  auto cont = [&](int cz, T sacc) -> T {   // sacc continued with plane cz's taps
+    const T* P = ec + (long long)cz * cplane;
+    sacc += P[t00];
+    if (ox) sacc += P[t01];
+    if (oy) {
+      sacc += P[t10];
+      if (ox) sacc += P[t11];
+    }
+    return sacc;
+  };
+  const T wxy = (ox ? T(0.5) : T(1)) * (oy ? T(0.5) : T(1));
   const int iz0 = chunk * g.per_chunk;
   const int iz1 = min(iz0 + g.per_chunk, Lfz);
-  // coarse values of plane cz feeding this row: [ky][kx][c]
-  auto load = [&](int cz, T q[2][2][3]) {
-    const T* row = ec + (long long)cz * cplane + 3LL * cy * Lcx + 3 * cx;
-#pragma unroll
-    for (int ky = 0; ky < 2; ++ky)
-#pragma unroll
-      for (int kx = 0; kx < 2; ++kx)
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-          q[ky][kx][c] = ((ky == 0 || oy) && (kx == 0 || two)) ? row[ky * 3 * Lcx + kx * 3 + c]
-                                                                : T(0);
-  };
-  // s[0]: even node (x taps kx = 0), s[1]: odd node (kx = 0, 1), y taps
-  // ky = 0 (and 1 on odd rows); sums continue from s
-  auto accum = [&](const T q[2][2][3], T s[2][3]) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      s[0][c] += q[0][0][c];
-      s[1][c] += q[0][0][c];
-      s[1][c] += q[0][1][c];
-      if (oy) {
-        s[0][c] += q[1][0][c];
-        s[1][c] += q[1][0][c];
-        s[1][c] += q[1][1][c];
-      }
-    }
-  };
-  const T wy = oy ? T(0.5) : T(1);
-  // fine planes in pairs (2cz, 2cz+1): the chunk starts on an even plane;
-  // both planes' loads are issued before either is updated
-  const long long fplane = (long long)Lfx * Lfy;
-  T lo[2][3];   // sums of coarse plane cz
-  {
-    T q[2][2][3];
-    load(iz0 >> 1, q);
-#pragma unroll
-    for (int k = 0; k < 2; ++k)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) lo[k][c] = T(0);
-    accum(q, lo);
-  }
-  for (int iz = iz0; iz < iz1; iz += 2) {
+  const long long fp3 = 3LL * fplane;
+  T lo = csum(iz0 >> 1);
+  // four fine planes (two coarse planes) per step, every load issued before
+  // the first store: more loads in flight per wave
+  for (int iz = iz0; iz < iz1; iz += 4) {
     const int cz = iz >> 1;
-    const bool odd = iz + 1 < iz1;
-    const long long nA = ((long long)iz * Lfy + iy) * Lfx + 2 * cx;
-    const long long nB = nA + fplane;
-    T* xa = x + 3 * nA;
-    T* xb = x + 3 * nB;
-    T va[6], vb[6], q[2][2][3];
-    unsigned ma0 = fmask[nA], ma1 = two ? fmask[nA + 1] : 0u, mb0 = 0u, mb1 = 0u;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      va[c] = xa[c];
-      va[3 + c] = two ? xa[3 + c] : T(0);
+    const bool h1 = iz + 1 < iz1, h2 = iz + 2 < iz1, h3 = iz + 3 < iz1;   // uniform
+    const long long ia = (long long)iz * fp3 + f;
+    const long long im = (long long)iz * fplane + node;
+    const unsigned ma = fmask[im];
+    const unsigned mb = h1 ? fmask[im + fplane] : 0u;
+    const unsigned mc = h2 ? fmask[im + 2LL * fplane] : 0u;
+    const unsigned md = h3 ? fmask[im + 3LL * fplane] : 0u;
+    T va = x[ia];
+    const T vb = h1 ? x[ia + fp3] : T(0);
+    const T vc = h2 ? x[ia + 2 * fp3] : T(0);
+    const T vd = h3 ? x[ia + 3 * fp3] : T(0);
+    T s1 = T(0), v1 = T(0), s2 = T(0), v3 = T(0);
+    if (h1) {
+      s1 = csum(cz + 1);        // plane cz+1's own sum
+      v1 = cont(cz + 1, lo);    // plane cz's sum continued with plane cz+1
     }
-    if (odd) {
-      mb0 = fmask[nB];
-      mb1 = two ? fmask[nB + 1] : 0u;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        vb[c] = xb[c];
-        vb[3 + c] = two ? xb[3 + c] : T(0);
-      }
-      load(cz + 1, q);
+    if (h3) {
+      s2 = csum(cz + 2);
+      v3 = cont(cz + 2, s1);
     }
-    // even plane: plane cz's sums
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      if ((ma0 >> c) & 1u) va[c] += lo[0][c] * wy;
-      if ((ma1 >> c) & 1u) va[3 + c] += lo[1][c] * (T(0.5) * wy);
+    // every value is stored (unchanged where masked): whole cache lines
+    if ((ma >> c) & 1u) va += lo * wxy;
+    x[ia] = va;
+    if (h1) {
+      T w = vb;
+      if ((mb >> c) & 1u) w += v1 * (T(0.5) * wxy);
+      x[ia + fp3] = w;
     }
-    // every value is stored (unchanged where masked): whole cache lines are
-    // written, not a byte-masked subset
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      xa[c] = va[c];
-      if (two) xa[3 + c] = va[3 + c];
+    if (h2) {
+      T w = vc;
+      if ((mc >> c) & 1u) w += s1 * wxy;
+      x[ia + 2 * fp3] = w;
     }
-    if (odd) {
-      // plane cz's sums continued with plane cz+1's values; plane cz+1's own
-      // sums become the next pair's lo
-      T val[2][3], hs[2][3];
-#pragma unroll
-      for (int k = 0; k < 2; ++k)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          val[k][c] = lo[k][c];
-          hs[k][c] = T(0);
-        }
-      accum(q, val);
-      accum(q, hs);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        if ((mb0 >> c) & 1u) vb[c] += val[0][c] * (T(0.5) * wy);
-        if ((mb1 >> c) & 1u) vb[3 + c] += val[1][c] * (T(0.25) * wy);
-      }
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        xb[c] = vb[c];
-        if (two) xb[3 + c] = vb[3 + c];
-      }
-#pragma unroll
-      for (int k = 0; k < 2; ++k)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) lo[k][c] = hs[k][c];
+    if (h3) {
+      T w = vd;
+      if ((md >> c) & 1u) w += v3 * (T(0.5) * wxy);
+      x[ia + 3 * fp3] = w;
+      lo = s2;
     }
   }
 }

@@ -255,17 +255,18 @@ int dot_partials(vx_solver* s, long long n, const T* a, const T* b, int* n_part)
 
 // launch shape of the z-marching transfers: cols threads per plane, planes
 // split into chunks so that about TR_TARGET_GROUPS work-groups run
-TrGrid tr_grid(long long cols, int planes, bool even_chunks);
+constexpr int PROLONG_TARGET_GROUPS = 8192;
+TrGrid tr_grid(long long cols, int planes, bool even_chunks, int target = TR_TARGET_GROUPS);
 // the XCD-ordered 1-D grid covers every work-group index only when its size
 // is a multiple of 8 (tr_block maps block b to (b mod 8) * per + b / 8)
 unsigned tr_blocks(const TrGrid& g, int nb) {
   const long long total = (long long)g.gxy * g.nchunk * nb;
   return (unsigned)(((total + 7) / 8) * 8);
 }
-TrGrid tr_grid(long long cols, int planes, bool even_chunks) {
+TrGrid tr_grid(long long cols, int planes, bool even_chunks, int target) {
   TrGrid g;
   g.gxy = (int)((cols + TR_THREADS - 1) / TR_THREADS);
-  int nchunk = (TR_TARGET_GROUPS + g.gxy - 1) / g.gxy;
+  int nchunk = (target + g.gxy - 1) / g.gxy;
   nchunk = std::max(1, std::min(nchunk, planes));
   g.per_chunk = (planes + nchunk - 1) / nchunk;
   if (even_chunks) g.per_chunk += g.per_chunk & 1;
@@ -290,9 +291,11 @@ int prolong_add(vx_solver* s, int li, const T* ec, T* x, int nb = 0) {
   Level& L = s->lv[li];
   const Level& C = s->lv[li + 1];
   if (nb <= 0) nb = s->nb;
-  // fine planes in chunks of an even count (the kernel pairs planes)
-  const TrGrid g = tr_grid((long long)C.Lx * L.Ly, L.Lz, true);
-  hipLaunchKernelGGL(k_prolong_z<T>, dim3(tr_blocks(g, nb)), dim3(TR_THREADS), 0, s->stream,
+  // one thread per fine value of a plane; fine planes in chunks of an even
+  // count (the kernel starts on coarse-aligned planes), enough chunks for
+  // several rounds of waves per CU
+  const TrGrid g = tr_grid(3LL * L.Lx * L.Ly, L.Lz, true, PROLONG_TARGET_GROUPS);
+  hipLaunchKernelGGL(k_prolong_f<T>, dim3(tr_blocks(g, nb)), dim3(TR_THREADS), 0, s->stream,
                      C.Lx, C.Ly, C.Lz, g, nb, ec, L.mask, x);
   LAUNCHCHK();
   return VX_OK;
