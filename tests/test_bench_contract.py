@@ -1,0 +1,44 @@
+"""CPU checks of bench.py's reporting contract: the one-GPU guard, the HBM
+traffic figure read from the committed counter summary, and the CPU
+baseline record (both of the reference's CPU modes) on a tiny workload."""
+
+import os
+import re
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr or "one MI355X" in r.stderr
+    assert r.stdout.strip() == ""
+
+
+def test_traffic_figure_matches_the_counter_summary():
+    sys.path.insert(0, ROOT)
+    import bench
+    text = open(os.path.join(ROOT, "profiles", "chebyshev_fp32_256_traffic.md")).read()
+    mb = float(re.search(r"\*\*([0-9.]+) MB\*\*", text).group(1))
+    assert bench.measured_traffic() == pytest.approx(mb * 1e6, rel=1e-9)
+
+
+def test_cpu_baseline_record():
+    sys.path.insert(0, ROOT)
+    sys.path.insert(0, os.path.join(ROOT, "voxelsim-fea_amd"))
+    import bench
+    from core import workloads as W
+    p = W.two_material(n=16)
+    p.max_levels = 3
+    r = bench.cpu_baseline(p, 10, budget_s=5)
+    for k in ("value", "unit", "cores", "kind", "sample"):
+        assert k in r
+    assert r["kind"] == "port" and r["unit"] == "s" and 1 <= r["cores"] <= 15
+    assert r["value"] > 0
+    single = r["single_process"]["value"]
+    assert r["value"] == (min(single, r["pool"]["value"]) if r["pool"] else single)
