@@ -222,8 +222,10 @@ void k_tile(TileArgs<T> a) {
     return buf_rsrc(v + (long long)(zin ? pz : 0) * plane3, zin ? plane3 * (int)sizeof(T) : 0);
   };
   T pre[PRE];
+  // planes above Z1 (the last layer's top) are never used: a zero-size
+  // resource makes those prefetches of the last two layers cost no traffic
   auto fetch = [&](int pz) {
-    const auto r = plane_rsrc(au, pz);
+    const auto r = plane_rsrc(au, pz <= Z1 ? pz : -1);
 #pragma unroll
     for (int k = 0; k < PRE; ++k) pre[k] = buf_ld(r, goff[k], T(0));
   };
@@ -307,7 +309,7 @@ void k_tile(TileArgs<T> a) {
     const int iz = Z0 - 1 + j;
     const int top = (j + 1) & 1;
     const T Ee = E_n;
-    E_n = loadE(iz + 1);
+    E_n = loadE(iz + 1 < Z1 ? iz + 1 : -1);   // the last layer is Z1 - 1
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       T u00, u10, u01, u11;
