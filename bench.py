@@ -50,7 +50,7 @@ def cpu_baseline(p, iters, budget_s=30.0):
     iters x 6.7 x apply with the GPU solver's iteration count."""
     import numpy as np
     from oracle.multigrid_ref import MGSolverRef
-    # a one-GPU box gives this process a 16-CPU share whatever os.cpu_count() says
+    # a one-GPU host may grant this process only 16 CPUs, whatever os.cpu_count() says
     threads = max(1, min((os.cpu_count() or 2) - 1, 15))
     try:
         import threadpoolctl

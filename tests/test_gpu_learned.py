@@ -5,7 +5,7 @@ The network (BF16 U-Net, core/learned.py) is trained on the GPU on residual
 show what the trained network adds over the method without it (the plain
 coarse-grid correction) on held-out residuals, that flexible CG with the
 corrector meets the same tolerance and solution as the plain solve, and that
-the FP64 true-residual judge falls back to the plain V-cycle.
+the FP64 true-residual check falls back to the plain V-cycle.
 """
 
 import numpy as np
@@ -96,7 +96,7 @@ def test_flexible_cg_with_corrector_matches_plain(setup):
     mg.detach_corrector()
 
 
-def test_true_residual_judge_falls_back(setup):
+def test_true_residual_check_falls_back(setup):
     s = setup
     torch, mg = s["torch"], s["mg"]
     from core.learned import CorrectorNet
