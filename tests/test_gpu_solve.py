@@ -245,6 +245,50 @@ def test_batched_solve_equals_single_solves(precision):
     batch.close()
 
 
+def test_batched_mixed_equals_single_mixed_solves():
+    """Two FP32 PCG paths must end on the same solution.  A single right-hand
+    side takes the fused path: the update kernel also writes the next
+    V-cycle's first smoothing step, and u += alpha p waits for the next
+    direction update (or the flush after the last step).  A batch whose
+    per-case vectors do not split into float4 rows (3 * 17^3 values) takes
+    the plain update.  The cases stop at different iterations."""
+    print("case batch mixed vs single mixed 16^3", flush=True)
+    import torch
+    from core.multigrid import MGSolver
+    p = _batch_problem(n=16, nrhs=4)
+    nrhs = p.force.shape[0]
+    assert (3 * 17 ** 3) % 4 != 0
+    E = torch.tensor(p.E_stiffness, device="cuda")
+    A = torch.tensor(p.active.astype(np.uint8), device="cuda")
+    kw = dict(precision="mixed", smoother="chebyshev", cheb_ratio=0.05, system="active",
+              inner_tol=1e-5)
+    single = MGSolver(p.nx, p.ny, p.nz, p.fixed_dofs, nu=p.nu, compute_mode="GPU", **kw)
+    single.set_density(E, active=A)
+    ref, its = [], []
+    for q in range(nrhs):
+        f = torch.tensor(p.force[q], device="cuda")
+        u = torch.empty_like(f)
+        single.solve_device(f, u, tol=1e-7)
+        ref.append(u.cpu().numpy())
+        its.append(single.last_iters)
+    print("single iterations", its, flush=True)
+    batch = MGSolver(p.nx, p.ny, p.nz, p.fixed_dofs, nu=p.nu, compute_mode="GPU",
+                     max_batch=nrhs, **kw)
+    batch.set_density(E, active=A)
+    F = torch.tensor(p.force, device="cuda")
+    U = torch.empty_like(F)
+    batch.solve_batch_device(F, U, tol=1e-7)
+    Ub = U.cpu().numpy()
+    for q in range(nrhs):
+        info = batch.last_batch[q]
+        assert info["converged"]
+        assert info["true_resid_ratio"] < 1e-7
+        assert _rel(Ub[q], ref[q]) < 1e-6, (q, _rel(Ub[q], ref[q]))
+        print("case", q, "iters", info["iters"], "bitwise", np.array_equal(Ub[q], ref[q]), flush=True)
+    single.close()
+    batch.close()
+
+
 def test_autotuned_solver_same_solution():
     """The z-chunk autotuner changes only the tiling: the solve is the same."""
     import torch

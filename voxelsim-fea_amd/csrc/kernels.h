@@ -973,7 +973,7 @@ __global__ void k_inject_mask(int Lcx, int Lcy, int Lcz, const uint8_t* __restri
 enum {
   F_SET_TMP = 0,    // s.tmp = sum
   F_FNORM = 1,      // s.fnorm = sqrt(sum)
-  F_RZ0 = 2,        // s.rz = sum
+  F_RZ0 = 2,        // s.rz = sum; beta = alpha = 0, frozen = 0
   F_ALPHA = 3,      // s.pAp = sum; s.alpha = sum == 0 ? 0 : rz / sum
   F_RATIO = 4,      // s.rr = sum; s.ratio = sqrt(sum) / fnorm
   F_BETA = 5,       // s.rz_new = sum; s.beta = rz == 0 ? 0 : sum / rz; s.rz = sum
@@ -1007,6 +1007,7 @@ __global__ __launch_bounds__(FIN_THREADS) void k_finalize(const double* __restri
       case F_RZ0:
         s->rz = v;
         s->beta = 0.0;
+        s->alpha = 0.0;   // no deferred solution update before the first step
         s->frozen = 0;
         break;
       case F_ALPHA:
@@ -1098,22 +1099,21 @@ __global__ void k_cg_update(long long n, const Scalars* __restrict__ s, T* __res
   if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = acc;
 }
 
-// k_cg_update fused with the first smoothing step from zero of the next
-// V-cycle on the same level: u += alpha p ; r -= alpha Ap ; partial r.r ;
+// The residual half of k_cg_update fused with the first smoothing step from
+// zero of the next V-cycle on the same level: r -= alpha Ap ; partial r.r ;
 // x1 = mask ? c * minv * r : 0 (the output k_zero_step would write from the
-// new r).  FP32, four nodes per thread through float4 / uint32 accesses
-// (every vector 16-byte aligned per right-hand side), the < 4 tail nodes one
-// by one.  Grid (EW blocks, rhs); partial[rhs * gridDim.x + block].
-__global__ void k_cg_update_x1(long long Nn, const Scalars* __restrict__ s, float* __restrict__ u,
-                               float* __restrict__ r, const float* __restrict__ p,
+// new r).  The solution update u += alpha p is deferred to the next
+// direction update (k_p_update_u), which streams p anyway.  FP32, four nodes
+// per thread through float4 / uint32 accesses (every vector 16-byte aligned
+// per right-hand side), the < 4 tail nodes one by one.  Grid (EW blocks,
+// rhs); partial[rhs * gridDim.x + block].
+__global__ void k_cg_update_x1(long long Nn, const Scalars* __restrict__ s, float* __restrict__ r,
                                const float* __restrict__ Ap, double* __restrict__ partial,
                                const uint8_t* __restrict__ mask, const float* __restrict__ minv,
                                float c, float* __restrict__ x1) {
   __shared__ double sred[4];
   const long long off = (long long)blockIdx.y * 3 * Nn;
-  u += off;
   r += off;
-  p += off;
   Ap += off;
   x1 += off;
   const float a = (float)s[blockIdx.y].alpha;
@@ -1123,19 +1123,16 @@ __global__ void k_cg_update_x1(long long Nn, const Scalars* __restrict__ s, floa
        g += (long long)gridDim.x * blockDim.x) {
     const unsigned mk4 = reinterpret_cast<const unsigned*>(mask)[g];
     const float4 mv = reinterpret_cast<const float4*>(minv)[g];
-    float uu[12], rr[12], pp[12], aa[12];
+    float rr[12], aa[12];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      ld4(u + 12 * g + 4 * k, uu + 4 * k);
       ld4(r + 12 * g + 4 * k, rr + 4 * k);
-      ld4(p + 12 * g + 4 * k, pp + 4 * k);
       ld4(Ap + 12 * g + 4 * k, aa + 4 * k);
     }
     const float w4[4] = {c * mv.x, c * mv.y, c * mv.z, c * mv.w};
     float xx[12];
 #pragma unroll
     for (int i = 0; i < 12; ++i) {
-      uu[i] = uu[i] + a * pp[i];
       rr[i] = rr[i] - a * aa[i];
       acc += (double)rr[i] * (double)rr[i];
       const int nd = i / 3, k = i - 3 * (i / 3);
@@ -1144,7 +1141,6 @@ __global__ void k_cg_update_x1(long long Nn, const Scalars* __restrict__ s, floa
     }
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      st4(u + 12 * g + 4 * k, uu + 4 * k);
       st4(r + 12 * g + 4 * k, rr + 4 * k);
       st4(x1 + 12 * g + 4 * k, xx + 4 * k);
     }
@@ -1156,7 +1152,6 @@ __global__ void k_cg_update_x1(long long Nn, const Scalars* __restrict__ s, floa
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       const long long i = 3 * n + k;
-      u[i] = u[i] + a * p[i];
       const float rn = r[i] - a * Ap[i];
       r[i] = rn;
       acc += (double)rn * (double)rn;
@@ -1189,6 +1184,55 @@ __global__ void k_p_update_b(long long n, const Scalars* __restrict__ s, const T
     const long long i = 4 * n4 + threadIdx.x;
     p[i] = z[i] + beta * p[i];
   }
+}
+
+// The deferred solution update of FP32 PCG with the direction update: u +=
+// alpha p with the previous step's alpha (this step's F_ALPHA has not run
+// yet; 0 before the first step), then p = z + beta p; the arithmetic of
+// k_cg_update's u update followed by k_p_update_b.  Per right-hand side
+// (blockIdx.y, n values each).
+template <typename T>
+__global__ void k_p_update_u(long long n, const Scalars* __restrict__ s, const T* __restrict__ z,
+                             T* __restrict__ p, T* __restrict__ u) {
+  const long long off = (long long)blockIdx.y * n;
+  z += off;
+  p += off;
+  u += off;
+  const T alpha = (T)s[blockIdx.y].alpha;
+  const T beta = (T)s[blockIdx.y].beta;
+  const long long n4 = n >> 2;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n4;
+       i += (long long)gridDim.x * blockDim.x) {
+    T zz[4], pp[4], uu[4];
+    ld4(z + 4 * i, zz);
+    ld4(p + 4 * i, pp);
+    ld4(u + 4 * i, uu);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      uu[k] = uu[k] + alpha * pp[k];
+      pp[k] = zz[k] + beta * pp[k];
+    }
+    st4(u + 4 * i, uu);
+    st4(p + 4 * i, pp);
+  }
+  if (blockIdx.x == 0 && threadIdx.x < (n & 3)) {
+    const long long i = 4 * n4 + threadIdx.x;
+    u[i] = u[i] + alpha * p[i];
+    p[i] = z[i] + beta * p[i];
+  }
+}
+
+// the last deferred update: u += alpha p per right-hand side (blockIdx.y)
+template <typename T>
+__global__ void k_u_flush(long long n, const Scalars* __restrict__ s, const T* __restrict__ p,
+                          T* __restrict__ u) {
+  const long long off = (long long)blockIdx.y * n;
+  p += off;
+  u += off;
+  const T alpha = (T)s[blockIdx.y].alpha;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    u[i] = u[i] + alpha * p[i];
 }
 
 // p = z + beta p

@@ -745,6 +745,9 @@ int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* Ap, T* t0, double tol_rel, int 
                        l0 < (int)s->lv.size() - 1 && (nb == 1 || (3 * L0.Nn) % 4 == 0) &&
                        al16(u) && al16(r) && al16(p) && al16(Ap) && al16(t0) && al16(L0.mask) &&
                        al16(L0.mi32);
+  // with that fusion the solution update u += alpha p is deferred to the next
+  // direction update (which streams p anyway) and a flush at the end
+  const bool lazy_u = fuse_x1;
   double x1_coef = s->opt.omega;
   if (s->opt.smoother == VX_SMOOTH_CHEBYSHEV) {
     const double beta = s->lam_max, alpha = s->opt.cheb_ratio * beta;
@@ -760,17 +763,21 @@ int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* Ap, T* t0, double tol_rel, int 
     // p = z + beta p, then Ap with the p.Ap partials.  (Forming p inside the
     // operator kernel instead costs more than this extra streaming pass: that
     // kernel then stages two halo streams and writes two outputs.)
-    hipLaunchKernelGGL(k_p_update_b<T>, dim3(EW_BLOCKS, nb), dim3(EW_THREADS), 0, s->stream, n,
-                       s->dscal, z, p);
+    if (lazy_u)
+      hipLaunchKernelGGL(k_p_update_u<T>, dim3(EW_BLOCKS, nb), dim3(EW_THREADS), 0, s->stream, n,
+                         s->dscal, z, p, u);
+    else
+      hipLaunchKernelGGL(k_p_update_b<T>, dim3(EW_BLOCKS, nb), dim3(EW_THREADS), 0, s->stream, n,
+                         s->dscal, z, p);
     if (hipGetLastError() != hipSuccess) { rc = fail(s, VX_ERR_HIP, "p update launch"); break; }
     rc = launch_tile_L<T, M_MATVEC, true>(s, L0, p, nullptr, Ap, s->partial, &np);
     if (rc == VX_OK) rc = finalize(s, np, F_ALPHA);
     if (rc != VX_OK) break;
     if (fuse_x1) {
       // also the next V-cycle's first smoothing step from zero, into t0
+      // (u += alpha p waits for the next direction update or the flush)
       hipLaunchKernelGGL(k_cg_update_x1, dim3(EW_BLOCKS, nb), dim3(EW_THREADS), 0, s->stream,
-                         L0.Nn, s->dscal, reinterpret_cast<float*>(u),
-                         reinterpret_cast<float*>(r), reinterpret_cast<const float*>(p),
+                         L0.Nn, s->dscal, reinterpret_cast<float*>(r),
                          reinterpret_cast<const float*>(Ap), s->partial, L0.mask, L0.mi32,
                          (float)x1_coef, reinterpret_cast<float*>(t0));
     } else {
@@ -859,6 +866,13 @@ int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* Ap, T* t0, double tol_rel, int 
       rc = queue_next();
       if (rc != VX_OK) break;
     }
+  }
+  if (lazy_u && rc == VX_OK) {
+    // the last step's u += alpha p (alpha is 0 for a right-hand side that
+    // stopped earlier: its pending update was applied by the step after)
+    hipLaunchKernelGGL(k_u_flush<T>, dim3(EW_BLOCKS, nb), dim3(EW_THREADS), 0, s->stream, n,
+                       s->dscal, p, u);
+    if (hipGetLastError() != hipSuccess) rc = fail(s, VX_ERR_HIP, "u flush launch");
   }
   cleanup();
   return rc;
