@@ -671,12 +671,15 @@ __device__ __forceinline__ bool tr_block(int total, int& v) {
 // thread of coarse column (jx, jy) reads each fine plane once.  Each coarse
 // value is the tap-by-tap sum over (z, y, x) in that order, the odd plane
 // below carried from the previous coarse plane as its own partial sum (the
-// weights are powers of 2, so that partial is exact to reuse).
+// weights are powers of 2, so that partial is exact to reuse).  With x1 set,
+// also the coarse level's first smoothing step from zero, w0 * minv * rc.
 template <typename T>
 __global__ __launch_bounds__(TR_THREADS) void k_restrict_z(int Lcx, int Lcy, int Lcz, TrGrid g,
                                                            int nb, const T* __restrict__ rf,
                                                            const uint8_t* __restrict__ cmask,
-                                                           T* __restrict__ rc) {
+                                                           T* __restrict__ rc,
+                                                           const T* __restrict__ cminv, T w0,
+                                                           T* __restrict__ x1) {
   int v;
   if (!tr_block(g.gxy * g.nchunk * nb, v)) return;
   const int bxy = v % g.gxy;
@@ -690,6 +693,7 @@ __global__ __launch_bounds__(TR_THREADS) void k_restrict_z(int Lcx, int Lcy, int
   const long long fplane = 3LL * Lfx * Lfy;
   rf += (long long)rhs * fplane * Lfz;
   rc += (long long)rhs * 3LL * Lcx * Lcy * Lcz;
+  if (x1) x1 += (long long)rhs * 3LL * Lcx * Lcy * Lcz;
   const int jz0 = chunk * g.per_chunk;
   const int jz1 = min(jz0 + g.per_chunk, Lcz);
   // x-y taps of this column (y outer, x inner): fine offsets and weights,
@@ -746,6 +750,14 @@ __global__ __launch_bounds__(TR_THREADS) void k_restrict_z(int Lcx, int Lcy, int
     rc[3 * J] = (mk & 1u) ? a[0] : T(0);
     rc[3 * J + 1] = (mk & 2u) ? a[1] : T(0);
     rc[3 * J + 2] = (mk & 4u) ? a[2] : T(0);
+    if (x1) {
+      // the coarse level's first smoothing step from zero, as k_zero_step
+      // forms it from rc
+      const T wm = w0 * cminv[J];
+      x1[3 * J] = (mk & 1u) ? wm * a[0] : T(0);
+      x1[3 * J + 1] = (mk & 2u) ? wm * a[1] : T(0);
+      x1[3 * J + 2] = (mk & 4u) ? wm * a[2] : T(0);
+    }
   }
 }
 

@@ -276,10 +276,11 @@ TrGrid tr_grid(long long cols, int planes, bool even_chunks, int target) {
 
 // rc = coarse mask * (R rf) into level C (= level li+1) for nb right-hand sides
 template <typename T>
-int restrict_to(vx_solver* s, Level& C, const T* rf, T* rc, int nb) {
+int restrict_to(vx_solver* s, Level& C, const T* rf, T* rc, int nb, T* x1 = nullptr,
+                double w0 = 0.0) {
   const TrGrid g = tr_grid((long long)C.Lx * C.Ly, C.Lz, false);
   hipLaunchKernelGGL(k_restrict_z<T>, dim3(tr_blocks(g, nb)), dim3(TR_THREADS), 0, s->stream,
-                     C.Lx, C.Ly, C.Lz, g, nb, rf, C.mask, rc);
+                     C.Lx, C.Ly, C.Lz, g, nb, rf, C.mask, rc, (const T*)Sel<T>::mi(C), (T)w0, x1);
   LAUNCHCHK();
   return VX_OK;
 }
@@ -406,6 +407,16 @@ int alloc_level_vectors(vx_solver* s) {
     CHK(dalloc(s, &Sel<T>::w(L), n));
   }
   return VX_OK;
+}
+
+// weight of the first smoothing step from zero, x1 = c D^-1 b: 1/theta for
+// Chebyshev, omega for damped Jacobi
+static double first_step_coef(const vx_solver* s) {
+  if (s->opt.smoother == VX_SMOOTH_CHEBYSHEV) {
+    const double beta = s->lam_max, alpha = s->opt.cheb_ratio * beta;
+    return 1.0 / (0.5 * (beta + alpha));
+  }
+  return s->opt.omega;
 }
 
 // smoothing step of the configured kind; result in dst (tmp, tmp2 scratch)
@@ -615,14 +626,17 @@ int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, T* w0, bool dot, int* n_part
   const bool learned = std::is_same<T, float>::value && s->corr_on && s->corr_level >= l0 &&
                        s->corr_level < nl;
   const int bottom = learned ? s->corr_level : nl - 1;
-  // descend
+  // descend.  With two smoothing steps, the restriction into a level that
+  // smooths from zero (not the coarsest) also writes that level's first
+  // step into its t vector, where the smoother expects it.
+  bool restricted_x1 = false;
   for (int li = l0; li <= bottom; ++li) {
     Level& L = s->lv[li];
     const T* b = li == l0 ? b0 : Sel<T>::b(L);
     T* x = li == l0 ? z0 : Sel<T>::x(L);
     T* t = li == l0 ? t0 : Sel<T>::t(L);
     T* w = li == l0 ? w0 : Sel<T>::w(L);
-    const bool pre_done = first_done && li == l0;
+    const bool pre_done = li == l0 ? first_done : restricted_x1;
     if (li == bottom && learned) {
       CHK(smooth_step<T>(s, li, nullptr, b, ns, x, t, w, false, nullptr, pre_done));
       CHK((launch_tile<T, M_RESID, false>(s, li, x, b, t, nullptr, nullptr)));
@@ -648,7 +662,11 @@ int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, T* w0, bool dot, int* n_part
     CHK((launch_tile<T, M_RESID, false>(s, li, x, b, t, nullptr, nullptr)));
     if (l0 == 0) CHK(capture_residual<T>(s, li, t));
     Level& C = s->lv[li + 1];
-    CHK(restrict_to<T>(s, C, t, Sel<T>::b(C), nb));
+    restricted_x1 = ns == 2 && li + 1 < nl - 1;
+    if (restricted_x1)
+      CHK(restrict_to<T>(s, C, t, Sel<T>::b(C), nb, Sel<T>::t(C), first_step_coef(s)));
+    else
+      CHK(restrict_to<T>(s, C, t, Sel<T>::b(C), nb));
   }
   // ascend
   for (int li = bottom - 1; li >= l0; --li) {
@@ -748,11 +766,7 @@ int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* Ap, T* t0, double tol_rel, int 
   // with that fusion the solution update u += alpha p is deferred to the next
   // direction update (which streams p anyway) and a flush at the end
   const bool lazy_u = fuse_x1;
-  double x1_coef = s->opt.omega;
-  if (s->opt.smoother == VX_SMOOTH_CHEBYSHEV) {
-    const double beta = s->lam_max, alpha = s->opt.cheb_ratio * beta;
-    x1_coef = 1.0 / (0.5 * (beta + alpha));
-  }
+  const double x1_coef = first_step_coef(s);
   // p starts at 0 so that the first p = z + 0 * p is exact
   if (hipMemsetAsync(p, 0, sizeof(T) * n * nb, s->stream) != hipSuccess) {
     cleanup();
