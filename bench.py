@@ -33,8 +33,9 @@ HBM_PEAK_GBS = 8000.0
 
 # the MI355X solver configuration measured as `value`
 OPT = dict(precision="mixed", smoother="chebyshev", cheb_ratio=0.05, system="active", autotune=True,
-           max_levels=7,
-           inner_tol=1e-5)
+           max_levels=7, inner_tol=1e-5,
+           # variable V-cycle: more smoothing steps on the latency-bound coarse levels
+           level_smooth=[2, 2, 6, 12, 24, 24])
 
 
 def log(msg):

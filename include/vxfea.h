@@ -62,7 +62,8 @@ typedef struct {
   int32_t precision;          /* VX_FP64 | VX_MIXED */
   int32_t smoother;           /* VX_SMOOTH_* */
   int32_t coarse_mode;        /* VX_COARSE_* */
-  int32_t cheb_degree;        /* sweeps per Chebyshev smoothing step */
+  int32_t cheb_degree;        /* unused: the Chebyshev polynomial degree of a smoothing pass is
+                                 that level's step count (n_smooth / level_smooth) */
   double cheb_ratio;          /* lower end of the Chebyshev interval / lambda_max */
   double inner_tol;           /* mixed: relative tolerance of each FP32 inner solve */
   int32_t max_outer;          /* mixed: max iterative-refinement steps */
@@ -72,7 +73,15 @@ typedef struct {
   int32_t max_batch;          /* right-hand sides per vx_solve_batch call (workspace size) */
   int32_t flexible;           /* 1: flexible CG direction update (-z.Ap / p.Ap) also without a
                                  corrector (the default update is Fletcher-Reeves) */
-  int32_t reserved[5];
+  uint8_t level_smooth[16];   /* smoothing steps of level l (pre and post), l < 16; 0: n_smooth.
+                                 The reference smooths every level alike (core/multigrid.py:
+                                 426-436).  Counts that grow toward the coarse levels give a
+                                 variable V-cycle (Bramble, Pasciak & Xu, Math. Comp. 56, 1991),
+                                 whose preconditioner stays uniform for rediscretised coarse
+                                 operators; those levels' launches are latency-bound, so their
+                                 extra steps cost little.  The coarsest level uses
+                                 coarse_sweeps. */
+  int32_t reserved[1];
 } vx_options;
 
 /* Per-solve statistics (mirrors MGSolver.last_* of core/multigrid.py:482-517). */

@@ -100,8 +100,11 @@ class MGSolverRef:
 
     def __init__(self, nx, ny, nz, fixed_dofs, nu=0.3, n_smooth=2, omega=0.6,
                  min_elems=4, max_levels=6, pool_workers=0, verbose=False,
-                 coarse="sweeps"):
+                 coarse="sweeps", level_smooth=None):
         self.coarse = coarse
+        # per-level smoothing steps (level 0 first; 0 or past the list:
+        # n_smooth).  The reference smooths every level alike.
+        self.level_smooth = list(level_smooth or [])
         self.coarse_chol = None
         self.nx, self.ny, self.nz = nx, ny, nz
         self.ndof = 3 * (nx + 1) * (ny + 1) * (nz + 1)
@@ -242,10 +245,17 @@ class MGSolverRef:
             if self.coarse_chol is not None and li > 0:
                 return self._coarse_solve(b)
             return self._smooth(li, np.zeros_like(b), b, 30)
-        u = self._smooth(li, np.zeros_like(b), b, self.n_smooth)
+        ns = self._ns(li)
+        u = self._smooth(li, np.zeros_like(b), b, ns)
         rc = self._restrict(li, b - self._apply(li, u))
         u = u + self._prolongate(li, self._vcycle(li + 1, rc))
-        return self._smooth(li, u, b, self.n_smooth)
+        return self._smooth(li, u, b, ns)
+
+    def _ns(self, li):
+        """Smoothing steps of level li (n_smooth everywhere in the reference,
+        core/multigrid.py:426-436)."""
+        k = self.level_smooth[li] if li < len(self.level_smooth) else 0
+        return k if k else self.n_smooth
 
     def solve(self, Evec_full, f, tol=1e-4, max_cg=None, x0=None,
               progress_cb=None):

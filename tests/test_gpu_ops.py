@@ -35,7 +35,8 @@ def _pair(nx, ny, nz, fixed, E, nu=0.3, **kw):
     g = MGSolver(nx, ny, nz, fixed, nu=nu, compute_mode="GPU", **kw)
     g.set_density(E)
     o = MGSolverRef(nx, ny, nz, fixed, nu=nu, **{k: v for k, v in kw.items()
-                                                   if k in ("max_levels", "n_smooth", "omega")})
+                                                   if k in ("max_levels", "n_smooth", "omega",
+                                                            "level_smooth")})
     o.set_density(E)
     return g, o
 
@@ -84,7 +85,9 @@ def test_smooth_restrict_prolong(dims):
 
 @pytest.mark.parametrize("dims,kw", [((16, 8, 8), {}), ((8, 8, 8), {"max_levels": 3}),
                                      ((6, 10, 4), {"n_smooth": 3}), ((5, 7, 3), {}),
-                                     ((160, 12, 4), {"max_levels": 2})])
+                                     ((160, 12, 4), {"max_levels": 2}),
+                                     ((16, 8, 8), {"level_smooth": [2, 5, 3]}),
+                                     ((8, 8, 8), {"max_levels": 3, "level_smooth": [1, 4]})])
 def test_vcycle(dims, kw):
     print("case vcycle", dims, kw, flush=True)
     E, fixed, ndof, rng = _problem(*dims, seed=5)
@@ -139,7 +142,7 @@ def test_vcycle_direct_coarse_singular_keeps_sweeps():
     g.close()
 
 
-@pytest.mark.parametrize("k", [1, 2, 3, 4])
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5, 6])
 @pytest.mark.parametrize("start", ["zero", "u"])
 def test_chebyshev_smoother_closed_form(k, start):
     """k Chebyshev steps on D^-1 A over [ratio*lam, lam] leave the error

@@ -125,6 +125,82 @@ def test_vcycle_equals_dense_operator_product():
     assert np.allclose(mg._vcycle(0, b), z, rtol=1e-12, atol=1e-14 * np.abs(z).max())
 
 
+def _dense_vcycle_operator(mg, n):
+    """B with B b = V-cycle(b), column by column (free DOFs only)."""
+    free = mg.levels[0]["free"]
+    B = np.zeros((n, n))
+    for j in np.flatnonzero(free):
+        e = np.zeros(n)
+        e[j] = 1.0
+        B[:, j] = mg._vcycle(0, e)
+    f = np.flatnonzero(free)
+    return B[np.ix_(f, f)]
+
+
+def test_vcycle_level_smooth_dense_and_symmetric():
+    """Per-level smoothing counts (2 on level 0, 5 on level 1, 30 sweeps on
+    the coarsest): the V-cycle equals the dense composition of its smoothers
+    and transfers, and stays a symmetric positive definite operator, which
+    needs the same count before and after the coarse correction on each
+    level and a convergent smoother (omega 0.35 < 2 / 5.32; the reference's
+    0.6 is not, for these contrasts).  A cycle that posts one more step than
+    it pre-smooths is not symmetric: the check can tell."""
+    dims = (4, 4, 4)
+    rng, n, E, fixed = _prob(*dims, seed=4)
+    mg = MGSolverRef(*dims, fixed, nu=0.3, max_levels=3, min_elems=1, level_smooth=[2, 5],
+                     omega=0.35)
+    mg.set_density(E)
+    L = mg.levels
+    assert len(L) == 3 and [mg._ns(0), mg._ns(1)] == [2, 5]
+    A = [np.diag(lv["free"].astype(float)) @ _assemble(*lv["dims"], lv["_scaled"], 0.3).toarray()
+         for lv in L]
+    D = [np.diag(np.where(lv["free"], lv["Minv"], 0.0)) for lv in L]
+    P, M = [], [np.diag(lv["free"].astype(float)) for lv in L]
+    for li in range(2):
+        idx, w = mg.prolong[li]
+        Pl = np.zeros((L[li]["ndof"], L[li + 1]["ndof"]))
+        for i in range(idx.shape[0]):
+            for k in range(8):
+                for c in range(3):
+                    Pl[3 * i + c, 3 * idx[i, k] + c] += w[i, k]
+        P.append(Pl)
+
+    def smooth(l, u, b, k):
+        for _ in range(k):
+            u = u + 0.35 * D[l] @ (b - A[l] @ u)
+        return u
+
+    def vcyc(l, b):
+        if l == 2:
+            return smooth(2, np.zeros(L[2]["ndof"]), b, 30)
+        k = [2, 5][l]
+        v = smooth(l, np.zeros(L[l]["ndof"]), b, k)
+        ec = vcyc(l + 1, M[l + 1] @ P[l].T @ (b - A[l] @ v))
+        return smooth(l, v + M[l] @ P[l] @ ec, b, k)
+
+    b = np.where(L[0]["free"], rng.standard_normal(n), 0.0)
+    z = vcyc(0, b)
+    assert np.allclose(mg._vcycle(0, b), z, rtol=1e-12, atol=1e-13 * np.abs(z).max())
+    B = _dense_vcycle_operator(mg, n)
+    assert np.abs(B - B.T).max() <= 1e-10 * np.abs(B).max()
+    assert np.linalg.eigvalsh(0.5 * (B + B.T)).min() > 0
+
+    class Unbalanced(MGSolverRef):
+        def _vcycle(self, li, b):
+            if li == len(self.levels) - 1:
+                return self._smooth(li, np.zeros_like(b), b, 30)
+            u = self._smooth(li, np.zeros_like(b), b, self._ns(li))
+            rc = self._restrict(li, b - self._apply(li, u))
+            u = u + self._prolongate(li, self._vcycle(li + 1, rc))
+            return self._smooth(li, u, b, self._ns(li) + 1)
+
+    bad = Unbalanced(*dims, fixed, nu=0.3, max_levels=3, min_elems=1, level_smooth=[2, 5],
+                     omega=0.35)
+    bad.set_density(E)
+    Bb = _dense_vcycle_operator(bad, n)
+    assert np.abs(Bb - Bb.T).max() > 1e-3 * np.abs(Bb).max()
+
+
 def test_direct_coarse_vcycle():
     """coarse="direct": the dense operator is the sparse assembly, the coarse
     solve satisfies K x = b on the free DOFs, and the two-grid cycle is the

@@ -52,7 +52,8 @@ class vx_options(ctypes.Structure):
         ("e_coarsen", ctypes.c_int32),
         ("max_batch", ctypes.c_int32),
         ("flexible", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 5),
+        ("level_smooth", ctypes.c_uint8 * 16),
+        ("reserved", ctypes.c_int32 * 1),
     ]
 
 

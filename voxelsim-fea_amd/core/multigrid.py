@@ -60,15 +60,29 @@ def _require_gpu(plan, what):
             f"MI355X only (compute plan: {plan.label})")
 
 
+def _level_smooth(steps):
+    """Per-level smoothing steps (level 0 first; levels past the list and
+    entries of 0 use n_smooth) as the option's 16 bytes."""
+    if steps is None:
+        return None
+    steps = [int(k) for k in steps]
+    if len(steps) > 16 or any(k < 0 or k > 255 for k in steps):
+        raise ValueError("level_smooth: at most 16 step counts, each 0-255")
+    return (ctypes.c_uint8 * 16)(*steps)
+
+
 class MGSolver:
-    """Multigrid-preconditioned CG on the full voxel grid (HIP)."""
+    """Multigrid-preconditioned CG on the full voxel grid (HIP).
+
+    ``level_smooth`` gives the smoothing steps per level (pre and post, level 0
+    first); the reference uses ``n_smooth`` on every level."""
 
     def __init__(self, nx, ny, nz, fixed_dofs, nu=0.3, xp=None, n_smooth=2, omega=0.6,
                  min_elems=4, max_levels=6, compute_mode="AUTO", cpu_threads=0,
                  verbose=False, precision="fp64", coarse_sweeps=30, inner_tol=1e-5,
                  max_outer=20, z_chunk=0, smoother="jacobi", cheb_ratio=0.3,
                  system="full", e_coarsen="mean", max_batch=1, flexible=False, pad=False,
-                 autotune=False, coarse="sweeps"):
+                 autotune=False, coarse="sweeps", level_smooth=None):
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
         self.nelem = self.nx * self.ny * self.nz
         self.ndof = 3 * (self.nx + 1) * (self.ny + 1) * (self.nz + 1)
@@ -98,7 +112,8 @@ class MGSolver:
             e_coarsen={"mean": 0, "geometric": 1, "harmonic": 2}[e_coarsen],
             max_batch=max_batch, flexible=1 if flexible else 0,
             coarse_mode={"sweeps": _native.VX_COARSE_SWEEPS,
-                         "direct": _native.VX_COARSE_DIRECT}[coarse])
+                         "direct": _native.VX_COARSE_DIRECT}[coarse],
+            level_smooth=_level_smooth(level_smooth))
         self.max_batch = max_batch
         h = ctypes.c_void_p()
         px, py, pz = self._pdims

@@ -329,16 +329,17 @@ int sweep(vx_solver* s, int li, const T* u, const T* b, T* out, bool dot, int* n
 
 // Chebyshev smoothing (degree k) on D^-1 A over [ratio*lmax, lmax]:
 // x1 = x0 + D^-1 r0 / theta, x_{j+1} = x_j + rho_j rho_{j-1} (x_j - x_{j-1})
-// + (2 rho_j / delta) D^-1 r_j.  u_in == nullptr starts from zero.  Buffers
-// dst/tmp/tmp2 rotate; the k-th iterate lands in dst.
+// + (2 rho_j / delta) D^-1 r_j.  u_in == nullptr starts from zero.  The
+// iterates alternate between tmp and tmp2 (a step may overwrite the iterate
+// before the one it reads) and the k-th lands in dst; the first is in tmp
+// for every k >= 2.
 template <typename T>
 int cheb_n(vx_solver* s, int li, const T* u_in, const T* b, int k, T* dst, T* tmp, T* tmp2,
            bool dot, int* n_part, bool first_done = false) {
   const double beta = s->lam_max, alpha = s->opt.cheb_ratio * beta;
   const double theta = 0.5 * (beta + alpha), delta = 0.5 * (beta - alpha);
   const double sigma = theta / delta;
-  T* ring[3] = {dst, tmp, tmp2};
-  auto target = [&](int j) { return ring[((k - 1 - j) % 3 + 3) % 3]; };
+  auto target = [&](int j) { return j == k - 1 ? dst : (j % 2 == 0 ? tmp : tmp2); };
   const T* xprev = u_in;
   const T* xcur = u_in;
   double rho_prev = 1.0 / sigma, rho = rho_prev;
@@ -409,6 +410,17 @@ int alloc_level_vectors(vx_solver* s) {
   return VX_OK;
 }
 
+// smoothing steps (pre and post) of level li
+static int level_ns(const vx_solver* s, int li) {
+  return li < 16 && s->opt.level_smooth[li] ? (int)s->opt.level_smooth[li] : s->opt.n_smooth;
+}
+
+// k smoothing steps from zero leave their first step in tmp (cheb_n for any
+// k >= 2, smooth_n for even k): the step another kernel may write ahead
+static bool first_in_tmp(const vx_solver* s, int k) {
+  return s->opt.smoother == VX_SMOOTH_CHEBYSHEV ? k >= 2 : k >= 2 && k % 2 == 0;
+}
+
 // weight of the first smoothing step from zero, x1 = c D^-1 b: 1/theta for
 // Chebyshev, omega for damped Jacobi
 static double first_step_coef(const vx_solver* s) {
@@ -425,16 +437,17 @@ int smooth_step(vx_solver* s, int li, const T* u_in, const T* b, int k, T* dst, 
                 bool dot, int* n_part, bool first_done = false) {
   const long long n = 3 * s->lv[li].Nn * s->nb;
   if (first_done) {
-    // the first step from zero (k_cg_update_x1) is already in tmp: both
-    // smoothers write their first step of k = 2 there
-    if (u_in || k != 2) return fail(s, VX_ERR_ARG, "precomputed first step needs k = 2 from zero");
+    // the first step from zero (k_cg_update_x1 or the restriction) is
+    // already in tmp, where the smoother writes it
+    if (u_in || !first_in_tmp(s, k))
+      return fail(s, VX_ERR_ARG, "a precomputed first step needs a start from zero and its step in tmp");
     if (s->opt.smoother == VX_SMOOTH_CHEBYSHEV)
       return cheb_n<T>(s, li, u_in, b, k, dst, tmp, tmp2, dot, n_part, true);
     return smooth_n<T>(s, li, u_in, b, k, dst, tmp, dot, n_part, true);
   }
   if (s->opt.smoother == VX_SMOOTH_CHEBYSHEV) {
-    if (u_in == dst && k % 3 == 1) {
-      // the first write would land on the start vector: go through tmp
+    if (u_in == dst && k == 1) {
+      // the only write would land on the start vector: go through tmp
       CHK(cheb_n<T>(s, li, u_in, b, k, tmp, dst, tmp2, dot, n_part));
       return copy_vec<T>(s, dst, tmp, n);
     }
@@ -621,14 +634,13 @@ template <typename T>
 int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, T* w0, bool dot, int* n_part, int l0,
            bool first_done) {
   const int nl = (int)s->lv.size();
-  const int ns = s->opt.n_smooth;
   const int nb = s->nb;
   const bool learned = std::is_same<T, float>::value && s->corr_on && s->corr_level >= l0 &&
                        s->corr_level < nl;
   const int bottom = learned ? s->corr_level : nl - 1;
-  // descend.  With two smoothing steps, the restriction into a level that
-  // smooths from zero (not the coarsest) also writes that level's first
-  // step into its t vector, where the smoother expects it.
+  // descend.  The restriction into a level that smooths from zero (not the
+  // coarsest) also writes that level's first step into its t vector, where
+  // the smoother expects it (first_in_tmp).
   bool restricted_x1 = false;
   for (int li = l0; li <= bottom; ++li) {
     Level& L = s->lv[li];
@@ -637,6 +649,7 @@ int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, T* w0, bool dot, int* n_part
     T* t = li == l0 ? t0 : Sel<T>::t(L);
     T* w = li == l0 ? w0 : Sel<T>::w(L);
     const bool pre_done = li == l0 ? first_done : restricted_x1;
+    const int ns = level_ns(s, li);
     if (li == bottom && learned) {
       CHK(smooth_step<T>(s, li, nullptr, b, ns, x, t, w, false, nullptr, pre_done));
       CHK((launch_tile<T, M_RESID, false>(s, li, x, b, t, nullptr, nullptr)));
@@ -662,7 +675,7 @@ int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, T* w0, bool dot, int* n_part
     CHK((launch_tile<T, M_RESID, false>(s, li, x, b, t, nullptr, nullptr)));
     if (l0 == 0) CHK(capture_residual<T>(s, li, t));
     Level& C = s->lv[li + 1];
-    restricted_x1 = ns == 2 && li + 1 < nl - 1;
+    restricted_x1 = li + 1 < nl - 1 && first_in_tmp(s, level_ns(s, li + 1));
     if (restricted_x1)
       CHK(restrict_to<T>(s, C, t, Sel<T>::b(C), nb, Sel<T>::t(C), first_step_coef(s)));
     else
@@ -677,7 +690,7 @@ int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, T* w0, bool dot, int* n_part
     T* t = li == l0 ? t0 : Sel<T>::t(L);
     T* w = li == l0 ? w0 : Sel<T>::w(L);
     CHK(prolong_add<T>(s, li, Sel<T>::x(C), x));
-    CHK(smooth_step<T>(s, li, x, b, ns, x, t, w, dot && li == l0, n_part));
+    CHK(smooth_step<T>(s, li, x, b, level_ns(s, li), x, t, w, dot && li == l0, n_part));
   }
   return VX_OK;
 }
@@ -755,11 +768,11 @@ int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* Ap, T* t0, double tol_rel, int 
   if (rc == VX_OK && std::any_of(frozen.begin(), frozen.end(), [](int f) { return f; }))
     rc = push_frozen(s, frozen);
   if (rc != VX_OK) { cleanup(); return rc; }
-  // FP32 with two smoothing steps from zero below the coarsest level: the
-  // update kernel also writes the next V-cycle's first step (its output
-  // buffer for k = 2 is t0 for both smoothers)
+  // FP32 with a smoothing pass from zero below the coarsest level whose
+  // first step lands in t0 (first_in_tmp): the update kernel also writes the
+  // next V-cycle's first step there
   const auto al16 = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; };
-  const bool fuse_x1 = std::is_same<T, float>::value && s->opt.n_smooth == 2 &&
+  const bool fuse_x1 = std::is_same<T, float>::value && first_in_tmp(s, level_ns(s, l0)) &&
                        l0 < (int)s->lv.size() - 1 && (nb == 1 || (3 * L0.Nn) % 4 == 0) &&
                        al16(u) && al16(r) && al16(p) && al16(Ap) && al16(t0) && al16(L0.mask) &&
                        al16(L0.mi32);
