@@ -99,6 +99,33 @@ def test_vcycle(dims, kw):
     g.close()
 
 
+@pytest.mark.parametrize("smoother,steps", [("chebyshev", [2, 3, 9]), ("jacobi", [2, 2, 5]),
+                                            ("chebyshev", [1, 1, 1])])
+def test_vcycle_fused_bottom_levels(smoother, steps):
+    """FP32 runs the two bottom levels (here 8x4x4 and 4x2x2 elements) in one
+    single-work-group launch; FP64 runs them launch by launch.  The cycles
+    agree to FP32 rounding, and the Jacobi form also matches the oracle."""
+    print("case fused bottom", smoother, steps, flush=True)
+    from core.multigrid import MGSolver
+    dims = (32, 16, 16)
+    E, fixed, ndof, rng = _problem(*dims, seed=12)
+    kw = dict(max_levels=4, smoother=smoother, level_smooth=steps, cheb_ratio=0.1)
+    if smoother == "jacobi":
+        kw["omega"] = 0.35
+    g = MGSolver(*dims, fixed, nu=0.3, compute_mode="GPU", **kw)
+    g.set_density(E)
+    assert [lv["dims"] for lv in g.levels] == [(32, 16, 16), (16, 8, 8), (8, 4, 4), (4, 2, 2)]
+    o = MGSolverRef(*dims, fixed, nu=0.3, max_levels=4, omega=0.35, level_smooth=steps)
+    o.set_density(E)
+    b = np.where(o.levels[0]["free"], rng.standard_normal(ndof), 0.0)
+    z64 = g._vcycle(0, b)
+    z32 = g._vcycle(0, b, precision="mixed")
+    assert _rel(z32, z64) < 1e-4, _rel(z32, z64)
+    if smoother == "jacobi":
+        assert _rel(z64, o._vcycle(0, b)) < 1e-11
+    g.close()
+
+
 @pytest.mark.parametrize("dims,kw,direct", [
     ((16, 8, 8), {}, True), ((8, 8, 8), {"max_levels": 3}, True),
     ((6, 10, 4), {"n_smooth": 3}, True), ((24, 16, 16), {"max_levels": 3}, True),

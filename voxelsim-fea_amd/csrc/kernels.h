@@ -592,6 +592,238 @@ __global__ __launch_bounds__(1024) void k_coarse_smooth(int nx, int ny, int nz,
   for (int i = tid; i < 3 * Nn; i += nt) out[i] = sx[i];
 }
 
+// ---------------------------------------------------------------------------
+// The two bottom levels of the V-cycle in one single-work-group launch per
+// right-hand side (FP32): on the second-to-last level f (held in LDS), kf
+// smoothing steps from zero, the residual, the full-weighting restriction to
+// the coarsest level c, kc smoothing steps from zero there, the trilinear
+// prolongation back (x += mask * P e) and kf smoothing steps from that x; the
+// result goes to x.  This is the sequence smooth_step / M_RESID /
+// k_restrict_z / k_coarse_smooth / k_prolong_f / smooth_step launches on
+// these levels (core/multigrid.py:426-436), with the same step arithmetic;
+// only the order of the element sums into a node differs.  The levels'
+// launches are latency-bound, and a variable V-cycle gives them tens of steps.
+// ---------------------------------------------------------------------------
+constexpr int BOT_MAXE = 640, BOT_MAXN = 900;     // level f: elements, nodes
+constexpr int BOT_MAXEC = 100, BOT_MAXNC = 160;   // coarsest level
+constexpr int BOT_THREADS = 1024;                 // >= every count above: one
+                                                  // element and one node per thread
+
+struct BottomArgs {
+  int nx, ny, nz;                       // level f elements (even; c has half)
+  const float* E;                       // level f scaled E, x-fastest
+  const uint8_t* mask;
+  const float* minv;
+  const float* b;                       // [nb][3 Nn_f]
+  float* x;                             // [nb][3 Nn_f]
+  const float* Ec;
+  const uint8_t* maskc;
+  const float* minvc;
+  int kf, kc;                           // smoothing steps on f and on c
+  ElemConst<float> K;
+};
+
+__global__ __launch_bounds__(BOT_THREADS) void k_bottom(BottomArgs a, CoarseCoef cc) {
+  __shared__ float sye[BOT_MAXE * 24];
+  __shared__ float sx[3 * BOT_MAXN], sxp[3 * BOT_MAXN], sb[3 * BOT_MAXN];
+  __shared__ float smi[BOT_MAXN];
+  __shared__ float sE[BOT_MAXE];
+  __shared__ uint8_t smk[BOT_MAXN];
+  __shared__ float cx[3 * BOT_MAXNC], cxp[3 * BOT_MAXNC], cb[3 * BOT_MAXNC];
+  __shared__ float cmi[BOT_MAXNC];
+  __shared__ float cE[BOT_MAXEC];
+  __shared__ uint8_t cmk[BOT_MAXNC];
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const int Lx = nx + 1, Ly = ny + 1, Nn = Lx * Ly * (nz + 1), Ne = nx * ny * nz;
+  const int cnx = nx / 2, cny = ny / 2, cnz = nz / 2;
+  const int CLx = cnx + 1, CLy = cny + 1, Nc = CLx * CLy * (cnz + 1), Nec = cnx * cny * cnz;
+  const float* b = a.b + (long long)blockIdx.x * 3 * Nn;
+  float* xo = a.x + (long long)blockIdx.x * 3 * Nn;
+  for (int n = tid; n < Nn; n += nt) {
+    smk[n] = a.mask[n];
+    smi[n] = a.minv[n];
+  }
+  for (int e = tid; e < Ne; e += nt) sE[e] = a.E[e];
+  for (int i = tid; i < 3 * Nn; i += nt) sb[i] = b[i];
+  for (int n = tid; n < Nc; n += nt) {
+    cmk[n] = a.maskc[n];
+    cmi[n] = a.minvc[n];
+  }
+  for (int e = tid; e < Nec; e += nt) cE[e] = a.Ec[e];
+  __syncthreads();
+
+  // Per-thread tables of a level in LDS (at most one element and one node
+  // per thread): the element's 8 corner nodes, and the node's 8 element
+  // slots in sye (-1: no element), in corner order q = x + 2y + 4z.
+  struct Tab {
+    int en[8];
+    int sl[8];
+  };
+  auto make_tab = [&](int ex_, int ey_, int ez_) {
+    Tab t;
+    const int lx = ex_ + 1, ly = ey_ + 1;
+    const int ne = ex_ * ey_ * ez_, nn = lx * ly * (ez_ + 1);
+    const int e = tid < ne ? tid : 0;
+    const int ex = e % ex_, ey = (e / ex_) % ey_, ez = e / (ex_ * ey_);
+    const int n = tid < nn ? tid : 0;
+    const int ix = n % lx, iy = (n / lx) % ly, iz = n / (lx * ly);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      t.en[q] = ((ez + (q >> 2)) * ly + (ey + ((q >> 1) & 1))) * lx + (ex + (q & 1));
+      const int fx = ix - (q & 1), fy = iy - ((q >> 1) & 1), fz = iz - (q >> 2);
+      t.sl[q] = (fx >= 0 && fx < ex_ && fy >= 0 && fy < ey_ && fz >= 0 && fz < ez_)
+                    ? ((fz * ey_ + fy) * ex_ + fx) * 24 + 3 * q
+                    : -1;
+    }
+    return t;
+  };
+  const Tab tf = make_tab(nx, ny, nz);
+  const Tab tc = make_tab(cnx, cny, cnz);
+  // K X: element forces into sye (one slot per element, corner, component)
+  auto elem_pass = [&](const Tab& t, int ne, const float* Es, const float* X) {
+    if (tid < ne) {
+      float u[8][3], y[8][3];
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) u[q][c] = X[3 * t.en[q] + c];
+      elem_apply<float>(u, Es[tid], a.K, y);
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) sye[tid * 24 + 3 * q + c] = y[q][c];
+    }
+  };
+  // node n = tid: the sum of its <= 8 element slots for component c
+  auto node_sum = [&](const Tab& t, int c) -> float {
+    float kx = 0.f;
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (t.sl[q] >= 0) kx += sye[t.sl[q] + c];
+    return kx;
+  };
+  // k smoothing steps on a level in LDS; from zero (x1 = w0 D^-1 b, as
+  // k_zero_step), or from X (first step x + w0 D^-1 r, as M_JACOBI); then
+  // the Chebyshev steps of M_CHEB (or further Jacobi steps)
+  auto smooth = [&](const Tab& t, int ne, int nn, const float* Es, float* X, float* XP,
+                    const float* B, const float* MI, const uint8_t* MK, int k, bool from_zero) {
+    const float w0 = (float)cc.w0;
+    int st = 0;
+    if (from_zero) {
+      if (tid < nn) {
+        const unsigned mk = MK[tid];
+        const float mi = MI[tid];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          X[3 * tid + c] = ((mk >> c) & 1u) ? w0 * mi * B[3 * tid + c] : 0.f;
+          XP[3 * tid + c] = 0.f;
+        }
+      }
+      __syncthreads();
+      st = 1;
+    }
+    for (; st < k; ++st) {
+      elem_pass(t, ne, Es, X);
+      __syncthreads();
+      const bool jac = !cc.cheb || st == 0;
+      const float cu = (float)cc.cu[st], cr = (float)cc.cr[st];
+      if (tid < nn) {
+        const unsigned mk = MK[tid];
+        const float mi = MI[tid];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const int i = 3 * tid + c;
+          const float kx = node_sum(t, c);
+          const float x = X[i];
+          float xn = x;
+          if ((mk >> c) & 1u) {
+            const float r = mi * (B[i] - kx);
+            xn = jac ? fmaT(w0, r, x) : x + fmaT(cu, x - XP[i], cr * r);
+          }
+          XP[i] = x;
+          X[i] = xn;
+        }
+      }
+      __syncthreads();
+    }
+  };
+
+  // pre-smoothing on f
+  smooth(tf, Ne, Nn, sE, sx, sxp, sb, smi, smk, a.kf, true);
+  // residual on f into sxp
+  elem_pass(tf, Ne, sE, sx);
+  __syncthreads();
+  if (tid < Nn) {
+    const unsigned mk = smk[tid];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float kx = node_sum(tf, c);
+      sxp[3 * tid + c] = sb[3 * tid + c] - (((mk >> c) & 1u) ? kx : 0.f);
+    }
+  }
+  __syncthreads();
+  // restriction: taps in (z, y, x) order, weights 1 or 1/2 per axis
+  for (int i = tid; i < 3 * Nc; i += nt) {
+    const int J = i / 3, c = i - 3 * J;
+    const int jx = J % CLx, jy = (J / CLx) % CLy, jz = J / (CLx * CLy);
+    float acc = 0.f;
+    for (int dz = -1; dz <= 1; ++dz) {
+      const int z = 2 * jz + dz;
+      if (z < 0 || z > nz) continue;
+      const float sz = dz == 0 ? 1.f : 0.5f;
+      for (int dy = -1; dy <= 1; ++dy) {
+        const int y = 2 * jy + dy;
+        if (y < 0 || y > ny) continue;
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int x = 2 * jx + dx;
+          if (x < 0 || x > nx) continue;
+          const float wk = sz * ((dx == 0 ? 1.f : 0.5f) * (dy == 0 ? 1.f : 0.5f));
+          acc += wk * sxp[3 * ((z * Ly + y) * Lx + x) + c];
+        }
+      }
+    }
+    cb[i] = ((cmk[J] >> c) & 1u) ? acc : 0.f;
+  }
+  __syncthreads();
+  // the coarsest level: kc steps from zero
+  smooth(tc, Nec, Nc, cE, cx, cxp, cb, cmi, cmk, a.kc, true);
+  // prolongation x += mask * P e (the tap order of k_prolong_f)
+  for (int i = tid; i < 3 * Nn; i += nt) {
+    const int n = i / 3, c = i - 3 * n;
+    if (!((smk[n] >> c) & 1u)) continue;
+    const int ix = n % Lx, iy = (n / Lx) % Ly, iz = n / (Lx * Ly);
+    const int ox = ix & 1, oy = iy & 1, oz = iz & 1;
+    const int t00 = 3 * ((iy >> 1) * CLx + (ix >> 1)) + c;
+    const int t01 = t00 + 3, t10 = t00 + 3 * CLx, t11 = t10 + 3;
+    const int cplane = 3 * CLx * CLy;
+    auto cont = [&](int cz, float s) {
+      const float* P = cx + cz * cplane;
+      s += P[t00];
+      if (ox) s += P[t01];
+      if (oy) {
+        s += P[t10];
+        if (ox) s += P[t11];
+      }
+      return s;
+    };
+    const float* P0 = cx + (iz >> 1) * cplane;
+    float s = P0[t00];
+    if (ox) s += P0[t01];
+    if (oy) {
+      s += P0[t10];
+      if (ox) s += P0[t11];
+    }
+    const float wxy = (ox ? 0.5f : 1.f) * (oy ? 0.5f : 1.f);
+    if (oz) s = cont((iz >> 1) + 1, s);
+    sx[i] += s * (oz ? 0.5f * wxy : wxy);
+  }
+  __syncthreads();
+  // post-smoothing on f from x
+  smooth(tf, Ne, Nn, sE, sx, sxp, sb, smi, smk, a.kf, false);
+  for (int i = tid; i < 3 * Nn; i += nt) xo[i] = sx[i];
+}
+
 // per-node values -> per-DOF (3 copies)
 __global__ void k_expand3(long long Nn, const double* __restrict__ a, double* __restrict__ b) {
   for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < Nn;

@@ -462,12 +462,10 @@ int smooth_step(vx_solver* s, int li, const T* u_in, const T* b, int k, T* dst, 
   return smooth_n<T>(s, li, u_in, b, k, dst, tmp, dot, n_part);
 }
 
-// All coarsest-level smoothing steps in one single-work-group launch per
-// right-hand side (same step sequence as smooth_step from zero).
-template <typename T>
-int coarse_fused(vx_solver* s, Level& L, const T* b, T* x) {
+// the step coefficients of k smoothing steps (k <= COARSE_MAX_STEPS), the
+// sequence cheb_n / smooth_n use on every level
+static CoarseCoef smooth_coef(const vx_solver* s, int k) {
   CoarseCoef cc{};
-  const int k = s->opt.coarse_sweeps;
   cc.steps = k;
   if (s->opt.smoother == VX_SMOOTH_CHEBYSHEV) {
     const double beta = s->lam_max, alpha = s->opt.cheb_ratio * beta;
@@ -486,6 +484,55 @@ int coarse_fused(vx_solver* s, Level& L, const T* b, T* x) {
     cc.cheb = 0;
     cc.w0 = s->opt.omega;
   }
+  return cc;
+}
+
+// The two bottom levels of the cycle in one launch (k_bottom): FP32, the
+// second-to-last level li below the cycle's first level, within the kernel's
+// LDS budget, and neither the exact coarse solve nor a residual capture on li
+template <typename T>
+bool bottom_fusable(const vx_solver* s, int li, int l0) {
+  if (!std::is_same<T, float>::value) return false;
+  const int nl = (int)s->lv.size();
+  if (li != nl - 2 || li <= l0 || s->cd_ok) return false;
+  if (s->cap_buf && s->cap_level == li) return false;
+  const Level& F = s->lv[li];
+  const Level& C = s->lv[li + 1];
+  return F.Ne <= BOT_MAXE && F.Nn <= BOT_MAXN && C.Ne <= BOT_MAXEC && C.Nn <= BOT_MAXNC &&
+         2 * C.nx == F.nx && 2 * C.ny == F.ny && 2 * C.nz == F.nz &&
+         level_ns(s, li) <= COARSE_MAX_STEPS && s->opt.coarse_sweeps <= COARSE_MAX_STEPS;
+}
+
+static int bottom_fused(vx_solver* s, int li, const float* b, float* x) {
+  Level& F = s->lv[li];
+  Level& C = s->lv[li + 1];
+  BottomArgs a;
+  a.nx = F.nx;
+  a.ny = F.ny;
+  a.nz = F.nz;
+  a.E = F.E32;
+  a.mask = F.mask;
+  a.minv = F.mi32;
+  a.b = b;
+  a.x = x;
+  a.Ec = C.E32;
+  a.maskc = C.mask;
+  a.minvc = C.mi32;
+  a.kf = level_ns(s, li);
+  a.kc = s->opt.coarse_sweeps;
+  a.K = kconst<float>(s);
+  const CoarseCoef cc = smooth_coef(s, std::max(a.kf, a.kc));
+  hipLaunchKernelGGL(k_bottom, dim3(s->nb), dim3(BOT_THREADS), 0, s->stream, a, cc);
+  LAUNCHCHK();
+  return VX_OK;
+}
+
+// All coarsest-level smoothing steps in one single-work-group launch per
+// right-hand side (same step sequence as smooth_step from zero).
+template <typename T>
+int coarse_fused(vx_solver* s, Level& L, const T* b, T* x) {
+  const int k = s->opt.coarse_sweeps;
+  const CoarseCoef cc = smooth_coef(s, k);
   // as many threads as the larger of the element and DOF loops needs (whole
   // waves, at most 1024): fewer waves make each step's two barriers cheaper
   const long long work = std::max<long long>(L.Ne, 3 * L.Nn);
@@ -638,6 +685,8 @@ int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, T* w0, bool dot, int* n_part
   const bool learned = std::is_same<T, float>::value && s->corr_on && s->corr_level >= l0 &&
                        s->corr_level < nl;
   const int bottom = learned ? s->corr_level : nl - 1;
+  // the two bottom levels in one launch (k_bottom) from level fuse_li
+  const int fuse_li = !learned && bottom_fusable<T>(s, nl - 2, l0) ? nl - 2 : -1;
   // descend.  The restriction into a level that smooths from zero (not the
   // coarsest) also writes that level's first step into its t vector, where
   // the smoother expects it (first_in_tmp).
@@ -658,6 +707,10 @@ int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, T* w0, bool dot, int* n_part
       CHK(smooth_step<T>(s, li, x, b, ns, x, t, w, dot && li == l0, n_part));
       break;
     }
+    if (li == fuse_li) {
+      CHK(bottom_fused(s, li, reinterpret_cast<const float*>(b), reinterpret_cast<float*>(x)));
+      break;
+    }
     if (li == nl - 1) {
       const int k = s->opt.coarse_sweeps;
       if (s->cd_ok && li > 0 && !(dot && li == l0)) {
@@ -675,14 +728,14 @@ int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, T* w0, bool dot, int* n_part
     CHK((launch_tile<T, M_RESID, false>(s, li, x, b, t, nullptr, nullptr)));
     if (l0 == 0) CHK(capture_residual<T>(s, li, t));
     Level& C = s->lv[li + 1];
-    restricted_x1 = li + 1 < nl - 1 && first_in_tmp(s, level_ns(s, li + 1));
+    restricted_x1 = li + 1 < nl - 1 && li + 1 != fuse_li && first_in_tmp(s, level_ns(s, li + 1));
     if (restricted_x1)
       CHK(restrict_to<T>(s, C, t, Sel<T>::b(C), nb, Sel<T>::t(C), first_step_coef(s)));
     else
       CHK(restrict_to<T>(s, C, t, Sel<T>::b(C), nb));
   }
-  // ascend
-  for (int li = bottom - 1; li >= l0; --li) {
+  // ascend (from below the fused levels' result when k_bottom ran)
+  for (int li = (fuse_li >= 0 ? fuse_li : bottom) - 1; li >= l0; --li) {
     Level& L = s->lv[li];
     Level& C = s->lv[li + 1];
     const T* b = li == l0 ? b0 : Sel<T>::b(L);
