@@ -577,8 +577,9 @@ def main():
         "speedup_vs_plain_fp64": (plaininfo["time_s"] / t_opt
                                   if plaininfo.get("converged") else None),
         "fp64_mgpcg_chebyshev": fp64,
-        "value_solver": "mixed FP32/FP64 Chebyshev MG-PCG on the active-DOF system; the "
-                        "learned corrector is measured beside it ('learned')",
+        "value_solver": "mixed FP32/FP64 Chebyshev MG-PCG (variable V-cycle) on the active-DOF "
+                        "system; the learned corrector is measured beside it ('learned') and "
+                        "is slower on this part",
         "mixed": {"iters": mixed_iters, "outer": mixed_outer,
                   "true_resid": mixed_true, "converged": mixed_conv,
                   "rel_disp_diff_vs_fp64": rel_err,
