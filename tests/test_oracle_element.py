@@ -182,6 +182,32 @@ def test_library_host_element_operator(nu):
     assert np.abs(Y - KE).max() < 1e-15 * np.abs(KE).max() * 8
 
 
+@pytest.mark.parametrize("struct", ["vx_options", "vx_info"])
+def test_ctypes_structs_match_the_header(struct, tmp_path):
+    """The ctypes mirrors have the C header's size and field offsets (gcc on
+    include/vxfea.h)."""
+    import os
+    import shutil
+    import subprocess
+    from core import _native
+    if shutil.which("gcc") is None:
+        pytest.skip("no gcc")
+    cls = getattr(_native, struct)
+    names = [f[0] for f in cls._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text("#include <stdio.h>\n#include <stddef.h>\n#include \"vxfea.h\"\nint main(void) {\n"
+                   + f'  printf("%zu\\n", sizeof({struct}));\n'
+                   + "".join(f'  printf("%zu\\n", offsetof({struct}, {n}));\n' for n in names)
+                   + "  return 0;\n}\n")
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", inc, str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True,
+                                          text=True).stdout.split()]
+    assert got[0] == ctypes.sizeof(cls)
+    assert got[1:] == [getattr(cls, n).offset for n in names]
+
+
 def test_default_options_are_reference_defaults():
     from core import _native
     o = _native.default_options()
