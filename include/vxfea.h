@@ -81,7 +81,9 @@ typedef struct {
                                  operators; those levels' launches are latency-bound, so their
                                  extra steps cost little.  The coarsest level uses
                                  coarse_sweeps. */
-  int32_t reserved[1];
+  int32_t eager;              /* 0: each PCG step's V-cycle is replayed from a captured HIP
+                                 graph (one submission instead of one per kernel; same
+                                 kernels, same results); 1: launched kernel by kernel */
 } vx_options;
 
 /* Per-solve statistics (mirrors MGSolver.last_* of core/multigrid.py:482-517). */

@@ -326,6 +326,43 @@ def test_variable_vcycle_same_solution(precision):
         m.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graph_replayed_cycle_equals_eager_launches(precision):
+    """Each PCG step's V-cycle replayed from a captured HIP graph gives the
+    bitwise result of launching it kernel by kernel: three solves in a row
+    (the graph is replayed every step, between the step's own direct
+    launches and host synchronisations), a batch, and a new density."""
+    print("case graph vs eager", precision, flush=True)
+    import torch
+    from core import workloads as W
+    from core.multigrid import MGSolver
+    p = W.two_material(n=32)
+    E = torch.tensor(p.E_stiffness, device="cuda")
+    A = torch.tensor(p.active.astype(np.uint8), device="cuda")
+    f = torch.tensor(p.force, device="cuda")
+    kw = dict(smoother="chebyshev", cheb_ratio=0.05, system="active", max_levels=4,
+              precision=precision, level_smooth=[2, 2, 6, 12], max_batch=2)
+    out = {}
+    for graphs in (True, False):
+        mg = MGSolver(p.nx, p.ny, p.nz, p.fixed_dofs, nu=p.nu, compute_mode="GPU",
+                      graphs=graphs, **kw)
+        res = []
+        for scale in (1.0, 1.0, 0.5):
+            mg.set_density(E * scale, active=A)
+            u = torch.empty_like(f)
+            mg.solve_device(f, u, tol=1e-7)
+            res.append((u.cpu().numpy(), mg.last_iters))
+        F = torch.stack([f, 0.5 * f])
+        U = torch.empty_like(F)
+        mg.solve_batch_device(F, U, tol=1e-7)
+        res.append((U.cpu().numpy(), tuple(b["iters"] for b in mg.last_batch)))
+        out[graphs] = res
+        mg.close()
+    for (ug, ig), (ue, ie) in zip(out[True], out[False]):
+        assert ig == ie
+        assert np.array_equal(ug, ue)
+
+
 def test_autotuned_solver_same_solution():
     """The z-chunk autotuner changes only the tiling: the solve is the same."""
     import torch

@@ -53,7 +53,7 @@ class vx_options(ctypes.Structure):
         ("max_batch", ctypes.c_int32),
         ("flexible", ctypes.c_int32),
         ("level_smooth", ctypes.c_uint8 * 16),
-        ("reserved", ctypes.c_int32 * 1),
+        ("eager", ctypes.c_int32),
     ]
 
 

@@ -75,14 +75,16 @@ class MGSolver:
     """Multigrid-preconditioned CG on the full voxel grid (HIP).
 
     ``level_smooth`` gives the smoothing steps per level (pre and post, level 0
-    first); the reference uses ``n_smooth`` on every level."""
+    first); the reference uses ``n_smooth`` on every level.  ``graphs=False``
+    launches each PCG step's V-cycle kernel by kernel instead of replaying it
+    from a captured HIP graph (same kernels, same results)."""
 
     def __init__(self, nx, ny, nz, fixed_dofs, nu=0.3, xp=None, n_smooth=2, omega=0.6,
                  min_elems=4, max_levels=6, compute_mode="AUTO", cpu_threads=0,
                  verbose=False, precision="fp64", coarse_sweeps=30, inner_tol=1e-5,
                  max_outer=20, z_chunk=0, smoother="jacobi", cheb_ratio=0.3,
                  system="full", e_coarsen="mean", max_batch=1, flexible=False, pad=False,
-                 autotune=False, coarse="sweeps", level_smooth=None):
+                 autotune=False, coarse="sweeps", level_smooth=None, graphs=True):
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
         self.nelem = self.nx * self.ny * self.nz
         self.ndof = 3 * (self.nx + 1) * (self.ny + 1) * (self.nz + 1)
@@ -113,7 +115,7 @@ class MGSolver:
             max_batch=max_batch, flexible=1 if flexible else 0,
             coarse_mode={"sweeps": _native.VX_COARSE_SWEEPS,
                          "direct": _native.VX_COARSE_DIRECT}[coarse],
-            level_smooth=_level_smooth(level_smooth))
+            level_smooth=_level_smooth(level_smooth), eager=0 if graphs else 1)
         self.max_batch = max_batch
         h = ctypes.c_void_p()
         px, py, pz = self._pdims

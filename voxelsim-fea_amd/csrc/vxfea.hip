@@ -122,6 +122,17 @@ struct vx_solver {
   float* cap_buf = nullptr;
   int cap_level = -1, cap_cap = 0, cap_count = 0, cap_every = 1;
   long long cap_calls = 0;
+  // V-cycles captured as HIP graphs (one per key; cleared whenever the
+  // launches they hold could change) and the stream they are captured on,
+  // which never runs work
+  struct CycleGraph {
+    const void* ptr[4];
+    int type, l0, nb, first_done, flexible;
+    hipGraphExec_t exec;
+  };
+  std::vector<CycleGraph> graphs;
+  hipStream_t cap_stream = nullptr;
+  bool graphs_failed = false;
   std::vector<void*> allocs;
 };
 
@@ -130,6 +141,13 @@ namespace {
 int fail(vx_solver* s, int code, const std::string& msg) {
   if (s) s->err = msg;
   return code;
+}
+
+// the captured V-cycles hold launch configurations (z-chunks, the coarse
+// mode) and buffers that set-up calls may change
+void drop_graphs(vx_solver* s) {
+  for (auto& g : s->graphs) (void)hipGraphExecDestroy(g.exec);
+  s->graphs.clear();
 }
 
 #define HIPCHK(call)                                                                   \
@@ -779,6 +797,67 @@ int push_frozen(vx_solver* s, const std::vector<int>& fr) {
   return VX_OK;
 }
 
+// A V-cycle holds only kernel launches unless a copy, a corrector or a
+// residual capture is in it: then it cannot be replayed from a graph
+bool cycle_graphable(const vx_solver* s) {
+  if (s->opt.eager || s->graphs_failed || s->corr_on || s->cap_buf) return false;
+  for (int li = 0; li < (int)s->lv.size(); ++li) {
+    const int k = level_ns(s, li);
+    if (s->opt.smoother == VX_SMOOTH_CHEBYSHEV ? k == 1 : k % 2 == 1) return false;   // copy_vec
+  }
+  return true;
+}
+
+// Run `enqueue` (kernel launches on s->stream) from a HIP graph captured
+// once per key on a stream that only records, and replayed on s->stream (the
+// null stream included): one submission instead of one per kernel.  If
+// capture or instantiation fails the launches run directly, from then on.
+template <typename F>
+int run_graphed(vx_solver* s, const void* const (&ptr)[4], int type, int l0, int first_done,
+                int flexible, F&& enqueue) {
+  if (!cycle_graphable(s)) return enqueue();
+  for (const auto& g : s->graphs)
+    if (g.type == type && g.l0 == l0 && g.nb == s->nb && g.first_done == first_done &&
+        g.flexible == flexible && std::equal(ptr, ptr + 4, g.ptr)) {
+      HIPCHK(hipGraphLaunch(g.exec, s->stream));
+      return VX_OK;
+    }
+  if (!s->cap_stream && hipStreamCreateWithFlags(&s->cap_stream, hipStreamNonBlocking) != hipSuccess) {
+    (void)hipGetLastError();
+    s->graphs_failed = true;
+    return enqueue();
+  }
+  hipStream_t run = s->stream;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool ok = hipStreamBeginCapture(s->cap_stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+  int rc = VX_OK;
+  if (ok) {
+    s->stream = s->cap_stream;
+    rc = enqueue();
+    s->stream = run;
+    ok = hipStreamEndCapture(s->cap_stream, &graph) == hipSuccess && rc == VX_OK && graph;
+  }
+  if (ok) ok = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+  if (graph) (void)hipGraphDestroy(graph);
+  if (!ok) {
+    (void)hipGetLastError();
+    s->graphs_failed = true;
+    return enqueue();
+  }
+  vx_solver::CycleGraph g{};
+  std::copy(ptr, ptr + 4, g.ptr);
+  g.type = type;
+  g.l0 = l0;
+  g.nb = s->nb;
+  g.first_done = first_done;
+  g.flexible = flexible;
+  g.exec = exec;
+  s->graphs.push_back(g);
+  HIPCHK(hipGraphLaunch(exec, s->stream));
+  return VX_OK;
+}
+
 struct PcgResult {
   int iters = 0;
   bool converged = false;
@@ -879,13 +958,20 @@ int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* Ap, T* t0, double tol_rel, int 
         d = prev[q] * (prev[q] / prev2[q]) < 3.0 * tol_rel;
       predicted_done = predicted_done && d;
     }
-    auto queue_next = [&]() -> int {
+    // the V-cycle and its beta: the same launches every step, replayed from
+    // a captured graph where the cycle allows it
+    auto cycle_and_beta = [&]() -> int {
       int np2 = 0;
       CHK(vcycle<T>(s, r, z, t0, Sel<T>::w(L0), true, &np2, l0, fuse_x1));
       if (!flexible) return finalize(s, np2, F_BETA);
       CHK(finalize(s, np2, F_RZNEW));
       CHK(dot_partials<T>(s, n, z, Ap, &np2));
       return finalize(s, np2, F_BETA_FLEX);
+    };
+    auto queue_next = [&]() -> int {
+      const void* const key[4] = {r, z, t0, Ap};
+      return run_graphed(s, key, std::is_same<T, float>::value ? 1 : 0, l0, fuse_x1 ? 1 : 0,
+                         flexible ? 1 : 0, cycle_and_beta);
     };
     bool queued = false;
     if (!predicted_done) {
@@ -1262,6 +1348,8 @@ void vx_destroy(vx_handle s) {
   if (!s) return;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   else (void)hipDeviceSynchronize();
+  drop_graphs(s);
+  if (s->cap_stream) (void)hipStreamDestroy(s->cap_stream);
   for (void* p : s->allocs) (void)hipFree(p);
   if (s->hscal) (void)hipHostFree(s->hscal);
   if (s->hflags) (void)hipHostFree(s->hflags);
@@ -1341,6 +1429,7 @@ int vx_set_density(vx_handle s, const double* E, const uint8_t* active, double e
     LAUNCHCHK();
   }
   s->density_set = true;
+  drop_graphs(s);   // the coarse mode may have changed
   s->cd_ok = false;
   if (s->opt.coarse_mode == VX_COARSE_DIRECT) CHK(build_coarse_direct(s));
   if (s->corr) return corr_features(s, s->corr_level);
@@ -2018,6 +2107,7 @@ extern "C" int vx_autotune(vx_handle s, int32_t precision, int32_t reps) {
   if (!s) return VX_ERR_ARG;
   CHK(require_ready(s));
   if (reps < 1) reps = 5;
+  drop_graphs(s);   // the z-chunks change
   const int pi = precision == VX_FP64 ? 0 : 1;
   if (pi == 0) CHK(alloc_pcg64(s));
   else CHK(alloc_pcg32(s));
