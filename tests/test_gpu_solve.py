@@ -326,13 +326,13 @@ def test_variable_vcycle_same_solution(precision):
         m.close()
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_graph_replayed_cycle_equals_eager_launches(precision):
+@pytest.mark.parametrize("precision,flexible", [("fp64", False), ("mixed", False), ("mixed", True)])
+def test_graph_replayed_cycle_equals_eager_launches(precision, flexible):
     """Each PCG step's V-cycle replayed from a captured HIP graph gives the
     bitwise result of launching it kernel by kernel: three solves in a row
     (the graph is replayed every step, between the step's own direct
     launches and host synchronisations), a batch, and a new density."""
-    print("case graph vs eager", precision, flush=True)
+    print("case graph vs eager", precision, flexible, flush=True)
     import torch
     from core import workloads as W
     from core.multigrid import MGSolver
@@ -341,7 +341,7 @@ def test_graph_replayed_cycle_equals_eager_launches(precision):
     A = torch.tensor(p.active.astype(np.uint8), device="cuda")
     f = torch.tensor(p.force, device="cuda")
     kw = dict(smoother="chebyshev", cheb_ratio=0.05, system="active", max_levels=4,
-              precision=precision, level_smooth=[2, 2, 6, 12], max_batch=2)
+              precision=precision, level_smooth=[2, 2, 6, 12], max_batch=2, flexible=flexible)
     out = {}
     for graphs in (True, False):
         mg = MGSolver(p.nx, p.ny, p.nz, p.fixed_dofs, nu=p.nu, compute_mode="GPU",
