@@ -214,3 +214,19 @@ def test_default_options_are_reference_defaults():
     assert (o.max_levels, o.min_elems, o.n_smooth, o.coarse_sweeps) == (6, 4, 2, 30)
     assert o.omega == 0.6 and o.precision == _native.VX_FP64
     assert o.struct_size == ctypes.sizeof(_native.vx_options)
+
+
+def test_level_smooth_option_packing():
+    """Per-level smoothing counts reach the C options as 16 bytes (0: n_smooth);
+    more than 16 levels or a count outside 0-255 is an error before any GPU
+    call."""
+    from core import _native
+    from core.multigrid import _level_smooth
+    o = _native.default_options(level_smooth=_level_smooth([2, 2, 6, 12, 24, 24]))
+    assert list(o.level_smooth) == [2, 2, 6, 12, 24, 24] + [0] * 10
+    assert list(_native.default_options().level_smooth) == [0] * 16
+    assert _native.default_options().eager == 0
+    assert _level_smooth(None) is None
+    for bad in ([1] * 17, [256], [-1]):
+        with pytest.raises(ValueError):
+            _level_smooth(bad)
