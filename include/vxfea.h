@@ -62,8 +62,11 @@ typedef struct {
   int32_t precision;          /* VX_FP64 | VX_MIXED */
   int32_t smoother;           /* VX_SMOOTH_* */
   int32_t coarse_mode;        /* VX_COARSE_* */
-  int32_t cheb_degree;        /* unused: the Chebyshev polynomial degree of a smoothing pass is
-                                 that level's step count (n_smooth / level_smooth) */
+  int32_t cheb_ratio_power;   /* p: a Chebyshev pass of k steps smooths over
+                                 [cheb_ratio (n_smooth / k)^p lambda_max, lambda_max]; 0 (the
+                                 default): one interval for every pass.  A degree-k polynomial
+                                 resolves a spectrum about k^2 times wider, so p = 2 lets the
+                                 long passes of a variable V-cycle reach lower modes */
   double cheb_ratio;          /* lower end of the Chebyshev interval / lambda_max */
   double inner_tol;           /* mixed: relative tolerance of each FP32 inner solve */
   int32_t max_outer;          /* mixed: max iterative-refinement steps */

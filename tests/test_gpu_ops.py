@@ -99,17 +99,20 @@ def test_vcycle(dims, kw):
     g.close()
 
 
-@pytest.mark.parametrize("smoother,steps", [("chebyshev", [2, 3, 9]), ("jacobi", [2, 2, 5]),
-                                            ("chebyshev", [1, 1, 1])])
-def test_vcycle_fused_bottom_levels(smoother, steps):
+@pytest.mark.parametrize("smoother,steps,power", [("chebyshev", [2, 3, 9], 0),
+                                                  ("jacobi", [2, 2, 5], 0),
+                                                  ("chebyshev", [1, 1, 1], 0),
+                                                  ("chebyshev", [2, 3, 9], 2)])
+def test_vcycle_fused_bottom_levels(smoother, steps, power):
     """FP32 runs the two bottom levels (here 8x4x4 and 4x2x2 elements) in one
     single-work-group launch; FP64 runs them launch by launch.  The cycles
     agree to FP32 rounding, and the Jacobi form also matches the oracle."""
-    print("case fused bottom", smoother, steps, flush=True)
+    print("case fused bottom", smoother, steps, power, flush=True)
     from core.multigrid import MGSolver
     dims = (32, 16, 16)
     E, fixed, ndof, rng = _problem(*dims, seed=12)
-    kw = dict(max_levels=4, smoother=smoother, level_smooth=steps, cheb_ratio=0.1)
+    kw = dict(max_levels=4, smoother=smoother, level_smooth=steps, cheb_ratio=0.1,
+              cheb_ratio_power=power)
     if smoother == "jacobi":
         kw["omega"] = 0.35
     g = MGSolver(*dims, fixed, nu=0.3, compute_mode="GPU", **kw)
@@ -169,16 +172,19 @@ def test_vcycle_direct_coarse_singular_keeps_sweeps():
     g.close()
 
 
-@pytest.mark.parametrize("k", [1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("k,power", [(1, 0), (2, 0), (3, 0), (4, 0), (5, 0), (6, 0), (3, 2),
+                                     (6, 2), (9, 1)])
 @pytest.mark.parametrize("start", ["zero", "u"])
-def test_chebyshev_smoother_closed_form(k, start):
+def test_chebyshev_smoother_closed_form(k, power, start):
     """k Chebyshev steps on D^-1 A over [ratio*lam, lam] leave the error
     P_k(D^-1 A) e0 with P_k(x) = T_k((theta - x)/delta) / T_k(theta/delta):
-    checked against that matrix polynomial built densely on a tiny grid."""
+    checked against that matrix polynomial built densely on a tiny grid.  With
+    cheb_ratio_power p the interval of a k-step pass starts at
+    ratio * (n_smooth / k)^p * lam."""
     from core.multigrid import MGSolver
     from oracle.fea_ref import hex8_KE
     from oracle.multigrid_ref import MGSolverRef
-    print(f"chebyshev closed form k={k} start={start}", flush=True)
+    print(f"chebyshev closed form k={k} power={power} start={start}", flush=True)
     rng = np.random.default_rng(10 + k)
     nx, ny, nz = 3, 2, 3
     fixed = np.arange(3 * (nx + 1) * (ny + 1))          # z = 0 plane
@@ -194,7 +200,7 @@ def test_chebyshev_smoother_closed_form(k, start):
     Dinv = 1.0 / np.diag(Af)
     KE = hex8_KE(1.0, 0.3)
     lam = np.linalg.eigvalsh(KE).max() / KE[0, 0]
-    a_, b_ = ratio * lam, lam
+    a_, b_ = ratio * (2.0 / k) ** power * lam, lam
     theta, delta = 0.5 * (b_ + a_), 0.5 * (b_ - a_)
     M = Dinv[:, None] * Af
     Y = (theta * np.eye(len(f)) - M) / delta
@@ -211,7 +217,7 @@ def test_chebyshev_smoother_closed_form(k, start):
     want = np.zeros(n)
     want[f] = xs - P @ (xs - u0[f])
     mg = MGSolver(nx, ny, nz, fixed, compute_mode="GPU", smoother="chebyshev", cheb_ratio=ratio,
-                  max_levels=1)
+                  max_levels=1, cheb_ratio_power=power)
     mg.set_density(E)
     got = mg._smooth(0, None if start == "zero" else u0, b, k, "fp64")
     assert np.abs(got - want).max() <= 1e-10 * np.abs(want).max()

@@ -44,7 +44,7 @@ class vx_options(ctypes.Structure):
         ("precision", ctypes.c_int32),
         ("smoother", ctypes.c_int32),
         ("coarse_mode", ctypes.c_int32),
-        ("cheb_degree", ctypes.c_int32),
+        ("cheb_ratio_power", ctypes.c_int32),
         ("cheb_ratio", ctypes.c_double),
         ("inner_tol", ctypes.c_double),
         ("max_outer", ctypes.c_int32),

@@ -75,7 +75,9 @@ class MGSolver:
     """Multigrid-preconditioned CG on the full voxel grid (HIP).
 
     ``level_smooth`` gives the smoothing steps per level (pre and post, level 0
-    first); the reference uses ``n_smooth`` on every level.  ``graphs=False``
+    first); the reference uses ``n_smooth`` on every level.  With
+    ``cheb_ratio_power=p`` a Chebyshev pass of k steps uses the interval
+    [cheb_ratio * (n_smooth / k)**p, 1] * lambda_max.  ``graphs=False``
     launches each PCG step's V-cycle kernel by kernel instead of replaying it
     from a captured HIP graph (same kernels, same results)."""
 
@@ -84,7 +86,8 @@ class MGSolver:
                  verbose=False, precision="fp64", coarse_sweeps=30, inner_tol=1e-5,
                  max_outer=20, z_chunk=0, smoother="jacobi", cheb_ratio=0.3,
                  system="full", e_coarsen="mean", max_batch=1, flexible=False, pad=False,
-                 autotune=False, coarse="sweeps", level_smooth=None, graphs=True):
+                 autotune=False, coarse="sweeps", level_smooth=None, graphs=True,
+                 cheb_ratio_power=0):
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
         self.nelem = self.nx * self.ny * self.nz
         self.ndof = 3 * (self.nx + 1) * (self.ny + 1) * (self.nz + 1)
@@ -115,7 +118,8 @@ class MGSolver:
             max_batch=max_batch, flexible=1 if flexible else 0,
             coarse_mode={"sweeps": _native.VX_COARSE_SWEEPS,
                          "direct": _native.VX_COARSE_DIRECT}[coarse],
-            level_smooth=_level_smooth(level_smooth), eager=0 if graphs else 1)
+            level_smooth=_level_smooth(level_smooth), eager=0 if graphs else 1,
+            cheb_ratio_power=int(cheb_ratio_power))
         self.max_batch = max_batch
         h = ctypes.c_void_p()
         px, py, pz = self._pdims

@@ -600,7 +600,8 @@ __global__ __launch_bounds__(1024) void k_coarse_smooth(int nx, int ny, int nz,
 // prolongation back (x += mask * P e) and kf smoothing steps from that x; the
 // result goes to x.  This is the sequence smooth_step / M_RESID /
 // k_restrict_z / k_coarse_smooth / k_prolong_f / smooth_step launches on
-// these levels (core/multigrid.py:426-436), with the same step arithmetic;
+// these levels (core/multigrid.py:426-436), with the same step arithmetic
+// (cf: the step coefficients of level f, cc: of the coarsest level);
 // only the order of the element sums into a node differs.  The levels'
 // launches are latency-bound, and a variable V-cycle gives them tens of steps.
 // ---------------------------------------------------------------------------
@@ -623,7 +624,7 @@ struct BottomArgs {
   ElemConst<float> K;
 };
 
-__global__ __launch_bounds__(BOT_THREADS) void k_bottom(BottomArgs a, CoarseCoef cc) {
+__global__ __launch_bounds__(BOT_THREADS) void k_bottom(BottomArgs a, CoarseCoef cf, CoarseCoef cc) {
   __shared__ float sye[BOT_MAXE * 24];
   __shared__ float sx[3 * BOT_MAXN], sxp[3 * BOT_MAXN], sb[3 * BOT_MAXN];
   __shared__ float smi[BOT_MAXN];
@@ -706,9 +707,10 @@ __global__ __launch_bounds__(BOT_THREADS) void k_bottom(BottomArgs a, CoarseCoef
   // k smoothing steps on a level in LDS; from zero (x1 = w0 D^-1 b, as
   // k_zero_step), or from X (first step x + w0 D^-1 r, as M_JACOBI); then
   // the Chebyshev steps of M_CHEB (or further Jacobi steps)
-  auto smooth = [&](const Tab& t, int ne, int nn, const float* Es, float* X, float* XP,
-                    const float* B, const float* MI, const uint8_t* MK, int k, bool from_zero) {
-    const float w0 = (float)cc.w0;
+  auto smooth = [&](const CoarseCoef& q, const Tab& t, int ne, int nn, const float* Es, float* X,
+                    float* XP, const float* B, const float* MI, const uint8_t* MK, int k,
+                    bool from_zero) {
+    const float w0 = (float)q.w0;
     int st = 0;
     if (from_zero) {
       if (tid < nn) {
@@ -726,8 +728,8 @@ __global__ __launch_bounds__(BOT_THREADS) void k_bottom(BottomArgs a, CoarseCoef
     for (; st < k; ++st) {
       elem_pass(t, ne, Es, X);
       __syncthreads();
-      const bool jac = !cc.cheb || st == 0;
-      const float cu = (float)cc.cu[st], cr = (float)cc.cr[st];
+      const bool jac = !q.cheb || st == 0;
+      const float cu = (float)q.cu[st], cr = (float)q.cr[st];
       if (tid < nn) {
         const unsigned mk = MK[tid];
         const float mi = MI[tid];
@@ -750,7 +752,7 @@ __global__ __launch_bounds__(BOT_THREADS) void k_bottom(BottomArgs a, CoarseCoef
   };
 
   // pre-smoothing on f
-  smooth(tf, Ne, Nn, sE, sx, sxp, sb, smi, smk, a.kf, true);
+  smooth(cf, tf, Ne, Nn, sE, sx, sxp, sb, smi, smk, a.kf, true);
   // residual on f into sxp
   elem_pass(tf, Ne, sE, sx);
   __syncthreads();
@@ -787,7 +789,7 @@ __global__ __launch_bounds__(BOT_THREADS) void k_bottom(BottomArgs a, CoarseCoef
   }
   __syncthreads();
   // the coarsest level: kc steps from zero
-  smooth(tc, Nec, Nc, cE, cx, cxp, cb, cmi, cmk, a.kc, true);
+  smooth(cc, tc, Nec, Nc, cE, cx, cxp, cb, cmi, cmk, a.kc, true);
   // prolongation x += mask * P e (the tap order of k_prolong_f)
   for (int i = tid; i < 3 * Nn; i += nt) {
     const int n = i / 3, c = i - 3 * n;
@@ -820,7 +822,7 @@ __global__ __launch_bounds__(BOT_THREADS) void k_bottom(BottomArgs a, CoarseCoef
   }
   __syncthreads();
   // post-smoothing on f from x
-  smooth(tf, Ne, Nn, sE, sx, sxp, sb, smi, smk, a.kf, false);
+  smooth(cf, tf, Ne, Nn, sE, sx, sxp, sb, smi, smk, a.kf, false);
   for (int i = tid; i < 3 * Nn; i += nt) xo[i] = sx[i];
 }
 

@@ -345,6 +345,13 @@ int sweep(vx_solver* s, int li, const T* u, const T* b, T* out, bool dot, int* n
   return launch_tile<T, M_JACOBI, false>(s, li, u, b, out, nullptr, nullptr, &sc);
 }
 
+// lower end of the Chebyshev interval (over lambda_max) of a k-step pass
+static double cheb_ratio_for(const vx_solver* s, int k) {
+  const int p = s->opt.cheb_ratio_power;
+  if (p <= 0 || k <= 0) return s->opt.cheb_ratio;
+  return s->opt.cheb_ratio * std::pow((double)s->opt.n_smooth / (double)k, (double)p);
+}
+
 // Chebyshev smoothing (degree k) on D^-1 A over [ratio*lmax, lmax]:
 // x1 = x0 + D^-1 r0 / theta, x_{j+1} = x_j + rho_j rho_{j-1} (x_j - x_{j-1})
 // + (2 rho_j / delta) D^-1 r_j.  u_in == nullptr starts from zero.  The
@@ -354,7 +361,7 @@ int sweep(vx_solver* s, int li, const T* u, const T* b, T* out, bool dot, int* n
 template <typename T>
 int cheb_n(vx_solver* s, int li, const T* u_in, const T* b, int k, T* dst, T* tmp, T* tmp2,
            bool dot, int* n_part, bool first_done = false) {
-  const double beta = s->lam_max, alpha = s->opt.cheb_ratio * beta;
+  const double beta = s->lam_max, alpha = cheb_ratio_for(s, k) * beta;
   const double theta = 0.5 * (beta + alpha), delta = 0.5 * (beta - alpha);
   const double sigma = theta / delta;
   auto target = [&](int j) { return j == k - 1 ? dst : (j % 2 == 0 ? tmp : tmp2); };
@@ -439,11 +446,11 @@ static bool first_in_tmp(const vx_solver* s, int k) {
   return s->opt.smoother == VX_SMOOTH_CHEBYSHEV ? k >= 2 : k >= 2 && k % 2 == 0;
 }
 
-// weight of the first smoothing step from zero, x1 = c D^-1 b: 1/theta for
-// Chebyshev, omega for damped Jacobi
-static double first_step_coef(const vx_solver* s) {
+// weight of the first smoothing step from zero of a k-step pass, x1 = c D^-1 b:
+// 1/theta for Chebyshev, omega for damped Jacobi
+static double first_step_coef(const vx_solver* s, int k) {
   if (s->opt.smoother == VX_SMOOTH_CHEBYSHEV) {
-    const double beta = s->lam_max, alpha = s->opt.cheb_ratio * beta;
+    const double beta = s->lam_max, alpha = cheb_ratio_for(s, k) * beta;
     return 1.0 / (0.5 * (beta + alpha));
   }
   return s->opt.omega;
@@ -486,7 +493,7 @@ static CoarseCoef smooth_coef(const vx_solver* s, int k) {
   CoarseCoef cc{};
   cc.steps = k;
   if (s->opt.smoother == VX_SMOOTH_CHEBYSHEV) {
-    const double beta = s->lam_max, alpha = s->opt.cheb_ratio * beta;
+    const double beta = s->lam_max, alpha = cheb_ratio_for(s, k) * beta;
     const double theta = 0.5 * (beta + alpha), delta = 0.5 * (beta - alpha);
     const double sigma = theta / delta;
     cc.cheb = 1;
@@ -539,8 +546,8 @@ static int bottom_fused(vx_solver* s, int li, const float* b, float* x) {
   a.kf = level_ns(s, li);
   a.kc = s->opt.coarse_sweeps;
   a.K = kconst<float>(s);
-  const CoarseCoef cc = smooth_coef(s, std::max(a.kf, a.kc));
-  hipLaunchKernelGGL(k_bottom, dim3(s->nb), dim3(BOT_THREADS), 0, s->stream, a, cc);
+  const CoarseCoef cf = smooth_coef(s, a.kf), cc = smooth_coef(s, a.kc);
+  hipLaunchKernelGGL(k_bottom, dim3(s->nb), dim3(BOT_THREADS), 0, s->stream, a, cf, cc);
   LAUNCHCHK();
   return VX_OK;
 }
@@ -748,7 +755,8 @@ int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, T* w0, bool dot, int* n_part
     Level& C = s->lv[li + 1];
     restricted_x1 = li + 1 < nl - 1 && li + 1 != fuse_li && first_in_tmp(s, level_ns(s, li + 1));
     if (restricted_x1)
-      CHK(restrict_to<T>(s, C, t, Sel<T>::b(C), nb, Sel<T>::t(C), first_step_coef(s)));
+      CHK(restrict_to<T>(s, C, t, Sel<T>::b(C), nb, Sel<T>::t(C),
+                         first_step_coef(s, level_ns(s, li + 1))));
     else
       CHK(restrict_to<T>(s, C, t, Sel<T>::b(C), nb));
   }
@@ -911,7 +919,7 @@ int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* Ap, T* t0, double tol_rel, int 
   // with that fusion the solution update u += alpha p is deferred to the next
   // direction update (which streams p anyway) and a flush at the end
   const bool lazy_u = fuse_x1;
-  const double x1_coef = first_step_coef(s);
+  const double x1_coef = first_step_coef(s, level_ns(s, l0));
   // p starts at 0 so that the first p = z + 0 * p is exact
   if (hipMemsetAsync(p, 0, sizeof(T) * n * nb, s->stream) != hipSuccess) {
     cleanup();
@@ -1234,7 +1242,7 @@ void vx_default_options(vx_options* o) {
   o->precision = VX_FP64;
   o->smoother = VX_SMOOTH_JACOBI;
   o->coarse_mode = VX_COARSE_SWEEPS;
-  o->cheb_degree = 2;
+  o->cheb_ratio_power = 0;
   o->cheb_ratio = 0.3;
   o->inner_tol = 1e-5;
   o->max_outer = 20;
