@@ -289,13 +289,20 @@ def test_batched_mixed_equals_single_mixed_solves():
     batch.close()
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_variable_vcycle_same_solution(precision):
+@pytest.mark.parametrize("precision,extra", [
+    ("fp64", dict(level_smooth=[2, 2, 6, 12, 24])),
+    ("mixed", dict(level_smooth=[2, 2, 6, 12, 24])),
+    # the bench's configuration: 5 levels, 48 coarsest sweeps, each k-step
+    # pass over [0.05 (2/k)^2, 1] lambda_max
+    ("mixed", dict(level_smooth=[2, 2, 6, 12], coarse_sweeps=48, cheb_ratio_power=2)),
+    ("fp64", dict(level_smooth=[2, 2, 6, 12], coarse_sweeps=48, cheb_ratio_power=2)),
+])
+def test_variable_vcycle_same_solution(precision, extra):
     """Smoothing counts that grow toward the coarse levels (a variable
     V-cycle) change the preconditioner, not the solution: both reach the
     FP64 solution of the plain cycle, the variable one in fewer iterations on
     a two-material part."""
-    print("case variable vcycle", precision, flush=True)
+    print("case variable vcycle", precision, extra, flush=True)
     import torch
     from core import workloads as W
     from core.multigrid import MGSolver
@@ -310,7 +317,7 @@ def test_variable_vcycle_same_solution(precision):
     u0 = torch.empty_like(f)
     ref.solve_device(f, u0, tol=1e-9)
     var = MGSolver(p.nx, p.ny, p.nz, p.fixed_dofs, nu=p.nu, compute_mode="GPU",
-                   precision=precision, level_smooth=[2, 2, 6, 12, 24], **kw)
+                   precision=precision, **dict(kw, **extra))
     var.set_density(E, active=A)
     u = torch.empty_like(f)
     var.solve_device(f, u, tol=1e-7)
