@@ -31,92 +31,110 @@ sys.path.insert(0, os.path.join(ROOT, "voxelsim-fea_amd"))
 
 HBM_PEAK_GBS = 8000.0
 
-# the MI355X solver configuration measured as `value`
-OPT = dict(precision="mixed", smoother="chebyshev", cheb_ratio=0.05, system="active", autotune=True,
-           inner_tol=3e-5,
-           # variable V-cycle over 5 levels: more smoothing steps on the latency-bound
-           # coarse levels and 48 sweeps on the 17^3-node coarsest one, each k-step
-           # pass over [0.05 (2/k)^2, 1] * lambda_max
-           max_levels=5, level_smooth=[2, 2, 6, 12], coarse_sweeps=48, cheb_ratio_power=2)
-# the single-case workloads that converge in 19-100 iterations with the plain
-# 7-level cycle, where the extra coarse-level steps cost more than they save
-OPT_PLAIN = dict({k: v for k, v in OPT.items()
-                  if k not in ("level_smooth", "cheb_ratio_power", "coarse_sweeps")},
-                 max_levels=7, inner_tol=1e-5)
+# the MI355X solver configurations: OPT (variable V-cycle over 5 levels) is
+# measured as `value`; OPT_PLAIN (plain 7-level cycle) runs the single-case
+# workloads that converge in 19-100 iterations
+from core.multigrid import PLAIN_CYCLE as OPT_PLAIN, VARIABLE_CYCLE as OPT  # noqa: E402
 
 
 def log(msg):
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(p, iters, budget_s=30.0):
+class _StopSolve(Exception):
+    """Raised by the timing callback to end an oracle solve early."""
+
+
+def cpu_baseline(p, ref_iters, single_iters=2, pool_iters=1):
     """The oracle (NumPy restatement of the reference algorithm) on this
-    host, on a bounded sample of the same workload: level-0 operator
-    applications (gather -> GEMM -> bincount, core/multigrid.py:318-341) and
-    the density set-up.  One MG-PCG iteration costs about 6.7 level-0
-    applications (core/multigrid.py:426-436); the estimate is set-up +
-    iters x 6.7 x apply with the GPU solver's iteration count."""
+    host: the plain FP64 MG-PCG -- damped Jacobi omega 0.4, V(2,2), 30
+    coarsest sweeps, 6 levels, voids at E*e_min on the full grid
+    (core/multigrid.py:85-583, core/fea.py:467-469) -- on the same workload,
+    in both of the reference's CPU modes.  Each run is a real solve whose
+    progress callback stamps the end of every PCG iteration and stops it
+    after a few; an iteration is one outer operator, its vector updates and
+    one V-cycle.  Estimate = hierarchy build + set_density + ref_iters x the
+    measured time per iteration (ref_iters: the same algorithm's iteration
+    count on the GPU).  Modes: one process with nproc-1 BLAS threads
+    (core/compute_plan.py:80-90, 186-188), and the element-partitioned pool
+    of nproc-1 spawned one-thread workers for the outer operator with one
+    BLAS thread in the main process (core/parallel_cpu.py:56-205,
+    core/compute_plan.py:168-172, 181-188)."""
     import numpy as np
     from oracle.multigrid_ref import MGSolverRef
     # a one-GPU host may grant this process only 16 CPUs, whatever os.cpu_count() says
     threads = max(1, min((os.cpu_count() or 2) - 1, 15))
     try:
         import threadpoolctl
-        threadpoolctl.threadpool_limits(limits=threads)
     except Exception:
-        threads = 1
-    log("cpu baseline: building the oracle hierarchy")
-    mg = MGSolverRef(p.nx, p.ny, p.nz, p.fixed_dofs, nu=p.nu, max_levels=p.max_levels)
-    E = np.where(p.active, p.E_stiffness, p.E_stiffness * 1e-9)
+        threadpoolctl = None
+
+    def blas(n):
+        if threadpoolctl is not None:
+            threadpoolctl.threadpool_limits(limits=n)
+
+    blas(threads if threadpoolctl else 1)
+    log("cpu baseline: building the oracle hierarchy (omega 0.4, 6 levels)")
     t0 = time.perf_counter()
-    mg.set_density(E)
-    t_density = time.perf_counter() - t0
-    log(f"cpu baseline: set_density {t_density:.1f} s")
-    u = np.random.default_rng(0).standard_normal(mg.ndof)
-    times = []
-    t_start = time.perf_counter()
-    while len(times) < 3 and (time.perf_counter() - t_start) < budget_s:
+    mg = MGSolverRef(p.nx, p.ny, p.nz, p.fixed_dofs, nu=p.nu, omega=0.4, max_levels=6)
+    t_build = time.perf_counter() - t0
+    E = np.where(p.active, p.E_stiffness, p.E_stiffness * 1e-9)
+
+    def iterations(n):
         t0 = time.perf_counter()
-        mg._apply(0, u)
-        times.append(time.perf_counter() - t0)
-        log(f"cpu baseline: level-0 apply {times[-1]:.2f} s")
-    t_apply = min(times)
-    single = t_density + iters * 6.7 * t_apply
-    # the reference's other CPU mode (AUTO at >= 500k DOF without a GPU):
-    # the level-0 operator over nproc-1 spawned workers, one BLAS thread each
-    # (core/parallel_cpu.py:56-65, 111-164; core/compute_plan.py:168-172)
+        mg.set_density(E)
+        t_density = time.perf_counter() - t0
+        stamps = []
+
+        def cb(it, ratio, tol):
+            stamps.append(time.perf_counter())
+            log(f"cpu baseline: iteration {it} ratio {ratio:.3e} at "
+                f"{stamps[-1] - t_start:.1f} s")
+            if it >= n:
+                raise _StopSolve()
+        t_start = time.perf_counter()
+        try:
+            mg.solve(E, p.force, tol=1e-6, max_cg=n + 1, progress_cb=cb)
+        except _StopSolve:
+            pass
+        # the solve repeats set_density before its first iteration
+        per_iter = (stamps[-1] - t_start - t_density) / len(stamps)
+        return {"iterations": len(stamps), "set_density_s": t_density,
+                "per_iteration_s": per_iter,
+                "value": t_build + t_density + ref_iters * per_iter}
+
+    single = iterations(single_iters)
+    single["threads"] = threads
+    log(f"cpu baseline, one process: {single}")
     pool = None
     try:
         from oracle.parallel_cpu_ref import CPUMatVecPoolRef
-        log(f"cpu baseline: starting a {threads}-worker matvec pool")
+        log(f"cpu baseline: starting a {threads}-worker operator pool")
         lv0 = mg.levels[0]
-        pl = CPUMatVecPoolRef(lv0["edof"], mg.KE, lv0["ndof"], threads)
+        mg._pool = CPUMatVecPoolRef(lv0["edof"], mg.KE, lv0["ndof"], threads)
+        blas(1)
         try:
-            pl.set_density(lv0["_scaled"])
-            ptimes = []
-            t_start = time.perf_counter()
-            while len(ptimes) < 3 and (time.perf_counter() - t_start) < budget_s:
-                t0 = time.perf_counter()
-                pl.apply(u)
-                ptimes.append(time.perf_counter() - t0)
-                log(f"cpu baseline: pooled level-0 apply {ptimes[-1]:.2f} s")
-            pool = {"workers": pl.n_workers, "apply_s": min(ptimes),
-                    "value": t_density + iters * 6.7 * min(ptimes)}
+            pool = iterations(pool_iters)
+            pool["workers"] = mg._pool.n_workers
         finally:
-            pl.close()
+            mg.close()
+        log(f"cpu baseline, pool: {pool}")
     except Exception as e:   # the timing leg must not stop the bench
         log(f"cpu baseline: pool mode skipped ({e})")
-    best = min(single, pool["value"]) if pool else single
-    mode = "pool" if pool and pool["value"] < single else "single process"
-    return {"value": best, "unit": "s", "cores": threads,
-            "kind": "port",
-            "sample": (f"{len(times)} oracle level-0 operator applications on the same "
-                       f"{p.nx}^3 workload (min {t_apply:.2f} s"
-                       + (f"; {pool['workers']}-worker pool {pool['apply_s']:.2f} s" if pool else "")
-                       + f") + set_density ({t_density:.1f} s); solve estimate = set_density + "
-                       f"{iters} iterations x 6.7 applications, faster mode: {mode}"),
-            "single_process": {"threads": threads, "apply_s": t_apply, "value": single},
-            "pool": pool, "cpu_model": cpu_model(), "ram_gb": ram_gb()}
+    best = min(single["value"], pool["value"]) if pool else single["value"]
+    mode = "pool" if pool and pool["value"] < single["value"] else "single process"
+    return {"value": best, "unit": "s", "cores": threads, "kind": "port",
+            "sample": (f"the reference algorithm (oracle: Jacobi omega 0.4, V(2,2), 6 levels, "
+                       f"e_min voids) on the same {p.nx}^3 workload, stopped after "
+                       f"{single['iterations']} complete PCG iterations in one process "
+                       f"({single['per_iteration_s']:.1f} s each)"
+                       + (f" and {pool['iterations']} with the {pool['workers']}-worker pool "
+                          f"({pool['per_iteration_s']:.1f} s each)" if pool else "")
+                       + f"; solve = hierarchy {t_build:.1f} s + set_density + {ref_iters} "
+                       f"iterations (its GPU count); faster mode: {mode}; the shipped omega "
+                       f"0.6 stalls (reference_fp64_mgpcg)"),
+            "hierarchy_s": t_build, "single_process": single, "pool": pool,
+            "cpu_model": cpu_model(), "ram_gb": ram_gb()}
 
 
 def cpu_model():
@@ -200,6 +218,124 @@ def _oracle_solve(p, max_levels=6, active_system=False):
     t0 = time.perf_counter()
     u = vf.solve(p.E_stiffness, p.force if p.force.ndim == 1 else p.force[0], tol=p.tol)
     return u, time.perf_counter() - t0, vf.last_iters, vf.last_converged
+
+
+def config_records():
+    """Compact records of the other four configurations for the headline line:
+    time to tolerance with inputs resident in HBM (second of two runs; the
+    first builds the autotuned launches and graphs), iterations, true
+    residual and the relative displacement error on the nodes of active
+    elements -- against the reference's own solution where this exact
+    problem has one (c1: tests/golden/cantilever.npz, written by the
+    reference), else against an FP64 solve to 1e-10 with the same
+    preconditioner (c2-c4 at full size; their scaled-down instances are
+    compared with the reference's solutions in tests/test_gpu_parity_configs.py)."""
+    import numpy as np
+    import torch
+    from core import workloads as W
+    from core.multigrid import MGSolver
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    def used_of(q):
+        return np.repeat(W.element_nodes_mask(q.active.reshape(q.nx, q.ny, q.nz)).ravel(), 3)
+
+    def rel(u, ref, used):
+        return float(np.abs(u[..., used] - ref[..., used]).max() / np.abs(ref[..., used]).max())
+
+    def dev(q):
+        return (torch.tensor(q.E_stiffness, device="cuda"),
+                torch.tensor(q.active.astype(np.uint8), device="cuda"),
+                torch.tensor(q.force, device="cuda"))
+
+    out = {}
+    # c1: the reference algorithm in FP64 (damped Jacobi 0.6, 3 levels, tol 1e-8)
+    log("configs: c1 cantilever")
+    q = W.cantilever()
+    E, A, f = dev(q)
+    u = torch.empty_like(f)
+    ref_u = np.load(os.path.join(ROOT, "tests", "golden", "cantilever.npz"))["u"]
+    rec = {}
+    for name, kw in (("reference_fp64", dict(max_levels=q.max_levels)),
+                     ("mixed", dict(OPT_PLAIN, max_levels=q.max_levels))):
+        mg = MGSolver(q.nx, q.ny, q.nz, q.fixed_dofs, nu=q.nu, compute_mode="GPU", **kw)
+        t = timed(lambda: (mg.set_density(E, active=A, e_min=1e-9),
+                           mg.solve_device(f, u, tol=q.tol)))
+        rec[name] = {"time_s": t, "iters": mg.last_iters,
+                     "true_resid": mg.last_true_resid_ratio,
+                     "rel_disp_err_vs_reference": rel(u.cpu().numpy(), ref_u, used_of(q))}
+        mg.close()
+    out["c1_cantilever_64x16x16"] = rec
+    # c2: 128^3 aluminium block with 40 spherical voids, plain 7-level cycle
+    log("configs: c2 voids block")
+    q = W.voids_block()
+    E, A, f = dev(q)
+    u = torch.empty_like(f)
+    exact = MGSolver(q.nx, q.ny, q.nz, q.fixed_dofs, nu=q.nu, compute_mode="GPU",
+                     **dict(OPT_PLAIN, precision="fp64"))
+    exact.set_density(E, active=A)
+    exact.solve_device(f, u, tol=1e-10)
+    u_ex = u.cpu().numpy()
+    exact.close()
+    mg = MGSolver(q.nx, q.ny, q.nz, q.fixed_dofs, nu=q.nu, compute_mode="GPU", **OPT_PLAIN)
+    t = timed(lambda: (mg.set_density(E, active=A), mg.solve_device(f, u, tol=q.tol)))
+    out["c2_voids_block_128"] = {"time_s": t, "iters": mg.last_iters,
+                                 "true_resid": mg.last_true_resid_ratio,
+                                 "rel_disp_err_vs_fp64_1e-10": rel(u.cpu().numpy(), u_ex,
+                                                                   used_of(q))}
+    mg.close()
+    # c3: 96^3 part, 32 load cases in one batched solve (variable cycle)
+    log("configs: c3 multi-RHS part")
+    q = W.multi_rhs_part()
+    E, A, F = dev(q)
+    U = torch.empty_like(F)
+    nrhs = F.shape[0]
+    exact = MGSolver(q.nx, q.ny, q.nz, q.fixed_dofs, nu=q.nu, compute_mode="GPU",
+                     max_batch=nrhs, **dict(OPT, precision="fp64"))
+    exact.set_density(E, active=A)
+    exact.solve_batch_device(F, U, tol=1e-10)
+    U_ex = U.cpu().numpy()
+    exact.close()
+    mg = MGSolver(q.nx, q.ny, q.nz, q.fixed_dofs, nu=q.nu, compute_mode="GPU", max_batch=nrhs,
+                  **OPT)
+    t = timed(lambda: (mg.set_density(E, active=A), mg.solve_batch_device(F, U, tol=q.tol)))
+    out["c3_multi_rhs_96x32"] = {
+        "time_s": t, "load_cases_per_s": nrhs / t,
+        "iters_max": max(b["iters"] for b in mg.last_batch),
+        "true_resid_max": max(b["true_resid_ratio"] for b in mg.last_batch),
+        "rel_disp_err_vs_fp64_1e-10": rel(U.cpu().numpy(), U_ex, used_of(q))}
+    mg.close()
+    # c4: 16 uneven ABS parts, each on its void-padded grid (plain cycle)
+    log("configs: c4 uneven parts")
+    parts = W.uneven_parts()
+    total, iters, errs, trues = 0.0, [], [], []
+    for q in parts:
+        E, A, f = dev(q)
+        u = torch.empty_like(f)
+        exact = MGSolver(q.nx, q.ny, q.nz, q.fixed_dofs, nu=q.nu, compute_mode="GPU", pad=True,
+                         **dict(OPT_PLAIN, precision="fp64"))
+        exact.set_density(E, active=A)
+        exact.solve_device(f, u, tol=1e-10, max_cg=5000)
+        u_ex = u.cpu().numpy()
+        exact.close()
+        mg = MGSolver(q.nx, q.ny, q.nz, q.fixed_dofs, nu=q.nu, compute_mode="GPU", pad=True,
+                      **OPT_PLAIN)
+        total += timed(lambda: (mg.set_density(E, active=A),
+                                mg.solve_device(f, u, tol=q.tol, max_cg=5000)))
+        iters.append(mg.last_iters)
+        trues.append(mg.last_true_resid_ratio)
+        errs.append(rel(u.cpu().numpy(), u_ex, used_of(q)))
+        mg.close()
+    out["c4_uneven_parts_16"] = {"time_s": total, "form": "one padded solve per part",
+                                 "iters": iters, "true_resid_max": max(trues),
+                                 "rel_disp_err_vs_fp64_1e-10": max(errs)}
+    return out
 
 
 def other_workload(args):
@@ -362,6 +498,8 @@ def main():
                          "per-configuration measurement")
     ap.add_argument("--z-chunk", type=int, default=10,
                     help="--kernel-only: node planes per work-group of the timed kernel")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the compact records of the other four configurations")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip training / measuring the learned corrector")
     ap.add_argument("--oracle-full", action="store_true",
@@ -554,10 +692,15 @@ def main():
     copy_gbs = copy_bandwidth()
     log(f"plain device copy: {copy_gbs:.0f} GB/s")
 
+    configs = None
+    if not args.no_configs:
+        configs = config_records()
+        log(f"configs: {configs}")
+
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(p, fp64["iters"])
+            cpu = cpu_baseline(p, plaininfo.get("iters") or fp64["iters"])
         except Exception as exc:
             log(f"cpu baseline failed: {exc}")
 
@@ -606,6 +749,7 @@ def main():
             "chebyshev_fp32": {"ms": ms_jac32, "GBs": gbs},
             "matvec_fp32": {"ms": ms_mv32, "GBs": bytes_mv32 / (ms_mv32 * 1e-3) / 1e9},
             "matvec_fp64": {"ms": ms_mv64, "GBs": bytes_mv64 / (ms_mv64 * 1e-3) / 1e9}},
+        "configs": configs,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

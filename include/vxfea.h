@@ -159,11 +159,14 @@ int vx_solve_batch(vx_handle h, int32_t nrhs, const double* F_dev, double* U_dev
  * the active-DOF system (inactive elements dropped, no e_min, loads on nodes
  * of no active element ignored).  E_dev/active_dev in reference element
  * order, f_dev/u_dev full-grid float64; u is 0 on DOFs outside the system.
- * max_iter <= 0 means 4 * (number of free DOFs). */
+ * x0_dev (may be NULL) is a full-grid warm start: the iteration starts from
+ * x0 on the system's free DOFs with r = f - K x0 there (core/fea.py:529-535);
+ * a zero load still returns u = 0.  max_iter <= 0 means 4 * (number of free
+ * DOFs). */
 int vx_solve_jacobi(vx_handle h, const double* E_dev, const uint8_t* active_dev,
-                    const double* f_dev, double* u_dev, double tol, int32_t max_iter,
-                    vx_progress_cb cb, void* user, double* hist_host, int32_t hist_cap,
-                    vx_info* info);
+                    const double* f_dev, double* u_dev, const double* x0_dev, double tol,
+                    int32_t max_iter, vx_progress_cb cb, void* user, double* hist_host,
+                    int32_t hist_cap, vx_info* info);
 
 /* Single-operator entry points for parity tests.  All float64, level-l sized.
  * vx_matvec: MGSolver._apply  (masked K u, core/multigrid.py:318-341)

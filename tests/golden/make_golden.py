@@ -7,9 +7,11 @@ no closed form pins, so they are recorded from the reference's own
 ``core.*`` with the reference directory on ``sys.path``, the way its solve
 worker does, core/solve_worker.py:76-83).  Inputs and outputs are stored as
 small ``.npz`` fixtures next to this script; the tests compare the oracle and
-the GPU solver against them.  Run from the repository root:
+the GPU solver against them.  ``configs/`` holds the reference's Jacobi-PCG
+solutions of scaled-down benchmark configurations (``run_configs``).  Run
+from the repository root:
 
-    python tests/golden/make_golden.py /root/reference
+    python tests/golden/make_golden.py /root/reference [--x0 | --configs]
 """
 
 import os
@@ -86,10 +88,62 @@ def cases():
     return out
 
 
-def run(ref_root):
+def x0_cases():
+    """Warm starts on the Jacobi-PCG path (core/fea.py:529-535): directly
+    (use_multigrid=False) and after the MG stall guard fired."""
+    out = []
+    p = W.voids_block(n=12, n_spheres=4, seed=21)
+    out.append(("jacobi_x0", p, {"use_multigrid": False, "x0_scale": 0.5}))
+    out.append(("point_stall_x0", point_stall_problem(8), {"max_cg": 60, "x0_scale": 0.5}))
+    return out
+
+
+def config_cases():
+    """Scaled-down instances of the benchmark configurations, solved by the
+    reference's Jacobi-PCG on the active-DOF system (use_multigrid=False,
+    core/fea.py:497-593) to a recurrence ratio of 1e-10: the system the
+    benchmark's active-DOF solver solves, so its u is the pin."""
+    out = []
+    out.append(("c5_two_material24", W.two_material(n=24), {}))
+    out.append(("c2_voids_block32", W.voids_block(n=32), {}))
+    out.append(("c3_multi_rhs_part32", W.multi_rhs_part(n=32, nrhs=2), {}))
+    out.append(("c4_uneven_part_a", W.uneven_parts(n_parts=2, lo=33, hi=48, seed=7)[1], {}))
+    out.append(("c4_uneven_part_b", W.uneven_parts(n_parts=1, lo=33, hi=48, seed=11)[0], {}))
+    return out
+
+
+def run_configs(ref_root, tol=1e-10):
     sys.path.insert(0, ref_root)
     from core.fea import VoxelFEA                       # reference
-    for name, p, opt in cases():
+    os.makedirs(os.path.join(HERE, "configs"), exist_ok=True)
+    for name, p, _ in config_cases():
+        forces = p.force if p.force.ndim == 2 else p.force[None]
+        us, vms, its = [], [], []
+        for f in forces:
+            vf = VoxelFEA(p.nx, p.ny, p.nz, nu=p.nu, active_elems=p.active,
+                          compute_mode="CPU", use_multigrid=False)
+            vf.set_fixed(p.fixed_dofs)
+            u = vf.solve(p.E_stiffness, f, tol=tol)
+            assert vf.last_converged, name
+            us.append(u)
+            vms.append(vf.element_von_mises_stress(u, p.E_pa) / p.vsize)
+            its.append(vf.last_iters)
+            vf.close()
+        np.savez_compressed(
+            os.path.join(HERE, "configs", f"{name}.npz"),
+            nx=p.nx, ny=p.ny, nz=p.nz, nu=p.nu, vsize=p.vsize, active=p.active,
+            E_pa=p.E_pa, fixed_dofs=p.fixed_dofs, force=p.force, ref_tol=tol,
+            u=np.asarray(us) if p.force.ndim == 2 else us[0],
+            vm=np.asarray(vms) if p.force.ndim == 2 else vms[0],
+            last_iters=np.asarray(its), meta_seed=int(p.meta.get("seed", -1)))
+        print(f"{name}: {p.nx}x{p.ny}x{p.nz} iters={its} |u|max={np.abs(us[0]).max():.3e}",
+              flush=True)
+
+
+def run(ref_root, which=None):
+    sys.path.insert(0, ref_root)
+    from core.fea import VoxelFEA                       # reference
+    for name, p, opt in (cases() + x0_cases() if which is None else which):
         vf = VoxelFEA(p.nx, p.ny, p.nz, nu=p.nu, active_elems=p.active,
                       compute_mode="CPU",
                       use_multigrid=opt.get("use_multigrid", True))
@@ -122,4 +176,11 @@ def run(ref_root):
 
 
 if __name__ == "__main__":
-    run(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
+    root = next((a for a in sys.argv[1:] if not a.startswith("--")), "/root/reference")
+    if "--x0" in sys.argv:
+        run(root, x0_cases())
+    elif "--configs" in sys.argv:
+        run_configs(root)
+    else:
+        run(root)
+        run_configs(root)

@@ -34,11 +34,18 @@ def test_cpu_baseline_record():
     import bench
     from core import workloads as W
     p = W.two_material(n=16)
-    p.max_levels = 3
-    r = bench.cpu_baseline(p, 10, budget_s=5)
+    r = bench.cpu_baseline(p, 10, single_iters=2, pool_iters=1)
     for k in ("value", "unit", "cores", "kind", "sample"):
         assert k in r
     assert r["kind"] == "port" and r["unit"] == "s" and 1 <= r["cores"] <= 15
     assert r["value"] > 0
-    single = r["single_process"]["value"]
-    assert r["value"] == (min(single, r["pool"]["value"]) if r["pool"] else single)
+    single = r["single_process"]
+    # measured iterations of the reference algorithm, not an operator count
+    assert single["iterations"] == 2 and single["per_iteration_s"] > 0
+    assert single["value"] == pytest.approx(r["hierarchy_s"] + single["set_density_s"]
+                                            + 10 * single["per_iteration_s"])
+    if r["pool"]:
+        assert r["pool"]["iterations"] == 1
+    assert r["value"] == (min(single["value"], r["pool"]["value"]) if r["pool"]
+                          else single["value"])
+    assert "omega 0.4" in r["sample"] and "complete PCG iterations" in r["sample"]

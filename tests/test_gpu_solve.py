@@ -17,7 +17,7 @@ from oracle.fea_ref import VoxelFEARef, von_mises
 
 pytestmark = pytest.mark.gpu
 
-CONVERGING = [c for c in golden_cases() if c not in ("twomat16", "point_stall")]
+CONVERGING = [c for c in golden_cases() if c not in ("twomat16", "point_stall", "point_stall_x0")]
 
 
 def _fea(g, **kw):
@@ -41,8 +41,6 @@ def test_fp64_matches_reference_trace(name):
     g = load_golden(name)
     vf = _fea(g)
     x0 = g["x0"] if g["x0"].size else None
-    if x0 is not None and not bool(g["use_multigrid"]):
-        pytest.skip("x0 on the Jacobi path is not a golden case")
     u = vf.solve(g["E_pa"] * float(g["vsize"]), g["force"], tol=float(g["tol"]), x0=x0)
     assert vf.last_iters == int(g["last_iters"])
     assert bool(vf.last_converged) == bool(g["last_converged"])
@@ -125,6 +123,36 @@ def test_stall_guard_falls_back_like_reference():
     h = np.asarray(vf.resid_history)[:8]
     ref = g["resid_history"][:8]
     assert np.max(np.abs(h - ref) / ref) < 1e-6
+    vf.close()
+
+
+def test_stall_with_x0_falls_back_to_warm_started_jacobi():
+    """The stall guard fires on a warm-started solve: the fallback Jacobi-PCG
+    starts from the same x0, as the reference does (core/fea.py:476-489,
+    529-535; golden point_stall_x0, written by the reference)."""
+    print("case point_stall_x0", flush=True)
+    g = load_golden("point_stall_x0")
+    vf = _fea(g)
+    vf.solve(g["E_pa"] * float(g["vsize"]), g["force"], tol=float(g["tol"]),
+             max_cg=int(g["max_cg"]), x0=g["x0"])
+    assert not vf.use_multigrid and bool(g["fell_back"])
+    assert vf.last_iters == int(g["last_iters"]) and not vf.last_converged
+    h = np.asarray(vf.resid_history)[:8]
+    ref = g["resid_history"][:8]
+    assert np.max(np.abs(h - ref) / ref) < 1e-6
+    # the cold-started fallback takes a different path from the first step
+    cold = load_golden("point_stall")["resid_history"][:8]
+    assert np.max(np.abs(cold - ref) / ref) > 1e-3
+    vf.close()
+
+
+def test_jacobi_x0_zero_load_returns_zeros():
+    """A zero load returns u = 0 even with a warm start (core/fea.py:542-545)."""
+    g = load_golden("jacobi_x0")
+    vf = _fea(g)
+    u = vf.solve(g["E_pa"] * float(g["vsize"]), np.zeros_like(g["force"]), tol=1e-6,
+                 x0=g["x0"])
+    assert np.all(u == 0) and vf.last_converged
     vf.close()
 
 

@@ -230,3 +230,112 @@ def test_level_smooth_option_packing():
     for bad in ([1] * 17, [256], [-1]):
         with pytest.raises(ValueError):
             _level_smooth(bad)
+
+
+def _create_status(dims=(16, 16, 16), **kw):
+    """vx_create's status for options kw; a handle it returns is destroyed."""
+    from core import _native
+    lib = _native.load()
+    h = ctypes.c_void_p()
+    rc = lib.vx_create(*dims, 0.3, ctypes.byref(_native.default_options(**kw)), ctypes.byref(h))
+    if rc == _native.VX_OK:
+        lib.vx_destroy(h)
+    return rc
+
+
+def test_create_rejects_degenerate_chebyshev_intervals():
+    """Every Chebyshev pass smooths over [r lambda, lambda] with r = cheb_ratio
+    (n_smooth / k)^p for its k steps; r must lie in (0, 1) (r = 1 gives a
+    zero-width interval and division by zero in the step coefficients).
+    vx_create checks each level's step count and the coarsest sweeps before
+    it looks for a device, so a bad set is VX_ERR_ARG on any machine."""
+    from core import _native
+    from core.multigrid import _level_smooth
+    cheb = _native.VX_SMOOTH_CHEBYSHEV
+    ok = (_native.VX_OK, _native.VX_ERR_NO_DEVICE)
+    bad = _native.VX_ERR_ARG
+    # level 1 with k = 2 < n_smooth = 4: r = 0.25 * (4/2)^2 = 1
+    assert _create_status(smoother=cheb, cheb_ratio=0.25, n_smooth=4, cheb_ratio_power=2,
+                          max_levels=3, level_smooth=_level_smooth([4, 2])) == bad
+    # the coarsest sweeps: r = 0.3 * (4/2)^2 = 1.2
+    assert _create_status(smoother=cheb, cheb_ratio=0.3, n_smooth=4, cheb_ratio_power=2,
+                          max_levels=3, coarse_sweeps=2) == bad
+    for r in (0.0, 1.0, -0.1, float("nan")):
+        assert _create_status(smoother=cheb, cheb_ratio=r) == bad, r
+    # the same ratios with the Jacobi smoother are never used
+    assert _create_status(cheb_ratio=1.0) in ok
+    # the benchmark's configuration: 0.05 (2/k)^2 for k = 2, 6, 12 and 48
+    assert _create_status(smoother=cheb, cheb_ratio=0.05, cheb_ratio_power=2, max_levels=5,
+                          level_smooth=_level_smooth([2, 2, 6, 12]), coarse_sweeps=48) in ok
+    # a one-level hierarchy (odd edge) uses only the coarsest sweeps
+    assert _create_status((15, 16, 16), smoother=cheb, cheb_ratio=0.3, n_smooth=4,
+                          cheb_ratio_power=2, coarse_sweeps=2) == bad
+
+
+# ---------------------------------------------------------------------------
+# element_strain_energy (core/fea.py:595-602)
+# ---------------------------------------------------------------------------
+
+def _sparse_K(nx, ny, nz, E, nu):
+    """Global K = sum_e E_e P_e^T KE P_e by scipy's sparse assembly."""
+    import scipy.sparse as sp
+    edof = build_edof(nx, ny, nz)
+    KE = hex8_KE(1.0, nu)
+    n = 3 * (nx + 1) * (ny + 1) * (nz + 1)
+    rows = np.repeat(edof, 24, axis=1).ravel()
+    cols = np.tile(edof, (1, 24)).ravel()
+    vals = (np.asarray(E, float)[:, None, None] * KE[None]).ravel()
+    return sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+
+
+@pytest.mark.parametrize("dims", [(3, 2, 2), (4, 3, 5)])
+def test_strain_energy_sums_to_global_quadratic_form(dims):
+    """sum_e E_e ce_e = u^T K u with K assembled by scipy, E_e = 0 on the
+    inactive elements (their ce is 0); and a field living on one element's
+    nodes only has energy there and in the elements sharing those nodes."""
+    from oracle.fea_ref import VoxelFEARef
+    nx, ny, nz = dims
+    rng = np.random.default_rng(sum(dims))
+    active = rng.random(nx * ny * nz) < 0.7
+    E = rng.uniform(0.5, 3.0, nx * ny * nz)
+    ref = VoxelFEARef(nx, ny, nz, nu=0.3, active_elems=active)
+    u = rng.standard_normal(3 * (nx + 1) * (ny + 1) * (nz + 1))
+    ce = ref.element_strain_energy(u)
+    assert np.all(ce[~active] == 0) and np.all(ce[active] > 0)
+    K = _sparse_K(nx, ny, nz, np.where(active, E, 0.0), 0.3)
+    assert np.isclose((E * ce).sum(), u @ (K @ u), rtol=1e-12)
+    # one element's corner DOFs only: the energy sits in the elements that
+    # share at least one of those nodes
+    e = int(np.nonzero(active)[0][0])
+    v = np.zeros_like(u)
+    edof = build_edof(nx, ny, nz)
+    v[edof[e]] = rng.standard_normal(24)
+    ce1 = ref.element_strain_energy(v)
+    nodes = set(edof[e][0::3] // 3)
+    share = np.array([bool(nodes & set(edof[k][0::3] // 3)) for k in range(nx * ny * nz)])
+    assert np.all(ce1[~share] == 0) and ce1[e] > 0
+
+
+@pytest.mark.parametrize("nu", [0.0, 0.3, 0.42])
+def test_strain_energy_closed_forms(nu):
+    """Linear fields in unit cubes with E = 1: uniaxial stress s_xx = e has
+    u = (e x, -nu e y, -nu e z) and u_e^T KE u_e = eps.sigma = e^2; simple
+    shear u = (g y, 0, 0) has g^2 G with G = 1 / (2 (1 + nu))."""
+    from oracle.fea_ref import VoxelFEARef
+    nx, ny, nz = 3, 2, 4
+    ix, iy, iz = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1),
+                             indexing="ij")
+
+    def field(ux, uy, uz):
+        u = np.stack([ux, uy, uz], -1).transpose(2, 1, 0, 3)   # node order x fastest
+        return u.reshape(-1)
+
+    ref = VoxelFEARef(nx, ny, nz, nu=nu)
+    e, g = 1e-3, 2e-3
+    ce = ref.element_strain_energy(field(e * ix, -nu * e * iy, -nu * e * iz))
+    assert np.allclose(ce, e * e, rtol=1e-12)
+    ce = ref.element_strain_energy(field(g * iy, 0 * iy, 0 * iz))
+    assert np.allclose(ce, g * g / (2 * (1 + nu)), rtol=1e-12)
+    # a rigid translation plus rotation carries no energy
+    ce = ref.element_strain_energy(field(1.0 + 0.2 * iy, -0.2 * ix, 0.5 + 0 * iz))
+    assert np.abs(ce).max() < 1e-14

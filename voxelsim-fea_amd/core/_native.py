@@ -97,7 +97,7 @@ SIGNATURES = {
                                 ctypes.POINTER(vx_info)]),
     "vx_solve_batch": (ctypes.c_int, [_P, _I32, _P, _P, _D, _I32, PROGRESS_CB, _P, _P, _I32,
                                       ctypes.POINTER(vx_info)]),
-    "vx_solve_jacobi": (ctypes.c_int, [_P, _P, _P, _P, _P, _D, _I32, PROGRESS_CB, _P, _P,
+    "vx_solve_jacobi": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _D, _I32, PROGRESS_CB, _P, _P,
                                        _I32, ctypes.POINTER(vx_info)]),
     "vx_matvec": (ctypes.c_int, [_P, _I32, _I32, _P, _P]),
     "vx_smooth": (ctypes.c_int, [_P, _I32, _I32, _P, _P, _I32, _P]),

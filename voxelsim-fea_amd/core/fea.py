@@ -250,11 +250,11 @@ class VoxelFEA:
         """Jacobi-PCG on the active-DOF system, on the device (core/fea.py:497-593)."""
         if self.free is None:
             raise RuntimeError("call set_fixed() before solve()")
-        if x0 is not None:
-            raise NotImplementedError("x0 warm start is supported on the multigrid path")
         E_dev = _dev(np.asarray(Evec, dtype=float))
         f_dev = _dev(f)
         u_dev = _torch().empty_like(f_dev)
+        # warm start: x0 on the reduced free DOFs, r = f - K x0 (core/fea.py:529-535)
+        x0_dev = None if x0 is None else _dev(x0)
         hist_cap = 200000
         hist = np.zeros(hist_cap)
         info = _native.vx_info()
@@ -266,8 +266,8 @@ class VoxelFEA:
         cb = _native.PROGRESS_CB(_cb) if progress_cb is not None else _native.PROGRESS_CB()
         self._lib.vx_set_stream(self._h, ctypes.c_void_p(_torch().cuda.current_stream().cuda_stream))
         rc = self._lib.vx_solve_jacobi(self._h, _native.ptr(E_dev), _native.ptr(self._active_dev),
-                                       _native.ptr(f_dev), _native.ptr(u_dev), float(tol),
-                                       int(max_cg or 0), cb, None,
+                                       _native.ptr(f_dev), _native.ptr(u_dev),
+                                       _native.ptr(x0_dev), float(tol), int(max_cg or 0), cb, None,
                                        hist.ctypes.data_as(ctypes.c_void_p), hist_cap,
                                        ctypes.byref(info))
         _native.check(rc, self._h, allow=(_native.VX_ERR_NOT_CONVERGED,))

@@ -25,6 +25,21 @@ _PRECISION = {"fp64": _native.VX_FP64, "mixed": _native.VX_MIXED}
 _SMOOTHER = {"jacobi": _native.VX_SMOOTH_JACOBI, "chebyshev": _native.VX_SMOOTH_CHEBYSHEV}
 
 
+# The two solver configurations the benchmark measures (MGSolver keyword
+# arguments).  VARIABLE_CYCLE: mixed precision, Chebyshev smoothing on the
+# active-DOF system, a 5-level V-cycle with more smoothing steps on the
+# latency-bound coarse levels (2, 2, 6, 12 on levels 0-3, 48 sweeps on the
+# coarsest), each k-step pass over [0.05 (2/k)^2, 1] lambda_max, inner FP32
+# solves to 3e-5.  PLAIN_CYCLE: the same with two steps on every level of a
+# 7-level hierarchy, for parts whose plain cycle converges in a few tens of
+# iterations (there the extra coarse-level steps cost more than they save).
+VARIABLE_CYCLE = dict(precision="mixed", smoother="chebyshev", cheb_ratio=0.05,
+                      system="active", autotune=True, inner_tol=3e-5, max_levels=5,
+                      level_smooth=[2, 2, 6, 12], coarse_sweeps=48, cheb_ratio_power=2)
+PLAIN_CYCLE = dict(precision="mixed", smoother="chebyshev", cheb_ratio=0.05, system="active",
+                   autotune=True, inner_tol=1e-5, max_levels=7)
+
+
 def _torch():
     import torch
     return torch
@@ -306,9 +321,10 @@ class MGSolver:
         return rc
 
     def solve_jacobi_device(self, E_dev, active_dev, f_dev, u_dev, tol=1e-4, max_cg=None,
-                            progress_cb=None, hist_cap=100000):
+                            progress_cb=None, hist_cap=100000, x0_dev=None):
         """Jacobi-PCG on the active-DOF system on the device
-        (core/fea.py:497-593); same fields as solve_device."""
+        (core/fea.py:497-593), warm-started from ``x0_dev`` when given; same
+        fields as solve_device."""
         self._unpadded_only('solve_jacobi_device')
         self._stream_bind()
         hist = np.zeros(hist_cap)
@@ -319,8 +335,8 @@ class MGSolver:
 
         cb = _native.PROGRESS_CB(_cb) if progress_cb is not None else _native.PROGRESS_CB()
         rc = self._lib.vx_solve_jacobi(self._h, _native.ptr(E_dev), _native.ptr(active_dev),
-                                       _native.ptr(f_dev), _native.ptr(u_dev), float(tol),
-                                       int(max_cg or 0), cb, None,
+                                       _native.ptr(f_dev), _native.ptr(u_dev),
+                                       _native.ptr(x0_dev), float(tol), int(max_cg or 0), cb, None,
                                        hist.ctypes.data_as(ctypes.c_void_p), hist_cap,
                                        ctypes.byref(info))
         self.last_iters = info.iters

@@ -1274,43 +1274,51 @@ void vx_element_apply_host(double nu, const double* u24, double* y24) {
 
 int vx_create(int32_t nx, int32_t ny, int32_t nz, double nu, const vx_options* opt,
               vx_handle* out) {
-  vx_solver* s = nullptr;
   if (!out) return VX_ERR_ARG;
   *out = nullptr;
   if (nx < 1 || ny < 1 || nz < 1 || !(nu > -1.0 && nu < 0.5)) return VX_ERR_ARG;
-  if (vx_device_count() < 1) return VX_ERR_NO_DEVICE;
-  s = new vx_solver();
-  s->nx = nx; s->ny = ny; s->nz = nz; s->nu = nu;
+  vx_options o;
   if (opt) {
-    if (opt->struct_size != (int32_t)sizeof(vx_options)) {
-      delete s;
-      return VX_ERR_ARG;
-    }
-    s->opt = *opt;
+    if (opt->struct_size != (int32_t)sizeof(vx_options)) return VX_ERR_ARG;
+    o = *opt;
   } else {
-    vx_default_options(&s->opt);
+    vx_default_options(&o);
   }
-  if (s->opt.max_batch < 1) s->opt.max_batch = 1;
-  if (s->opt.n_smooth < 1 || s->opt.coarse_sweeps < 1 || s->opt.max_levels < 1 ||
-      (s->opt.coarse_mode != VX_COARSE_SWEEPS && s->opt.coarse_mode != VX_COARSE_DIRECT)) {
-    delete s;
+  if (o.max_batch < 1) o.max_batch = 1;
+  if (o.n_smooth < 1 || o.coarse_sweeps < 1 || o.max_levels < 1 ||
+      (o.coarse_mode != VX_COARSE_SWEEPS && o.coarse_mode != VX_COARSE_DIRECT))
     return VX_ERR_ARG;
+  // hierarchy rule of core/multigrid.py:141-148
+  std::vector<std::array<int, 3>> dims;
+  dims.push_back({nx, ny, nz});
+  while ((int)dims.size() < o.max_levels) {
+    auto d = dims.back();
+    const int mn = std::min(d[0], std::min(d[1], d[2]));
+    if (d[0] % 2 || d[1] % 2 || d[2] % 2) break;
+    if (!(mn / 2 >= o.min_elems / 2 && mn >= 2)) break;
+    dims.push_back({d[0] / 2, d[1] / 2, d[2] / 2});
   }
+  // every Chebyshev pass needs an interval [r lambda, lambda] with 0 < r < 1
+  // (r = 1 makes its width 0 and the step coefficients divide by it): check
+  // the ratio of each level's step count and of the coarsest sweeps
+  if (o.smoother == VX_SMOOTH_CHEBYSHEV) {
+    vx_solver probe;
+    probe.opt = o;
+    for (size_t li = 0; li < dims.size(); ++li) {
+      const int k = li + 1 == dims.size() ? o.coarse_sweeps : level_ns(&probe, (int)li);
+      const double r = cheb_ratio_for(&probe, k);
+      if (!(r > 0.0 && r < 1.0)) return VX_ERR_ARG;
+    }
+  }
+  if (vx_device_count() < 1) return VX_ERR_NO_DEVICE;
+  vx_solver* s = new vx_solver();
+  s->nx = nx; s->ny = ny; s->nz = nz; s->nu = nu;
+  s->opt = o;
   s->K64 = elem_const(nu);
   s->K32 = {(float)s->K64.k1, (float)s->K64.k2, (float)s->K64.b1, (float)s->K64.b2,
             (float)s->K64.t1};
   s->kd = elem_diag(nu);
   s->lam_max = elem_lambda_max(nu) / s->kd;
-  // hierarchy rule of core/multigrid.py:141-148
-  std::vector<std::array<int, 3>> dims;
-  dims.push_back({nx, ny, nz});
-  while ((int)dims.size() < s->opt.max_levels) {
-    auto d = dims.back();
-    const int mn = std::min(d[0], std::min(d[1], d[2]));
-    if (d[0] % 2 || d[1] % 2 || d[2] % 2) break;
-    if (!(mn / 2 >= s->opt.min_elems / 2 && mn >= 2)) break;
-    dims.push_back({d[0] / 2, d[1] / 2, d[2] / 2});
-  }
   long long maxpart = 0;
   for (auto& d : dims) {
     Level L;
@@ -1398,6 +1406,7 @@ int vx_set_fixed(vx_handle s, const int64_t* fixed, int64_t n_fixed) {
     HIPCHK(hipMemcpyAsync(L.mask, L.mask_fixed, L.Nn, hipMemcpyDeviceToDevice, s->stream));
   s->density_set = false;   // the effective masks are rebuilt by set_density
   s->cd_ok = false;
+  drop_graphs(s);           // captured with the previous coarsest-level treatment
   HIPCHK(hipStreamSynchronize(s->stream));
   s->fixed_set = true;
   return VX_OK;
@@ -1437,9 +1446,18 @@ int vx_set_density(vx_handle s, const double* E, const uint8_t* active, double e
     LAUNCHCHK();
   }
   s->density_set = true;
-  drop_graphs(s);   // the coarse mode may have changed
+  // The captured V-cycles hold the level buffers (unchanged here) and the
+  // coarsest-level treatment: whether the direct solve runs, its size and
+  // its buffers.  Only a change of those invalidates them.
+  const bool cd_ok0 = s->cd_ok;
+  const int cd_nf0 = s->cd_nf;
+  const void* const cd_bufs0[3] = {s->cd_inv, s->cd_pos, s->cd_idx};
   s->cd_ok = false;
   if (s->opt.coarse_mode == VX_COARSE_DIRECT) CHK(build_coarse_direct(s));
+  if (s->cd_ok != cd_ok0 ||
+      (s->cd_ok && (s->cd_nf != cd_nf0 || s->cd_inv != cd_bufs0[0] || s->cd_pos != cd_bufs0[1] ||
+                    s->cd_idx != cd_bufs0[2])))
+    drop_graphs(s);
   if (s->corr) return corr_features(s, s->corr_level);
   return VX_OK;
 }
@@ -1703,8 +1721,9 @@ int vx_solve_batch(vx_handle s, int32_t nrhs, const double* F, double* U, double
 // update runs over exactly the reference's reduced DOF set.
 // ---------------------------------------------------------------------------
 int vx_solve_jacobi(vx_handle s, const double* E, const uint8_t* active, const double* f,
-                    double* u, double tol, int32_t max_iter, vx_progress_cb cb, void* user,
-                    double* hist, int32_t hist_cap, vx_info* info) {
+                    double* u, const double* x0, double tol, int32_t max_iter,
+                    vx_progress_cb cb, void* user, double* hist, int32_t hist_cap,
+                    vx_info* info) {
   if (!s || !E || !f || !u) return fail(s, VX_ERR_ARG, "null Jacobi argument");
   s->nb = 1;
   if (!s->fixed_set) return fail(s, VX_ERR_ARG, "call vx_set_fixed() before solving");
@@ -1760,7 +1779,16 @@ int vx_solve_jacobi(vx_handle s, const double* E, const uint8_t* active, const d
     double* p = s->p64;
     double* Ap = s->Ap64;
     int np = 0;
-    HIPCHK(hipMemcpyAsync(r, s->fm64, sizeof(double) * n, hipMemcpyDeviceToDevice, s->stream));
+    if (x0) {
+      // warm start (core/fea.py:529-535): u = x0 on the reduced free DOFs,
+      // r = f - K u there (M_RESID masks K u to the same set)
+      hipLaunchKernelGGL((k_mask_copy<double, double>), dim3(EW_BLOCKS), dim3(EW_THREADS), 0,
+                         s->stream, J.Nn, J.mask, x0, u, nullptr);
+      LAUNCHCHK();
+      CHK((launch_tile_L<double, M_RESID, false>(s, J, u, s->fm64, r, nullptr, nullptr)));
+    } else {
+      HIPCHK(hipMemcpyAsync(r, s->fm64, sizeof(double) * n, hipMemcpyDeviceToDevice, s->stream));
+    }
     hipLaunchKernelGGL(k_diag_precond, dim3(EW_BLOCKS), dim3(EW_THREADS), 0, s->stream, J.Nn,
                        J.mask, s->jac_minv, r, z, s->partial);
     LAUNCHCHK();
