@@ -214,6 +214,19 @@ int vx_strain_energy(vx_handle h, const double* u_dev, const uint8_t* active_dev
 int vx_drop_islands(vx_handle h, const uint8_t* active_dev, const int64_t* fixed_dofs_host,
                     int64_t n_fixed, uint8_t* kept_dev, int64_t* n_removed_host);
 
+/* vx_drop_islands without any device allocation.  It has no counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * next to HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same result as vx_drop_islands; fixed_dofs_dev is a DEVICE
+ * array of n_fixed DOF ids (checked on the device: an id outside the grid is
+ * VX_ERR_ARG).  Its working memory (per-node flags, counters) is one static
+ * device array per GPU, held until the process ends, for grids of at most
+ * 2^25 = 33,554,432 nodes (32 MiB; 320^3 elements); a larger grid is
+ * VX_ERR_ARG.  Calls that overlap on one GPU share that array and race. */
+int vx_drop_islands_noalloc(vx_handle h, const uint8_t* active_dev,
+                            const int64_t* fixed_dofs_dev, int64_t n_fixed, uint8_t* kept_dev,
+                            int64_t* n_removed_host);
+
 /* Timing helper: average duration (ms) of `reps` back-to-back launches of the
  * level-l operator kernel (kind 0 = masked matvec, 1 = damped-Jacobi sweep,
  * 2 = Chebyshev smoothing step) in the given precision, measured with HIP

@@ -109,6 +109,7 @@ SIGNATURES = {
     "vx_strain_energy": (ctypes.c_int, [_P, _P, _P, _P]),
     "vx_post_fields": (ctypes.c_int, [_P, _P, _P, _P, _D, _D, _P, _P, _P, _P]),
     "vx_drop_islands": (ctypes.c_int, [_P, _P, _P, _I64, _P, ctypes.POINTER(_I64)]),
+    "vx_drop_islands_noalloc": (ctypes.c_int, [_P, _P, _P, _I64, _P, ctypes.POINTER(_I64)]),
     "vx_time_kernel": (ctypes.c_int, [_P, _I32, _I32, _I32, _I32, ctypes.POINTER(_D)]),
     "vx_element_apply_host": (None, [_D, _P, _P]),
     "vx_autotune": (ctypes.c_int, [_P, _I32, _I32]),

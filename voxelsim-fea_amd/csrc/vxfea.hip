@@ -14,6 +14,7 @@
 #include <cstddef>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -2070,6 +2071,153 @@ int vx_drop_islands(vx_handle s, const uint8_t* active, const int64_t* fixed, in
     *n_removed = (int64_t)(hc[0] - hc[1]);
   }
   done();
+  return VX_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Island removal without device allocation (vx_drop_islands_noalloc).
+//
+// The per-call buffers of vx_drop_islands -- the per-node "touches a fixed
+// DOF" flags, the changed flag of a dilation pass, the two element counts and
+// an argument-error flag -- live in one file-scope device array, `scratch`,
+// sized for grids of at most NOALLOC_MAX_NODES nodes.  Every GPU the library
+// runs on holds its own copy until the process ends.  Host code never uses
+// the array's name (a host-side stand-in): the device address is resolved
+// once per device with hipGetSymbolAddress and passed to the kernels.  All
+// calls on one GPU share the array and it keeps the last call's contents, so
+// every call first clears what must start at zero, and calls that overlap on
+// one GPU race.
+// ---------------------------------------------------------------------------
+constexpr long long NOALLOC_MAX_NODES = 1LL << 25;   // 321^3 nodes: grids up to 320^3
+constexpr int NOALLOC_HEAD = 256;                    // changed, error, two counts
+static __device__ __align__(256) uint8_t scratch[NOALLOC_HEAD + NOALLOC_MAX_NODES];
+
+namespace {
+
+struct NoallocHead {
+  int changed;
+  int error;
+  unsigned long long count[2];   // active elements, kept elements
+};
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex noalloc_mu;
+uint8_t* noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// device address of `scratch` on the current device (resolved on first use)
+int noalloc_scratch(vx_solver* s, uint8_t** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+    return fail(s, VX_ERR_ARG, "device index beyond the no-allocation table");
+  std::lock_guard<std::mutex> lock(noalloc_mu);
+  if (!noalloc_ptr[dev]) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    noalloc_ptr[dev] = static_cast<uint8_t*>(p);
+  }
+  *out = noalloc_ptr[dev];
+  return VX_OK;
+}
+
+// clear the head and the first nn node flags
+__global__ void k_noalloc_clear(uint8_t* __restrict__ sc, long long nn) {
+  NoallocHead* h = reinterpret_cast<NoallocHead*>(sc);
+  uint8_t* flags = sc + NOALLOC_HEAD;
+  const long long i0 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i0 == 0) {
+    h->changed = 0;
+    h->error = 0;
+    h->count[0] = h->count[1] = 0ULL;
+  }
+  for (long long i = i0; i < nn; i += (long long)gridDim.x * blockDim.x) flags[i] = 0;
+}
+
+// flags[d / 3] = 1 for every fixed DOF id d; an id outside [0, 3 nn) sets error
+__global__ void k_noalloc_mark_fixed(long long n_fixed, const int64_t* __restrict__ fixed,
+                                     long long nn, uint8_t* __restrict__ sc) {
+  NoallocHead* h = reinterpret_cast<NoallocHead*>(sc);
+  uint8_t* flags = sc + NOALLOC_HEAD;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n_fixed;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long d = fixed[i];
+    if (d < 0 || d >= 3 * nn) atomicOr(&h->error, 1);
+    else flags[d / 3] = 1;
+  }
+}
+
+__global__ void k_noalloc_reset_changed(uint8_t* sc) {
+  reinterpret_cast<NoallocHead*>(sc)->changed = 0;
+}
+
+__global__ void k_copy_u8(long long n, const uint8_t* __restrict__ a, uint8_t* __restrict__ b) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
+int noalloc_read_head(vx_solver* s, const uint8_t* sc, NoallocHead* h) {
+  HIPCHK(hipMemcpyAsync(h, sc, sizeof(NoallocHead), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return VX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vx_drop_islands_noalloc(vx_handle s, const uint8_t* active, const int64_t* fixed_dev,
+                            int64_t n_fixed, uint8_t* kept, int64_t* n_removed) {
+  if (!s || !active || !kept || n_fixed < 0 || (n_fixed > 0 && !fixed_dev))
+    return fail(s, VX_ERR_ARG, "bad island arguments");
+  Level& L = s->lv[0];
+  if (L.Nn > NOALLOC_MAX_NODES)
+    return fail(s, VX_ERR_ARG, "grid of " + std::to_string(L.Nn) +
+                                   " nodes exceeds the no-allocation limit of " +
+                                   std::to_string(NOALLOC_MAX_NODES) + " nodes");
+  uint8_t* sc = nullptr;
+  CHK(noalloc_scratch(s, &sc));
+  NoallocHead* dh = reinterpret_cast<NoallocHead*>(sc);
+  uint8_t* flags = sc + NOALLOC_HEAD;
+  hipLaunchKernelGGL(k_noalloc_clear, dim3(ew_grid(L.Nn)), dim3(EW_THREADS), 0, s->stream, sc,
+                     L.Nn);
+  if (n_fixed > 0)
+    hipLaunchKernelGGL(k_noalloc_mark_fixed, dim3(ew_grid(n_fixed)), dim3(EW_THREADS), 0,
+                       s->stream, (long long)n_fixed, fixed_dev, L.Nn, sc);
+  LAUNCHCHK();
+  NoallocHead h{};
+  CHK(noalloc_read_head(s, sc, &h));
+  if (h.error) return fail(s, VX_ERR_ARG, "fixed DOF out of range");
+  hipLaunchKernelGGL(k_island_seed, dim3(ew_grid(L.Ne)), dim3(EW_THREADS), 0, s->stream, L.nx,
+                     L.ny, L.nz, active, flags, kept);
+  LAUNCHCHK();
+  long long passes = 0;
+  do {
+    hipLaunchKernelGGL(k_noalloc_reset_changed, dim3(1), dim3(1), 0, s->stream, sc);
+    hipLaunchKernelGGL(k_island_grow, dim3(ew_grid((long long)L.nx * L.ny)), dim3(EW_THREADS), 0,
+                       s->stream, L.nx, L.ny, L.nz, active, kept, &dh->changed);
+    LAUNCHCHK();
+    CHK(noalloc_read_head(s, sc, &h));
+    if (++passes > 4 * (L.Ne + 1)) return fail(s, VX_ERR_HIP, "island flood fill did not terminate");
+  } while (h.changed);
+  hipLaunchKernelGGL(k_count_u8, dim3(ew_grid(L.Ne)), dim3(EW_THREADS), 0, s->stream, L.Ne, active,
+                     &dh->count[0]);
+  hipLaunchKernelGGL(k_count_u8, dim3(ew_grid(L.Ne)), dim3(EW_THREADS), 0, s->stream, L.Ne, kept,
+                     &dh->count[1]);
+  LAUNCHCHK();
+  CHK(noalloc_read_head(s, sc, &h));
+  if (h.count[1] == 0) {
+    // nothing reaches a support: leave the part untouched (core/fea.py:220-225)
+    hipLaunchKernelGGL(k_copy_u8, dim3(ew_grid(L.Ne)), dim3(EW_THREADS), 0, s->stream, L.Ne,
+                       active, kept);
+    LAUNCHCHK();
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (n_removed) *n_removed = 0;
+  } else if (n_removed) {
+    *n_removed = (int64_t)(h.count[0] - h.count[1]);
+  }
   return VX_OK;
 }
 
