@@ -168,6 +168,47 @@ int vx_solve_jacobi(vx_handle h, const double* E_dev, const uint8_t* active_dev,
                     int32_t max_iter, vx_progress_cb cb, void* user, double* hist_host,
                     int32_t hist_cap, vx_info* info);
 
+/* ---------------------------------------------------------------------------
+ * Parts of uneven size in one launch per operation.  The reference solves
+ * each part with its own VoxelFEA / MGSolver (core/fea.py:244-305,
+ * core/multigrid.py:439-583); here every part keeps its own grid, hierarchy,
+ * fixed DOFs, density and PCG recurrence (its own scalars, stall guard and
+ * convergence), while each operator, smoother, transfer and reduction
+ * launch covers the parts of a level, and a converged part's share of a
+ * launch returns at once.  Arrays are the parts' arrays packed one after
+ * another in part order: elements (E, active) in the reference's element
+ * order per part, DOFs (F, U) full-grid per part.
+ * ------------------------------------------------------------------------- */
+typedef struct vx_parts* vx_parts_handle;
+
+/* nparts grids of element dims dims_host [nparts][3] (host), one Poisson
+ * ratio, options as vx_create (coarse_mode VX_COARSE_SWEEPS, no flexible
+ * update; max_batch is ignored).  Each part gets the hierarchy rule of
+ * core/multigrid.py:141-148. */
+int vx_parts_create(int32_t nparts, const int32_t* dims_host, double nu, const vx_options* opt,
+                    vx_parts_handle* out);
+void vx_parts_destroy(vx_parts_handle h);
+const char* vx_parts_last_error(vx_parts_handle h);
+int vx_parts_set_stream(vx_parts_handle h, void* hip_stream);
+/* number of levels of one part's hierarchy */
+int vx_parts_num_levels(vx_parts_handle h, int32_t part);
+/* MGSolver._build_free per part: fixed_dofs_host holds every part's DOF ids
+ * one part after another, counts_host [nparts] how many each */
+int vx_parts_set_fixed(vx_parts_handle h, const int64_t* fixed_dofs_host,
+                       const int64_t* counts_host);
+/* MGSolver.set_density per part (the void rule of vx_set_density; e_min < 0:
+ * the active-DOF system) */
+int vx_parts_set_density(vx_parts_handle h, const double* E_dev, const uint8_t* active_dev,
+                         double e_min);
+/* Every part solved at once, each to its own tol with the recurrence of its
+ * own vx_solve (FP64) or iterative refinement (VX_MIXED).  hist_host (may be
+ * NULL) is [nparts][hist_cap]; infos (may be NULL) one vx_info per part.
+ * Returns VX_ERR_STALL if a part's stall guard fired (FP64; the message names
+ * the part), VX_ERR_NOT_CONVERGED if one did not converge. */
+int vx_solve_parts(vx_parts_handle h, const double* F_dev, double* U_dev, double tol,
+                   int32_t max_iter, vx_progress_cb cb, void* user, double* hist_host,
+                   int32_t hist_cap, vx_info* infos);
+
 /* Single-operator entry points for parity tests.  All float64, level-l sized.
  * vx_matvec: MGSolver._apply  (masked K u, core/multigrid.py:318-341)
  * vx_smooth: MGSolver._smooth (iters damped-Jacobi sweeps, :367-402; with

@@ -13,7 +13,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("VX_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = [os.path.join(HERE, "csrc", f) for f in ("vxfea.hip", "post.hip", "net.hip", "inside.hip")]
-DEPS = [os.path.join(HERE, "csrc", f) for f in ("kernels.h", "elem.h")] + [
+DEPS = [os.path.join(HERE, "csrc", f) for f in ("kernels.h", "elem.h", "parts.h")] + [
     os.path.join(ROOT, "include", "vxfea.h")]
 OUT = os.path.join(HERE, "lib", "libvxfea.so")
 

@@ -99,6 +99,16 @@ SIGNATURES = {
                                       ctypes.POINTER(vx_info)]),
     "vx_solve_jacobi": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _D, _I32, PROGRESS_CB, _P, _P,
                                        _I32, ctypes.POINTER(vx_info)]),
+    "vx_parts_create": (ctypes.c_int, [_I32, _P, _D, ctypes.POINTER(vx_options),
+                                       ctypes.POINTER(_P)]),
+    "vx_parts_destroy": (None, [_P]),
+    "vx_parts_last_error": (ctypes.c_char_p, [_P]),
+    "vx_parts_set_stream": (ctypes.c_int, [_P, _P]),
+    "vx_parts_num_levels": (ctypes.c_int, [_P, _I32]),
+    "vx_parts_set_fixed": (ctypes.c_int, [_P, _P, _P]),
+    "vx_parts_set_density": (ctypes.c_int, [_P, _P, _P, _D]),
+    "vx_solve_parts": (ctypes.c_int, [_P, _P, _P, _D, _I32, PROGRESS_CB, _P, _P, _I32,
+                                      ctypes.POINTER(vx_info)]),
     "vx_matvec": (ctypes.c_int, [_P, _I32, _I32, _P, _P]),
     "vx_smooth": (ctypes.c_int, [_P, _I32, _I32, _P, _P, _I32, _P]),
     "vx_restrict": (ctypes.c_int, [_P, _I32, _P, _P]),

@@ -68,6 +68,52 @@ def padded_dims(dims, max_levels=6, overhead=1.3):
     return tuple(int(d) for d in dims)
 
 
+# A part grown with void elements to coarsenable dims (padded_dims): its node
+# and element arrays embedded at the low corner of the padded grid.
+def pad_dof_ids(ids, dims, pdims):
+    """DOF ids of a dims grid -> the same DOFs of the padded pdims grid."""
+    ids = np.asarray(ids, dtype=np.int64)
+    if tuple(dims) == tuple(pdims):
+        return ids
+    n, c = ids // 3, ids % 3
+    lx, ly = dims[0] + 1, dims[1] + 1
+    ix, iy, iz = n % lx, (n // lx) % ly, n // (lx * ly)
+    return 3 * (ix + (pdims[0] + 1) * (iy + (pdims[1] + 1) * iz)) + c
+
+
+def pad_elems(t, dims, pdims):
+    """Per-element device tensor (reference element order) -> padded grid, 0 outside."""
+    if tuple(dims) == tuple(pdims):
+        return t
+    out = _torch().zeros(tuple(pdims), dtype=t.dtype, device=t.device)
+    out[:dims[0], :dims[1], :dims[2]] = t.view(*dims)
+    return out.view(-1)
+
+
+def _node_view(v, dims):
+    nx, ny, nz = dims
+    return v.view(-1, nz + 1, ny + 1, nx + 1, 3)
+
+
+def pad_nodes(v, dims, pdims):
+    """[..., ndof of dims] device tensor -> [..., ndof of pdims] (zeros outside)."""
+    if tuple(dims) == tuple(pdims):
+        return v
+    nx, ny, nz = dims
+    px, py, pz = pdims
+    out = _torch().zeros(v.shape[:-1] + (3 * (px + 1) * (py + 1) * (pz + 1),), dtype=v.dtype,
+                         device=v.device)
+    _node_view(out, pdims)[:, :nz + 1, :ny + 1, :nx + 1] = _node_view(v.contiguous(), dims)
+    return out
+
+
+def unpad_nodes(vp, dims, pdims, out):
+    """The part's nodes of a padded-grid vector vp into out."""
+    nx, ny, nz = dims
+    out.view(-1, nz + 1, ny + 1, nx + 1, 3).copy_(
+        _node_view(vp, pdims)[:, :nz + 1, :ny + 1, :nx + 1])
+
+
 def _require_gpu(plan, what):
     if not plan.on_gpu:
         raise RuntimeError(
@@ -165,44 +211,17 @@ class MGSolver:
 
     # -- padding ------------------------------------------------------------
     def _pad_dof_ids(self, ids):
-        if not self.padded:
-            return ids
-        n, c = ids // 3, ids % 3
-        lx, ly = self.nx + 1, self.ny + 1
-        ix, iy, iz = n % lx, (n // lx) % ly, n // (lx * ly)
-        px, py, _ = self._pdims
-        return 3 * (ix + (px + 1) * (iy + (py + 1) * iz)) + c
+        return pad_dof_ids(ids, (self.nx, self.ny, self.nz), self._pdims)
 
     def _pad_elems(self, a, dtype):
-        t = _dev(a, dtype)
-        if not self.padded:
-            return t
-        px, py, pz = self._pdims
-        out = _torch().zeros((px, py, pz), dtype=t.dtype, device=t.device)
-        out[:self.nx, :self.ny, :self.nz] = t.view(self.nx, self.ny, self.nz)
-        return out.view(-1)
-
-    def _node_view(self, v, dims):
-        nx, ny, nz = dims
-        return v.view(-1, nz + 1, ny + 1, nx + 1, 3)
+        return pad_elems(_dev(a, dtype), (self.nx, self.ny, self.nz), self._pdims)
 
     def _pad_nodes(self, v):
-        """[..., user ndof] -> [..., padded ndof] (zeros outside the part)."""
-        if not self.padded:
-            return v
-        lead = v.shape[:-1]
-        px, py, pz = self._pdims
-        out = _torch().zeros(lead + (3 * (px + 1) * (py + 1) * (pz + 1),), dtype=v.dtype,
-                             device=v.device)
-        self._node_view(out, self._pdims)[:, :self.nz + 1, :self.ny + 1, :self.nx + 1] = \
-            self._node_view(v.contiguous(), (self.nx, self.ny, self.nz))
-        return out
+        return pad_nodes(v, (self.nx, self.ny, self.nz), self._pdims)
 
     def _unpad_nodes(self, vp, out):
-        if not self.padded:
-            return
-        out.view(-1, self.nz + 1, self.ny + 1, self.nx + 1, 3).copy_(
-            self._node_view(vp, self._pdims)[:, :self.nz + 1, :self.ny + 1, :self.nx + 1])
+        if self.padded:
+            unpad_nodes(vp, (self.nx, self.ny, self.nz), self._pdims, out)
 
     # -- plumbing ----------------------------------------------------------
     def _stream_bind(self):
@@ -515,3 +534,142 @@ class MGSolver:
         self._check(self._lib.vx_time_kernel(self._h, {"matvec": 0, "jacobi": 1, "chebyshev": 2}[kind], level,
                                              _PRECISION[precision], reps, ctypes.byref(out)))
         return out.value
+
+
+class PartsSolver:
+    """Many parts of uneven size solved together (``vx_solve_parts``).
+
+    The reference solves each part with its own ``VoxelFEA`` / ``MGSolver``
+    (core/fea.py:244-305, core/multigrid.py:439-583).  Here every part keeps
+    its own grid, hierarchy, fixed DOFs, density and PCG recurrence -- it stops
+    at its own tolerance with its own iteration count -- while each operator,
+    smoother, transfer and reduction launch covers all the parts of a level.
+
+    ``parts``: a list of (nx, ny, nz); ``fixed``: a list of each part's fixed
+    DOF ids.  Options as ``MGSolver`` (``pad=True`` grows each part with void
+    elements to coarsenable dims, which needs ``system="active"``).  The tile
+    kernels' z-chunks follow one rule over the whole group (``z_chunk`` fixes
+    them); ``autotune`` and ``graphs`` are accepted for ``MGSolver``'s option
+    sets and have no effect here."""
+
+    def __init__(self, parts, fixed, nu=0.3, n_smooth=2, omega=0.6, min_elems=4, max_levels=6,
+                 compute_mode="AUTO", cpu_threads=0, verbose=False, precision="fp64",
+                 coarse_sweeps=30, inner_tol=1e-5, max_outer=20, z_chunk=0, smoother="jacobi",
+                 cheb_ratio=0.3, system="full", e_coarsen="mean", pad=False, autotune=False,
+                 level_smooth=None, graphs=True, cheb_ratio_power=0):
+        self.dims = [tuple(int(v) for v in d) for d in parts]
+        if len(fixed) != len(self.dims):
+            raise ValueError("one fixed-DOF list per part")
+        if system not in ("full", "active"):
+            raise ValueError("system must be 'full' (voids at E*e_min) or 'active'")
+        if pad and system != "active":
+            raise ValueError("pad=True needs system='active'")
+        self.system, self.precision = system, precision
+        self.pdims = [padded_dims(d, max_levels) if pad else d for d in self.dims]
+        ndof = sum(3 * (d[0] + 1) * (d[1] + 1) * (d[2] + 1) for d in self.pdims)
+        self.plan = compute_plan.choose(ndof, mode=compute_mode, cpu_threads=cpu_threads,
+                                        verbose=verbose)
+        _require_gpu(self.plan, "PartsSolver")
+        self._lib = _native.load()
+        opt = _native.default_options(
+            max_levels=max_levels, min_elems=min_elems, n_smooth=n_smooth, omega=omega,
+            coarse_sweeps=coarse_sweeps, precision=_PRECISION[precision], inner_tol=inner_tol,
+            max_outer=max_outer, z_chunk=z_chunk, smoother=_SMOOTHER[smoother],
+            cheb_ratio=cheb_ratio, e_coarsen={"mean": 0, "geometric": 1, "harmonic": 2}[e_coarsen],
+            level_smooth=_level_smooth(level_smooth), cheb_ratio_power=int(cheb_ratio_power))
+        dims = np.ascontiguousarray(np.asarray(self.pdims, dtype=np.int32).reshape(-1))
+        h = ctypes.c_void_p()
+        rc = self._lib.vx_parts_create(len(self.pdims), dims.ctypes.data_as(ctypes.c_void_p),
+                                       float(nu), ctypes.byref(opt), ctypes.byref(h))
+        _native.check(rc)
+        self._h = h
+        self._stream_bind()
+        self.levels = [int(self._lib.vx_parts_num_levels(h, i)) for i in range(len(self.dims))]
+        ids = [pad_dof_ids(np.asarray(f, dtype=np.int64).ravel(), d, p)
+               for f, d, p in zip(fixed, self.dims, self.pdims)]
+        counts = np.asarray([i.size for i in ids], dtype=np.int64)
+        flat = np.ascontiguousarray(np.concatenate(ids) if ids else np.zeros(0, np.int64))
+        self._check(self._lib.vx_parts_set_fixed(h, flat.ctypes.data_as(ctypes.c_void_p),
+                                                 counts.ctypes.data_as(ctypes.c_void_p)))
+        self.last_parts = []
+        self.last_solve_ms = float("nan")
+
+    def _stream_bind(self):
+        torch = _torch()
+        self._lib.vx_parts_set_stream(self._h,
+                                      ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+
+    def _check(self, rc, allow=()):
+        if rc == _native.VX_OK or rc in allow:
+            return rc
+        msg = self._lib.vx_parts_last_error(self._h).decode()
+        if rc == _native.VX_ERR_STALL:
+            raise _native.StallError(rc, msg)
+        raise _native.NativeError(rc, msg)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            self._lib.vx_parts_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_density(self, E_list, active_list=None, e_min=1e-9):
+        """Per-part E (E*vsize per element) and active masks, packed on the
+        device; with system="active" voids carry no stiffness."""
+        torch = _torch()
+        act = active_list or [None] * len(self.dims)
+        E = [pad_elems(_dev(e), d, p) for e, d, p in zip(E_list, self.dims, self.pdims)]
+        A = [pad_elems(_dev(np.ones(int(np.prod(d)), np.uint8) if a is None else a, torch.uint8),
+                       d, p) for a, d, p in zip(act, self.dims, self.pdims)]
+        self._E = torch.cat(E)
+        self._A = torch.cat(A)
+        self._stream_bind()
+        em = -1.0 if self.system == "active" else float(e_min)
+        self._check(self._lib.vx_parts_set_density(self._h, _native.ptr(self._E),
+                                                   _native.ptr(self._A), em))
+
+    def solve_device(self, F_list, tol=1e-4, max_cg=None, progress_cb=None, hist_cap=2000):
+        """Solve every part; F_list: device float64 vectors, one per part.
+        Returns the list of u (device tensors, each part's own dims) and the
+        native status; per-part statistics go to ``last_parts``."""
+        torch = _torch()
+        n = len(self.dims)
+        F = torch.cat([pad_nodes(_dev(f), d, p) for f, d, p in zip(F_list, self.dims, self.pdims)])
+        U = torch.empty_like(F)
+        hist = np.zeros(n * hist_cap)
+        infos = (_native.vx_info * n)()
+
+        def _cb(it, ratio, t, _user):
+            progress_cb(int(it), float(ratio), float(t))
+
+        cb = _native.PROGRESS_CB(_cb) if progress_cb is not None else _native.PROGRESS_CB()
+        self._stream_bind()
+        rc = self._lib.vx_solve_parts(self._h, _native.ptr(F), _native.ptr(U), float(tol),
+                                      int(max_cg or 0), cb, None,
+                                      hist.ctypes.data_as(ctypes.c_void_p), hist_cap, infos)
+        out, off = [], 0
+        for d, p in zip(self.dims, self.pdims):
+            m = 3 * (p[0] + 1) * (p[1] + 1) * (p[2] + 1)
+            up = U[off:off + m]
+            if tuple(d) == tuple(p):
+                out.append(up)
+            else:
+                u = torch.empty(3 * (d[0] + 1) * (d[1] + 1) * (d[2] + 1), dtype=U.dtype,
+                                device=U.device)
+                unpad_nodes(up, d, p, u)
+                out.append(u)
+            off += m
+        self.last_parts = [{"iters": i.iters, "converged": bool(i.converged),
+                            "outer_iters": i.outer_iters, "resid_ratio": i.resid_ratio,
+                            "true_resid_ratio": i.true_resid_ratio, "levels": i.n_levels}
+                           for i in infos]
+        self.parts_history = [list(hist[q * hist_cap:q * hist_cap + min(infos[q].iters, hist_cap)])
+                              for q in range(n)]
+        self.last_solve_ms = infos[0].solve_ms
+        self._check(rc, allow=(_native.VX_ERR_NOT_CONVERGED,))
+        return out, rc

@@ -157,26 +157,16 @@ __device__ inline double block_sum(double v, double* sred) {
 // FP32: at least 4 waves per SIMD (at most 128 registers: no spills with the
 // layer loop unrolled by two; forcing 5 spills and is slower); FP64 keeps
 // what its registers give (forcing 3 spills and is slower)
+// The tile work of one work-group: tile vt of the grid described by a, on the
+// vectors au / ab / aup / aout of one right-hand side (or part); its DOT
+// partial goes to a.partial[slot].
 template <typename T, int MODE, bool DOT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
-void k_tile(TileArgs<T> a) {
+__device__ __forceinline__ void tile_run(const TileArgs<T>& a, int vt, int slot,
+                                         const T* __restrict__ au, const T* __restrict__ ab,
+                                         const T* __restrict__ aup, T* __restrict__ aout) {
   __shared__ T sP[2][3 * FOOT * FOOT];
   __shared__ T sQ[2][4][TILE][TILE + 1];
   __shared__ double sred[NT / 64];
-
-  // XCD-aware tile numbering over (right-hand side, tile)
-  const int G = a.gx * a.gy * a.gz;
-  const int GT = G * a.nb;
-  const int per = (GT + 7) >> 3;
-  const int v = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
-  if (v >= GT) return;
-  const int rhs = v / G;
-  const int vt = v - rhs * G;
-  const long long voff = (long long)rhs * a.vstride;
-  const T* __restrict__ au = a.u + voff;
-  const T* __restrict__ ab = a.b ? a.b + voff : nullptr;
-  const T* __restrict__ aup = a.up ? a.up + voff : nullptr;
-  T* __restrict__ aout = a.out + voff;
   const int bx = vt % a.gx;
   const int byz = vt / a.gx;
   const int by = byz % a.gy;
@@ -400,8 +390,24 @@ void k_tile(TileArgs<T> a) {
 
   if (DOT) {
     const double s = block_sum(dacc, sred);
-    if (tid == 0) a.partial[v] = s;
+    if (tid == 0) a.partial[slot] = s;
   }
+}
+
+template <typename T, int MODE, bool DOT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
+void k_tile(TileArgs<T> a) {
+  // XCD-aware tile numbering over (right-hand side, tile)
+  const int G = a.gx * a.gy * a.gz;
+  const int GT = G * a.nb;
+  const int per = (GT + 7) >> 3;
+  const int v = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+  if (v >= GT) return;
+  const int rhs = v / G;
+  const int vt = v - rhs * G;
+  const long long voff = (long long)rhs * a.vstride;
+  tile_run<T, MODE, DOT>(a, vt, v, a.u + voff, a.b ? a.b + voff : nullptr,
+                         a.up ? a.up + voff : nullptr, a.out + voff);
 }
 
 // Inverse Jacobi diagonal per node of a level: 1/(kd * sum of the <= 8
@@ -456,13 +462,11 @@ struct CoarseCoef {
 };
 
 template <typename T>
-__global__ __launch_bounds__(1024) void k_coarse_smooth(int nx, int ny, int nz,
-                                                        const T* __restrict__ E,
-                                                        const uint8_t* __restrict__ mask,
-                                                        const T* __restrict__ minv,
-                                                        const T* __restrict__ b,
-                                                        T* __restrict__ out, ElemConst<T> K,
-                                                        CoarseCoef cc) {
+__device__ __forceinline__ void coarse_smooth_run(int nx, int ny, int nz, const T* __restrict__ E,
+                                                  const uint8_t* __restrict__ mask,
+                                                  const T* __restrict__ minv,
+                                                  const T* __restrict__ b, T* __restrict__ out,
+                                                  const ElemConst<T>& K, const CoarseCoef& cc) {
   constexpr int MAXN = coarse_max_nodes<T>(), MAXE = coarse_max_elems<T>();
   __shared__ T sye[MAXE * 24];
   __shared__ T sx[3 * MAXN], sxp[3 * MAXN], sb[3 * MAXN];
@@ -473,8 +477,6 @@ __global__ __launch_bounds__(1024) void k_coarse_smooth(int nx, int ny, int nz,
   const int Nn = Lx * Ly * (nz + 1);
   const int Ne = nx * ny * nz;
   const int tid = threadIdx.x, nt = blockDim.x;
-  b += (long long)blockIdx.x * 3 * Nn;
-  out += (long long)blockIdx.x * 3 * Nn;
   for (int n = tid; n < Nn; n += nt) {
     smk[n] = mask[n];
     smi[n] = minv[n];
@@ -590,6 +592,18 @@ __global__ __launch_bounds__(1024) void k_coarse_smooth(int nx, int ny, int nz,
     __syncthreads();
   }
   for (int i = tid; i < 3 * Nn; i += nt) out[i] = sx[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_coarse_smooth(int nx, int ny, int nz,
+                                                        const T* __restrict__ E,
+                                                        const uint8_t* __restrict__ mask,
+                                                        const T* __restrict__ minv,
+                                                        const T* __restrict__ b,
+                                                        T* __restrict__ out, ElemConst<T> K,
+                                                        CoarseCoef cc) {
+  const long long off = (long long)blockIdx.x * 3 * (nx + 1) * (ny + 1) * (nz + 1);
+  coarse_smooth_run<T>(nx, ny, nz, E, mask, minv, b + off, out + off, K, cc);
 }
 
 // ---------------------------------------------------------------------------
@@ -840,11 +854,9 @@ __global__ void k_expand3(long long Nn, const double* __restrict__ a, double* __
 // FP32 with 16-byte aligned rows (vec4): four nodes per thread through
 // float4 / uint32 accesses, the tail node by node.
 template <typename T>
-__global__ void k_zero_step(long long Nn, const uint8_t* __restrict__ mask,
-                            const T* __restrict__ minv, const T* __restrict__ b,
-                            T* __restrict__ out, T c, int vec4) {
-  b += blockIdx.y * 3 * Nn;
-  out += blockIdx.y * 3 * Nn;
+__device__ __forceinline__ void zero_step_run(long long Nn, const uint8_t* __restrict__ mask,
+                                              const T* __restrict__ minv, const T* __restrict__ b,
+                                              T* __restrict__ out, T c, int vec4) {
   long long start = 0;
   if (std::is_same<T, float>::value && vec4) {
     const long long ng = Nn >> 2;
@@ -881,6 +893,14 @@ __global__ void k_zero_step(long long Nn, const uint8_t* __restrict__ mask,
   }
 }
 
+template <typename T>
+__global__ void k_zero_step(long long Nn, const uint8_t* __restrict__ mask,
+                            const T* __restrict__ minv, const T* __restrict__ b,
+                            T* __restrict__ out, T c, int vec4) {
+  const long long off = blockIdx.y * 3 * Nn;
+  zero_step_run<T>(Nn, mask, minv, b + off, out + off, c, vec4);
+}
+
 // Grid transfers marching in z.  One thread per column of the 2-D plane
 // grid, z split into chunks so that about TR_TARGET_GROUPS work-groups run;
 // the 1-D launch is numbered XCD-aware (blocks are dealt to the 8 XCDs
@@ -907,27 +927,20 @@ __device__ __forceinline__ bool tr_block(int total, int& v) {
 // below carried from the previous coarse plane as its own partial sum (the
 // weights are powers of 2, so that partial is exact to reuse).  With x1 set,
 // also the coarse level's first smoothing step from zero, w0 * minv * rc.
+// work-group v (of g.gxy * g.nchunk) of one right-hand side's restriction
 template <typename T>
-__global__ __launch_bounds__(TR_THREADS) void k_restrict_z(int Lcx, int Lcy, int Lcz, TrGrid g,
-                                                           int nb, const T* __restrict__ rf,
-                                                           const uint8_t* __restrict__ cmask,
-                                                           T* __restrict__ rc,
-                                                           const T* __restrict__ cminv, T w0,
-                                                           T* __restrict__ x1) {
-  int v;
-  if (!tr_block(g.gxy * g.nchunk * nb, v)) return;
+__device__ __forceinline__ void restrict_run(int Lcx, int Lcy, int Lcz, const TrGrid& g, int v,
+                                             const T* __restrict__ rf,
+                                             const uint8_t* __restrict__ cmask,
+                                             T* __restrict__ rc, const T* __restrict__ cminv,
+                                             T w0, T* __restrict__ x1) {
   const int bxy = v % g.gxy;
-  const int rest = v / g.gxy;
-  const int chunk = rest % g.nchunk;
-  const int rhs = rest / g.nchunk;
+  const int chunk = v / g.gxy;
   const int t = bxy * TR_THREADS + (int)threadIdx.x;
   const int jx = t % Lcx, jy = t / Lcx;
   if (jy >= Lcy) return;
   const int Lfx = 2 * Lcx - 1, Lfy = 2 * Lcy - 1, Lfz = 2 * Lcz - 1;
   const long long fplane = 3LL * Lfx * Lfy;
-  rf += (long long)rhs * fplane * Lfz;
-  rc += (long long)rhs * 3LL * Lcx * Lcy * Lcz;
-  if (x1) x1 += (long long)rhs * 3LL * Lcx * Lcy * Lcz;
   const int jz0 = chunk * g.per_chunk;
   const int jz1 = min(jz0 + g.per_chunk, Lcz);
   // x-y taps of this column (y outer, x inner): fine offsets and weights,
@@ -995,6 +1008,23 @@ __global__ __launch_bounds__(TR_THREADS) void k_restrict_z(int Lcx, int Lcy, int
   }
 }
 
+template <typename T>
+__global__ __launch_bounds__(TR_THREADS) void k_restrict_z(int Lcx, int Lcy, int Lcz, TrGrid g,
+                                                           int nb, const T* __restrict__ rf,
+                                                           const uint8_t* __restrict__ cmask,
+                                                           T* __restrict__ rc,
+                                                           const T* __restrict__ cminv, T w0,
+                                                           T* __restrict__ x1) {
+  int v;
+  if (!tr_block(g.gxy * g.nchunk * nb, v)) return;
+  const int per_rhs = g.gxy * g.nchunk;
+  const int rhs = v / per_rhs;
+  const long long fstride = 3LL * (2 * Lcx - 1) * (2 * Lcy - 1) * (2 * Lcz - 1);
+  const long long cstride = 3LL * Lcx * Lcy * Lcz;
+  restrict_run<T>(Lcx, Lcy, Lcz, g, v - rhs * per_rhs, rf + rhs * fstride, cmask,
+                  rc + rhs * cstride, cminv, w0, x1 ? x1 + rhs * cstride : nullptr);
+}
+
 // x += fine mask * (P e) (core/multigrid.py:418-424, 434) with one thread per
 // fine value (node, component) of a plane: a wave's fine loads and stores
 // cover one contiguous span (whole cache lines), its coarse loads a
@@ -1003,17 +1033,14 @@ __global__ __launch_bounds__(TR_THREADS) void k_restrict_z(int Lcx, int Lcy, int
 // once.  Each interpolant is the sum of its coarse values in (z, y, x) order
 // times the product of the 1/2 weights (a power of 2, so applying it last is
 // exact).
+// work-group v (of g.gxy * g.nchunk) of one right-hand side's prolongation
 template <typename T>
-__global__ __launch_bounds__(TR_THREADS) void k_prolong_f(int Lcx, int Lcy, int Lcz, TrGrid g,
-                                                          int nb, const T* __restrict__ ec,
-                                                          const uint8_t* __restrict__ fmask,
-                                                          T* __restrict__ x) {
-  int v;
-  if (!tr_block(g.gxy * g.nchunk * nb, v)) return;
+__device__ __forceinline__ void prolong_run(int Lcx, int Lcy, int Lcz, const TrGrid& g, int v,
+                                            const T* __restrict__ ec,
+                                            const uint8_t* __restrict__ fmask,
+                                            T* __restrict__ x) {
   const int bxy = v % g.gxy;
-  const int rest = v / g.gxy;
-  const int chunk = rest % g.nchunk;
-  const int rhs = rest / g.nchunk;
+  const int chunk = v / g.gxy;
   const int Lfx = 2 * Lcx - 1, Lfy = 2 * Lcy - 1, Lfz = 2 * Lcz - 1;
   const int fplane = Lfx * Lfy;   // nodes per fine plane
   const int f = bxy * TR_THREADS + (int)threadIdx.x;   // value index in a plane
@@ -1023,8 +1050,6 @@ __global__ __launch_bounds__(TR_THREADS) void k_prolong_f(int Lcx, int Lcy, int 
   const int cx = ix >> 1, cy = iy >> 1;
   const bool ox = ix & 1, oy = iy & 1;
   const long long cplane = 3LL * Lcx * Lcy;
-  ec += (long long)rhs * cplane * Lcz;
-  x += (long long)rhs * 3LL * fplane * Lfz;
   // the coarse taps of this value inside a coarse plane, (ky, kx) order
   const int t00 = 3 * (cy * Lcx + cx) + c;
   const int t01 = t00 + 3, t10 = t00 + 3 * Lcx, t11 = t10 + 3;
@@ -1097,6 +1122,20 @@ __global__ __launch_bounds__(TR_THREADS) void k_prolong_f(int Lcx, int Lcy, int 
       lo = s2;
     }
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(TR_THREADS) void k_prolong_f(int Lcx, int Lcy, int Lcz, TrGrid g,
+                                                          int nb, const T* __restrict__ ec,
+                                                          const uint8_t* __restrict__ fmask,
+                                                          T* __restrict__ x) {
+  int v;
+  if (!tr_block(g.gxy * g.nchunk * nb, v)) return;
+  const int per_rhs = g.gxy * g.nchunk;
+  const int rhs = v / per_rhs;
+  const long long cstride = 3LL * Lcx * Lcy * Lcz;
+  const long long fstride = 3LL * (2 * Lcx - 1) * (2 * Lcy - 1) * (2 * Lcz - 1);
+  prolong_run<T>(Lcx, Lcy, Lcz, g, v - rhs * per_rhs, ec + rhs * cstride, fmask, x + rhs * fstride);
 }
 
 // E of level l+1 = 2 * mean of the 8 children (scaled E carries 2^l;
@@ -1229,11 +1268,10 @@ enum {
 };
 
 constexpr int FIN_THREADS = 1024;
-__global__ __launch_bounds__(FIN_THREADS) void k_finalize(const double* __restrict__ partial, int n,
-                                                          int op, Scalars* s) {
+// the n partials of one right-hand side summed in a fixed order, then op
+__device__ __forceinline__ void finalize_run(const double* __restrict__ partial, int n, int op,
+                                             Scalars* s) {
   __shared__ double sred[FIN_THREADS / 64];
-  partial += (long long)blockIdx.x * n;
-  s += blockIdx.x;
   // four independent chains per thread keep four loads in flight
   double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
   const int bd = blockDim.x;
@@ -1279,6 +1317,11 @@ __global__ __launch_bounds__(FIN_THREADS) void k_finalize(const double* __restri
   }
 }
 
+__global__ __launch_bounds__(FIN_THREADS) void k_finalize(const double* __restrict__ partial, int n,
+                                                          int op, Scalars* s) {
+  finalize_run(partial + (long long)blockIdx.x * n, n, op, s + blockIdx.x);
+}
+
 __global__ void k_set_fnorm(Scalars* s, double v) { s[blockIdx.x].fnorm = v; }
 
 // frozen flags of all right-hand sides from a device int array
@@ -1306,16 +1349,9 @@ __device__ __forceinline__ void st4(double* p, const double v[4]) {
 
 // u += alpha p ; r -= alpha Ap ; partial sum r.r  (arrays 16-byte aligned)
 template <typename T>
-__global__ void k_cg_update(long long n, const Scalars* __restrict__ s, T* __restrict__ u,
-                            T* __restrict__ r, const T* __restrict__ p, const T* __restrict__ Ap,
-                            double* __restrict__ partial) {
-  __shared__ double sred[4];
-  const long long off = (long long)blockIdx.y * n;
-  u += off;
-  r += off;
-  p += off;
-  Ap += off;
-  const T a = (T)s[blockIdx.y].alpha;
+__device__ __forceinline__ double cg_update_run(long long n, T a, T* __restrict__ u,
+                                                T* __restrict__ r, const T* __restrict__ p,
+                                                const T* __restrict__ Ap) {
   double acc = 0.0;
   const long long n4 = n >> 2;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n4;
@@ -1341,6 +1377,16 @@ __global__ void k_cg_update(long long n, const Scalars* __restrict__ s, T* __res
     r[i] = rn;
     acc += (double)rn * (double)rn;
   }
+  return acc;
+}
+
+template <typename T>
+__global__ void k_cg_update(long long n, const Scalars* __restrict__ s, T* __restrict__ u,
+                            T* __restrict__ r, const T* __restrict__ p, const T* __restrict__ Ap,
+                            double* __restrict__ partial) {
+  __shared__ double sred[4];
+  const long long off = (long long)blockIdx.y * n;
+  double acc = cg_update_run<T>(n, (T)s[blockIdx.y].alpha, u + off, r + off, p + off, Ap + off);
   acc = block_sum(acc, sred);
   if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = acc;
 }
@@ -1353,16 +1399,11 @@ __global__ void k_cg_update(long long n, const Scalars* __restrict__ s, T* __res
 // per thread through float4 / uint32 accesses (every vector 16-byte aligned
 // per right-hand side), the < 4 tail nodes one by one.  Grid (EW blocks,
 // rhs); partial[rhs * gridDim.x + block].
-__global__ void k_cg_update_x1(long long Nn, const Scalars* __restrict__ s, float* __restrict__ r,
-                               const float* __restrict__ Ap, double* __restrict__ partial,
-                               const uint8_t* __restrict__ mask, const float* __restrict__ minv,
-                               float c, float* __restrict__ x1) {
-  __shared__ double sred[4];
-  const long long off = (long long)blockIdx.y * 3 * Nn;
-  r += off;
-  Ap += off;
-  x1 += off;
-  const float a = (float)s[blockIdx.y].alpha;
+__device__ __forceinline__ double cg_update_x1_run(long long Nn, float a, float* __restrict__ r,
+                                                   const float* __restrict__ Ap,
+                                                   const uint8_t* __restrict__ mask,
+                                                   const float* __restrict__ minv, float c,
+                                                   float* __restrict__ x1) {
   double acc = 0.0;
   const long long ng = Nn >> 2;
   for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < ng;
@@ -1404,18 +1445,25 @@ __global__ void k_cg_update_x1(long long Nn, const Scalars* __restrict__ s, floa
       x1[i] = ((mk >> k) & 1u) ? w * rn : 0.f;
     }
   }
+  return acc;
+}
+
+__global__ void k_cg_update_x1(long long Nn, const Scalars* __restrict__ s, float* __restrict__ r,
+                               const float* __restrict__ Ap, double* __restrict__ partial,
+                               const uint8_t* __restrict__ mask, const float* __restrict__ minv,
+                               float c, float* __restrict__ x1) {
+  __shared__ double sred[4];
+  const long long off = (long long)blockIdx.y * 3 * Nn;
+  double acc = cg_update_x1_run(Nn, (float)s[blockIdx.y].alpha, r + off, Ap + off, mask, minv, c,
+                                x1 + off);
   acc = block_sum(acc, sred);
   if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = acc;
 }
 
 // p = z + beta p per right-hand side (blockIdx.y = rhs, n values each)
 template <typename T>
-__global__ void k_p_update_b(long long n, const Scalars* __restrict__ s, const T* __restrict__ z,
-                             T* __restrict__ p) {
-  const long long off = (long long)blockIdx.y * n;
-  z += off;
-  p += off;
-  const T beta = (T)s[blockIdx.y].beta;
+__device__ __forceinline__ void p_update_b_run(long long n, T beta, const T* __restrict__ z,
+                                               T* __restrict__ p) {
   const long long n4 = n >> 2;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n4;
        i += (long long)gridDim.x * blockDim.x) {
@@ -1432,20 +1480,22 @@ __global__ void k_p_update_b(long long n, const Scalars* __restrict__ s, const T
   }
 }
 
+template <typename T>
+__global__ void k_p_update_b(long long n, const Scalars* __restrict__ s, const T* __restrict__ z,
+                             T* __restrict__ p) {
+  const long long off = (long long)blockIdx.y * n;
+  p_update_b_run<T>(n, (T)s[blockIdx.y].beta, z + off, p + off);
+}
+
 // The deferred solution update of FP32 PCG with the direction update: u +=
 // alpha p with the previous step's alpha (this step's F_ALPHA has not run
 // yet; 0 before the first step), then p = z + beta p; the arithmetic of
 // k_cg_update's u update followed by k_p_update_b.  Per right-hand side
 // (blockIdx.y, n values each).
 template <typename T>
-__global__ void k_p_update_u(long long n, const Scalars* __restrict__ s, const T* __restrict__ z,
-                             T* __restrict__ p, T* __restrict__ u) {
-  const long long off = (long long)blockIdx.y * n;
-  z += off;
-  p += off;
-  u += off;
-  const T alpha = (T)s[blockIdx.y].alpha;
-  const T beta = (T)s[blockIdx.y].beta;
+__device__ __forceinline__ void p_update_u_run(long long n, T alpha, T beta,
+                                               const T* __restrict__ z, T* __restrict__ p,
+                                               T* __restrict__ u) {
   const long long n4 = n >> 2;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n4;
        i += (long long)gridDim.x * blockDim.x) {
@@ -1468,17 +1518,27 @@ __global__ void k_p_update_u(long long n, const Scalars* __restrict__ s, const T
   }
 }
 
+template <typename T>
+__global__ void k_p_update_u(long long n, const Scalars* __restrict__ s, const T* __restrict__ z,
+                             T* __restrict__ p, T* __restrict__ u) {
+  const long long off = (long long)blockIdx.y * n;
+  p_update_u_run<T>(n, (T)s[blockIdx.y].alpha, (T)s[blockIdx.y].beta, z + off, p + off, u + off);
+}
+
+template <typename T>
+__device__ __forceinline__ void u_flush_run(long long n, T alpha, const T* __restrict__ p,
+                                            T* __restrict__ u) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    u[i] = u[i] + alpha * p[i];
+}
+
 // the last deferred update: u += alpha p per right-hand side (blockIdx.y)
 template <typename T>
 __global__ void k_u_flush(long long n, const Scalars* __restrict__ s, const T* __restrict__ p,
                           T* __restrict__ u) {
   const long long off = (long long)blockIdx.y * n;
-  p += off;
-  u += off;
-  const T alpha = (T)s[blockIdx.y].alpha;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x)
-    u[i] = u[i] + alpha * p[i];
+  u_flush_run<T>(n, (T)s[blockIdx.y].alpha, p + off, u + off);
 }
 
 // p = z + beta p
@@ -1493,11 +1553,8 @@ __global__ void k_p_update(long long n, const Scalars* __restrict__ s, const T* 
 
 // out = mask * in (per DOF), partial sum out.out
 template <typename Tin, typename Tout>
-__global__ void k_mask_copy(long long nn, const uint8_t* __restrict__ mask, const Tin* in,
-                            Tout* out, double* __restrict__ partial) {
-  __shared__ double sred[4];
-  in += blockIdx.y * 3 * nn;
-  out += blockIdx.y * 3 * nn;
+__device__ __forceinline__ double mask_copy_run(long long nn, const uint8_t* __restrict__ mask,
+                                                const Tin* in, Tout* out) {
   double acc = 0.0;
   for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < nn;
        n += (long long)gridDim.x * blockDim.x) {
@@ -1509,6 +1566,15 @@ __global__ void k_mask_copy(long long nn, const uint8_t* __restrict__ mask, cons
       acc += (double)v * (double)v;
     }
   }
+  return acc;
+}
+
+template <typename Tin, typename Tout>
+__global__ void k_mask_copy(long long nn, const uint8_t* __restrict__ mask, const Tin* in,
+                            Tout* out, double* __restrict__ partial) {
+  __shared__ double sred[4];
+  const long long off = blockIdx.y * 3 * nn;
+  double acc = mask_copy_run<Tin, Tout>(nn, mask, in + off, out + off);
   if (partial) {
     acc = block_sum(acc, sred);
     if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = acc;
@@ -2032,6 +2098,263 @@ __global__ void k_train_loss(const double* __restrict__ part, int np, int B,
     }
   }
   if (threadIdx.x == 0) *loss = tot / B;
+}
+
+// ---------------------------------------------------------------------------
+// Parts of uneven size in one launch per operation (vx_solve_parts).
+//
+// Every part has its own grid and hierarchy.  The parts present at a level
+// are packed one after another in that level's buffers, each part's block
+// starting on a 64-element boundary (16-byte aligned for the vector paths),
+// and a launch covers a range [q0, q1) of them.  Per part and level, a
+// PartGrid holds the dims and the offsets of its vectors (3 Nn values), its
+// node arrays (mask, inverse diagonal) and its element array (E), plus, on
+// level 0, the offsets of its values and elements in the caller's packed
+// arrays.  A PartTiles / PartTr holds its tile / transfer grid and its first
+// work-group in the launch (a prefix over the level's parts; entry np is the
+// total).  A work-group finds its part by binary search over that prefix and
+// then runs the single-grid kernel body on the part's own buffers, so each
+// part computes exactly what its own solve would.  Given a scalars pointer,
+// the work-groups of a frozen (converged) part return at once: a converged
+// part costs no bandwidth while the others iterate.
+// ---------------------------------------------------------------------------
+struct PartGrid {
+  int nx, ny, nz;
+  long long voff, noff, eoff;   // this level's packed vectors / node arrays / element arrays
+  long long uoff, ueoff;        // level 0: offsets in the caller's packed values / elements
+};
+
+struct PartTiles {
+  int zc, gx, gy, gz;
+  int begin;
+};
+
+struct PartTr {
+  int Lcx, Lcy, Lcz;
+  TrGrid g;
+  int begin;
+  long long src, dst, node;     // input and output vector offsets; node offset of the mask
+                                // (and the inverse diagonal) of the output level
+};
+
+__device__ __forceinline__ long long part_nodes(const PartGrid& g) {
+  return (long long)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
+}
+
+// the part of global work-group vg: the last q in [q0, q1) with t[q].begin <= vg
+template <typename D>
+__device__ __forceinline__ int part_of(const D* __restrict__ t, int q0, int q1, int vg) {
+  int lo = q0, hi = q1 - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (t[mid].begin <= vg) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <typename T, int MODE, bool DOT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
+void k_tile_parts(TileArgs<T> a, const PartGrid* __restrict__ pg,
+                  const PartTiles* __restrict__ pt, int q0, int q1,
+                  const Scalars* __restrict__ fz) {
+  const int base = pt[q0].begin;
+  int v;
+  if (!tr_block(pt[q1].begin - base, v)) return;
+  const int vg = v + base;
+  const int q = part_of(pt, q0, q1, vg);
+  if (fz && fz[q].frozen) return;
+  const PartGrid g = pg[q];
+  const PartTiles t = pt[q];
+  TileArgs<T> b = a;
+  b.nx = g.nx; b.ny = g.ny; b.nz = g.nz;
+  b.Lx = g.nx + 1; b.Ly = g.ny + 1; b.Lz = g.nz + 1;
+  b.zc = t.zc; b.gx = t.gx; b.gy = t.gy; b.gz = t.gz;
+  b.nb = 1;
+  b.E = a.E + g.eoff;
+  b.mask = a.mask + g.noff;
+  tile_run<T, MODE, DOT>(b, vg - t.begin, vg, a.u + g.voff, a.b ? a.b + g.voff : nullptr,
+                         a.up ? a.up + g.voff : nullptr, a.out + g.voff);
+}
+
+// restriction of parts [q0, q1) of the fine level into the coarse level; the
+// coarse first smoothing step x1 is written for parts q < x1_lim
+template <typename T>
+__global__ __launch_bounds__(TR_THREADS) void k_restrict_parts(
+    const PartTr* __restrict__ pr, int q0, int q1, const T* __restrict__ rf,
+    const uint8_t* __restrict__ cmask, T* __restrict__ rc, const T* __restrict__ cminv, T w0,
+    T* __restrict__ x1, int x1_lim, const Scalars* __restrict__ fz) {
+  const int base = pr[q0].begin;
+  int v;
+  if (!tr_block(pr[q1].begin - base, v)) return;
+  const int vg = v + base;
+  const int q = part_of(pr, q0, q1, vg);
+  if (fz && fz[q].frozen) return;
+  const PartTr t = pr[q];
+  restrict_run<T>(t.Lcx, t.Lcy, t.Lcz, t.g, vg - t.begin, rf + t.src, cmask + t.node,
+                  rc + t.dst, cminv + t.node, w0, (x1 && q < x1_lim) ? x1 + t.dst : nullptr);
+}
+
+template <typename T>
+__global__ __launch_bounds__(TR_THREADS) void k_prolong_parts(
+    const PartTr* __restrict__ pp, int q0, int q1, const T* __restrict__ ec,
+    const uint8_t* __restrict__ fmask, T* __restrict__ x, const Scalars* __restrict__ fz) {
+  const int base = pp[q0].begin;
+  int v;
+  if (!tr_block(pp[q1].begin - base, v)) return;
+  const int vg = v + base;
+  const int q = part_of(pp, q0, q1, vg);
+  if (fz && fz[q].frozen) return;
+  const PartTr t = pp[q];
+  prolong_run<T>(t.Lcx, t.Lcy, t.Lcz, t.g, vg - t.begin, ec + t.src, fmask + t.node, x + t.dst);
+}
+
+// all coarsest-level steps of parts q0 + blockIdx.x, one work-group each
+template <typename T>
+__global__ __launch_bounds__(1024) void k_coarse_smooth_parts(
+    const PartGrid* __restrict__ pg, int q0, const T* __restrict__ E,
+    const uint8_t* __restrict__ mask, const T* __restrict__ minv, const T* __restrict__ b,
+    T* __restrict__ out, ElemConst<T> K, CoarseCoef cc, const Scalars* __restrict__ fz) {
+  const int q = q0 + blockIdx.x;
+  if (fz && fz[q].frozen) return;
+  const PartGrid g = pg[q];
+  coarse_smooth_run<T>(g.nx, g.ny, g.nz, E + g.eoff, mask + g.noff, minv + g.noff, b + g.voff,
+                       out + g.voff, K, cc);
+}
+
+// The elementwise kernels below run on parts q0 + blockIdx.y.
+template <typename T>
+__global__ void k_zero_step_parts(const PartGrid* __restrict__ pg, int q0,
+                                  const uint8_t* __restrict__ mask, const T* __restrict__ minv,
+                                  const T* __restrict__ b, T* __restrict__ out, T c, int vec4,
+                                  const Scalars* __restrict__ fz) {
+  const int q = q0 + blockIdx.y;
+  if (fz && fz[q].frozen) return;
+  const PartGrid g = pg[q];
+  zero_step_run<T>(part_nodes(g), mask + g.noff, minv + g.noff, b + g.voff, out + g.voff, c, vec4);
+}
+
+// r -= alpha Ap, partial r.r, next V-cycle's first step (FP32, level 0, all parts)
+__global__ void k_cg_update_x1_parts(const PartGrid* __restrict__ pg,
+                                     const Scalars* __restrict__ s, float* __restrict__ r,
+                                     const float* __restrict__ Ap, double* __restrict__ partial,
+                                     const uint8_t* __restrict__ mask,
+                                     const float* __restrict__ minv, float c,
+                                     float* __restrict__ x1) {
+  __shared__ double sred[4];
+  const int q = blockIdx.y;
+  if (s[q].frozen) return;
+  const PartGrid g = pg[q];
+  double acc = cg_update_x1_run(part_nodes(g), (float)s[q].alpha, r + g.voff, Ap + g.voff,
+                                mask + g.noff, minv + g.noff, c, x1 + g.voff);
+  acc = block_sum(acc, sred);
+  if (threadIdx.x == 0) partial[q * gridDim.x + blockIdx.x] = acc;
+}
+
+// u += alpha p (the deferred update), then p = z + beta p; a frozen part
+// takes only its pending update u += alpha p (its later alphas are 0)
+template <typename T>
+__global__ void k_p_update_u_parts(const PartGrid* __restrict__ pg,
+                                   const Scalars* __restrict__ s, const T* __restrict__ z,
+                                   T* __restrict__ p, T* __restrict__ u) {
+  const int q = blockIdx.y;
+  const PartGrid g = pg[q];
+  const long long n = 3 * part_nodes(g);
+  if (s[q].frozen)
+    u_flush_run<T>(n, (T)s[q].alpha, p + g.voff, u + g.voff);
+  else
+    p_update_u_run<T>(n, (T)s[q].alpha, (T)s[q].beta, z + g.voff, p + g.voff, u + g.voff);
+}
+
+template <typename T>
+__global__ void k_u_flush_parts(const PartGrid* __restrict__ pg, const Scalars* __restrict__ s,
+                                const T* __restrict__ p, T* __restrict__ u) {
+  const int q = blockIdx.y;
+  const PartGrid g = pg[q];
+  u_flush_run<T>(3 * part_nodes(g), (T)s[q].alpha, p + g.voff, u + g.voff);
+}
+
+template <typename T>
+__global__ void k_cg_update_parts(const PartGrid* __restrict__ pg, const Scalars* __restrict__ s,
+                                  T* __restrict__ u, T* __restrict__ r, const T* __restrict__ p,
+                                  const T* __restrict__ Ap, double* __restrict__ partial) {
+  __shared__ double sred[4];
+  const int q = blockIdx.y;
+  if (s[q].frozen) return;
+  const PartGrid g = pg[q];
+  double acc = cg_update_run<T>(3 * part_nodes(g), (T)s[q].alpha, u + g.voff, r + g.voff,
+                                p + g.voff, Ap + g.voff);
+  acc = block_sum(acc, sred);
+  if (threadIdx.x == 0) partial[q * gridDim.x + blockIdx.x] = acc;
+}
+
+template <typename T>
+__global__ void k_p_update_b_parts(const PartGrid* __restrict__ pg,
+                                   const Scalars* __restrict__ s, const T* __restrict__ z,
+                                   T* __restrict__ p) {
+  const int q = blockIdx.y;
+  if (s[q].frozen) return;
+  const PartGrid g = pg[q];
+  p_update_b_run<T>(3 * part_nodes(g), (T)s[q].beta, z + g.voff, p + g.voff);
+}
+
+// out = mask * in on level 0 of all parts (and partials of out.out when
+// partial is set).  dir 0: in from the caller's packing, out to this level's;
+// dir 1: in from this level's packing, out to the caller's; dir 2: both here.
+template <typename Tin, typename Tout>
+__global__ void k_mask_copy_parts(const PartGrid* __restrict__ pg, int dir,
+                                  const uint8_t* __restrict__ mask, const Tin* in, Tout* out,
+                                  double* __restrict__ partial) {
+  __shared__ double sred[4];
+  const int q = blockIdx.y;
+  const PartGrid g = pg[q];
+  const long long io = dir == 0 ? g.uoff : g.voff;
+  const long long oo = dir == 1 ? g.uoff : g.voff;
+  double acc = mask_copy_run<Tin, Tout>(part_nodes(g), mask + g.noff, in + io, out + oo);
+  if (partial) {
+    acc = block_sum(acc, sred);
+    if (threadIdx.x == 0) partial[q * gridDim.x + blockIdx.x] = acc;
+  }
+}
+
+__global__ void k_scale_to_f32_parts(const PartGrid* __restrict__ pg,
+                                     const Scalars* __restrict__ s,
+                                     const double* __restrict__ r64, float* __restrict__ r32) {
+  const int q = blockIdx.y;
+  const PartGrid g = pg[q];
+  const double nr = sqrt(s[q].tmp);
+  const double sc = (nr > 0.0 && s[q].tmp2 == 0.0) ? 1.0 / nr : 0.0;
+  const long long n = 3 * part_nodes(g);
+  r64 += g.voff;
+  r32 += g.voff;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    r32[i] = (float)(r64[i] * sc);
+}
+
+__global__ void k_add_scaled_parts(const PartGrid* __restrict__ pg,
+                                   const Scalars* __restrict__ s,
+                                   const float* __restrict__ d32, double* __restrict__ u64) {
+  const int q = blockIdx.y;
+  const PartGrid g = pg[q];
+  const double sc = sqrt(s[q].tmp);
+  const long long n = 3 * part_nodes(g);
+  d32 += g.voff;
+  u64 += g.voff;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    u64[i] += (double)d32[i] * sc;
+}
+
+// per part blockIdx.x: its partials are [pt[q].begin, pt[q + 1].begin) of a
+// tile launch, or the n_ew of an elementwise launch (pt == nullptr)
+__global__ __launch_bounds__(FIN_THREADS) void k_finalize_parts(
+    const double* __restrict__ partial, const PartTiles* __restrict__ pt, int n_ew, int op,
+    Scalars* s) {
+  const int q = blockIdx.x;
+  const int b0 = pt ? pt[q].begin : q * n_ew;
+  const int n = pt ? pt[q + 1].begin - b0 : n_ew;
+  finalize_run(partial + b0, n, op, s + q);
 }
 
 }  // namespace vx

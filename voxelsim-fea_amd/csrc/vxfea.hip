@@ -321,29 +321,62 @@ int prolong_add(vx_solver* s, int li, const T* ec, T* x, int nb = 0) {
   return VX_OK;
 }
 
-// one damped-Jacobi sweep with weight omega; u == nullptr means "from zero"
 template <typename T>
-int sweep(vx_solver* s, int li, const T* u, const T* b, T* out, bool dot, int* n_part,
-          double omega) {
-  Level& L = s->lv[li];
-  if (!u) {
-    // float4 path: every right-hand side's vector and the per-node arrays
-    // start 16-byte aligned (allocation alignment; rhs stride 3 Nn floats)
-    const bool aligned =
-        ((reinterpret_cast<uintptr_t>(b) | reinterpret_cast<uintptr_t>(out) |
-          reinterpret_cast<uintptr_t>(Sel<T>::mi(L)) | reinterpret_cast<uintptr_t>(L.mask)) & 15) ==
-            0 &&
-        (s->nb == 1 || (3 * L.Nn) % 4 == 0);
-    hipLaunchKernelGGL(k_zero_step<T>, dim3(ew_grid(L.Nn), s->nb), dim3(EW_THREADS), 0, s->stream,
-                       L.Nn, L.mask, Sel<T>::mi(L), b, out, (T)omega, aligned ? 1 : 0);
-    LAUNCHCHK();
-    if (dot) CHK(dot_partials<T>(s, 3 * L.Nn, out, b, n_part));
-    return VX_OK;
-  }
+int copy_vec(vx_solver* s, T* dst, const T* src, long long n) {
+  HIPCHK(hipMemcpyAsync(dst, src, sizeof(T) * n, hipMemcpyDeviceToDevice, s->stream));
+  return VX_OK;
+}
+
+// Where a smoothing pass runs.  LevelTarget: level li of the handle's grid,
+// all s->nb right-hand sides.  A target supplies the solver (options and
+// work buffers), the tile launch, the first step from zero and the number of
+// values of one vector set (the parts group defines its own target).
+struct LevelTarget {
+  vx_solver* s;
+  int li;
+};
+
+vx_solver* tsol(const LevelTarget& t) { return t.s; }
+
+// offset of the target's first value in its vectors
+long long tfirst(const LevelTarget&) { return 0; }
+
+long long tvals(const LevelTarget& t) { return 3 * t.s->lv[t.li].Nn * t.s->nb; }
+
+template <typename T, int MODE, bool DOT>
+int ttile(const LevelTarget& t, const T* u, const T* b, T* out, int* n_part, const SmoothCoef* sc) {
+  return launch_tile<T, MODE, DOT>(t.s, t.li, u, b, out, DOT ? t.s->partial : nullptr, n_part,
+                                   sc);
+}
+
+// the first smoothing step from zero, out = omega mask D^-1 b (A 0 = 0); dot:
+// partials of out.b
+template <typename T>
+int tzero(const LevelTarget& t, const T* b, T* out, double omega, bool dot, int* n_part) {
+  vx_solver* s = t.s;
+  Level& L = s->lv[t.li];
+  // float4 path: every right-hand side's vector and the per-node arrays
+  // start 16-byte aligned (allocation alignment; rhs stride 3 Nn floats)
+  const bool aligned =
+      ((reinterpret_cast<uintptr_t>(b) | reinterpret_cast<uintptr_t>(out) |
+        reinterpret_cast<uintptr_t>(Sel<T>::mi(L)) | reinterpret_cast<uintptr_t>(L.mask)) & 15) ==
+          0 &&
+      (s->nb == 1 || (3 * L.Nn) % 4 == 0);
+  hipLaunchKernelGGL(k_zero_step<T>, dim3(ew_grid(L.Nn), s->nb), dim3(EW_THREADS), 0, s->stream,
+                     L.Nn, L.mask, Sel<T>::mi(L), b, out, (T)omega, aligned ? 1 : 0);
+  LAUNCHCHK();
+  if (dot) CHK(dot_partials<T>(s, 3 * L.Nn, out, b, n_part));
+  return VX_OK;
+}
+
+// one damped-Jacobi sweep with weight omega; u == nullptr means "from zero"
+template <typename T, typename Tgt>
+int tsweep(const Tgt& t, const T* u, const T* b, T* out, bool dot, int* n_part, double omega) {
+  if (!u) return tzero<T>(t, b, out, omega, dot, n_part);
   SmoothCoef sc;
   sc.omega = omega;
-  if (dot) return launch_tile<T, M_JACOBI, true>(s, li, u, b, out, s->partial, n_part, &sc);
-  return launch_tile<T, M_JACOBI, false>(s, li, u, b, out, nullptr, nullptr, &sc);
+  if (dot) return ttile<T, M_JACOBI, true>(t, u, b, out, n_part, &sc);
+  return ttile<T, M_JACOBI, false>(t, u, b, out, nullptr, &sc);
 }
 
 // lower end of the Chebyshev interval (over lambda_max) of a k-step pass
@@ -359,9 +392,10 @@ static double cheb_ratio_for(const vx_solver* s, int k) {
 // iterates alternate between tmp and tmp2 (a step may overwrite the iterate
 // before the one it reads) and the k-th lands in dst; the first is in tmp
 // for every k >= 2.
-template <typename T>
-int cheb_n(vx_solver* s, int li, const T* u_in, const T* b, int k, T* dst, T* tmp, T* tmp2,
-           bool dot, int* n_part, bool first_done = false) {
+template <typename T, typename Tgt>
+int tcheb_n(const Tgt& tg, const T* u_in, const T* b, int k, T* dst, T* tmp, T* tmp2, bool dot,
+            int* n_part, bool first_done = false) {
+  vx_solver* s = tsol(tg);
   const double beta = s->lam_max, alpha = cheb_ratio_for(s, k) * beta;
   const double theta = 0.5 * (beta + alpha), delta = 0.5 * (beta - alpha);
   const double sigma = theta / delta;
@@ -374,7 +408,7 @@ int cheb_n(vx_solver* s, int li, const T* u_in, const T* b, int k, T* dst, T* tm
     const bool last = (j == k - 1);
     if (j == 0) {
       if (o == xcur) return fail(s, VX_ERR_ARG, "chebyshev aliasing");
-      if (!first_done) CHK(sweep<T>(s, li, xcur, b, o, dot && last, n_part, 1.0 / theta));
+      if (!first_done) CHK(tsweep<T>(tg, xcur, b, o, dot && last, n_part, 1.0 / theta));
     } else {
       rho = 1.0 / (2.0 * sigma - rho_prev);
       SmoothCoef sc;
@@ -385,9 +419,9 @@ int cheb_n(vx_solver* s, int li, const T* u_in, const T* b, int k, T* dst, T* tm
       // write) but never u, whose halo other work-groups read
       if (o == xcur) return fail(s, VX_ERR_ARG, "chebyshev aliasing");
       if (dot && last)
-        CHK((launch_tile<T, M_CHEB, true>(s, li, xcur, b, o, s->partial, n_part, &sc)));
+        CHK((ttile<T, M_CHEB, true>(tg, xcur, b, o, n_part, &sc)));
       else
-        CHK((launch_tile<T, M_CHEB, false>(s, li, xcur, b, o, nullptr, nullptr, &sc)));
+        CHK((ttile<T, M_CHEB, false>(tg, xcur, b, o, nullptr, &sc)));
       rho_prev = rho;
     }
     xprev = xcur;
@@ -399,9 +433,10 @@ int cheb_n(vx_solver* s, int li, const T* u_in, const T* b, int k, T* dst, T* tm
 // k damped-Jacobi sweeps starting at u_in (nullptr = zero) writing the result
 // to `dst`, using `tmp` as the other ping-pong buffer.  dot: last sweep adds
 // partials of out.b.
-template <typename T>
-int smooth_n(vx_solver* s, int li, const T* u_in, const T* b, int k, T* dst, T* tmp, bool dot,
-             int* n_part, bool first_done = false) {
+template <typename T, typename Tgt>
+int tsmooth_n(const Tgt& tg, const T* u_in, const T* b, int k, T* dst, T* tmp, bool dot,
+              int* n_part, bool first_done = false) {
+  vx_solver* s = tsol(tg);
   T* bufs[2] = {dst, tmp};
   int cur = (k % 2 == 1) ? 0 : 1;   // first write goes to bufs[cur]
   const T* src = u_in;
@@ -409,16 +444,10 @@ int smooth_n(vx_solver* s, int li, const T* u_in, const T* b, int k, T* dst, T* 
     T* o = bufs[cur];
     if (src == o) return fail(s, VX_ERR_ARG, "smoother aliasing");
     if (!(first_done && i == 0))
-      CHK(sweep<T>(s, li, src, b, o, dot && i == k - 1, n_part, s->opt.omega));
+      CHK(tsweep<T>(tg, src, b, o, dot && i == k - 1, n_part, s->opt.omega));
     src = o;
     cur ^= 1;
   }
-  return VX_OK;
-}
-
-template <typename T>
-int copy_vec(vx_solver* s, T* dst, const T* src, long long n) {
-  HIPCHK(hipMemcpyAsync(dst, src, sizeof(T) * n, hipMemcpyDeviceToDevice, s->stream));
   return VX_OK;
 }
 
@@ -458,34 +487,41 @@ static double first_step_coef(const vx_solver* s, int k) {
 }
 
 // smoothing step of the configured kind; result in dst (tmp, tmp2 scratch)
-template <typename T>
-int smooth_step(vx_solver* s, int li, const T* u_in, const T* b, int k, T* dst, T* tmp, T* tmp2,
-                bool dot, int* n_part, bool first_done = false) {
-  const long long n = 3 * s->lv[li].Nn * s->nb;
+template <typename T, typename Tgt>
+int tsmooth_step(const Tgt& tg, const T* u_in, const T* b, int k, T* dst, T* tmp, T* tmp2,
+                 bool dot, int* n_part, bool first_done = false) {
+  vx_solver* s = tsol(tg);
+  const long long n = tvals(tg), off = tfirst(tg);
   if (first_done) {
     // the first step from zero (k_cg_update_x1 or the restriction) is
     // already in tmp, where the smoother writes it
     if (u_in || !first_in_tmp(s, k))
       return fail(s, VX_ERR_ARG, "a precomputed first step needs a start from zero and its step in tmp");
     if (s->opt.smoother == VX_SMOOTH_CHEBYSHEV)
-      return cheb_n<T>(s, li, u_in, b, k, dst, tmp, tmp2, dot, n_part, true);
-    return smooth_n<T>(s, li, u_in, b, k, dst, tmp, dot, n_part, true);
+      return tcheb_n<T>(tg, u_in, b, k, dst, tmp, tmp2, dot, n_part, true);
+    return tsmooth_n<T>(tg, u_in, b, k, dst, tmp, dot, n_part, true);
   }
   if (s->opt.smoother == VX_SMOOTH_CHEBYSHEV) {
     if (u_in == dst && k == 1) {
       // the only write would land on the start vector: go through tmp
-      CHK(cheb_n<T>(s, li, u_in, b, k, tmp, dst, tmp2, dot, n_part));
-      return copy_vec<T>(s, dst, tmp, n);
+      CHK(tcheb_n<T>(tg, u_in, b, k, tmp, dst, tmp2, dot, n_part));
+      return copy_vec<T>(s, dst + off, tmp + off, n);
     }
-    return cheb_n<T>(s, li, u_in, b, k, dst, tmp, tmp2, dot, n_part);
+    return tcheb_n<T>(tg, u_in, b, k, dst, tmp, tmp2, dot, n_part);
   }
   if (u_in == dst) {
     // Jacobi from dst: ping-pong through tmp, copy back for odd counts
-    if (k % 2 == 0) return smooth_n<T>(s, li, u_in, b, k, dst, tmp, dot, n_part);
-    CHK(smooth_n<T>(s, li, u_in, b, k, tmp, dst, dot, n_part));
-    return copy_vec<T>(s, dst, tmp, n);
+    if (k % 2 == 0) return tsmooth_n<T>(tg, u_in, b, k, dst, tmp, dot, n_part);
+    CHK(tsmooth_n<T>(tg, u_in, b, k, tmp, dst, dot, n_part));
+    return copy_vec<T>(s, dst + off, tmp + off, n);
   }
-  return smooth_n<T>(s, li, u_in, b, k, dst, tmp, dot, n_part);
+  return tsmooth_n<T>(tg, u_in, b, k, dst, tmp, dot, n_part);
+}
+
+template <typename T>
+int smooth_step(vx_solver* s, int li, const T* u_in, const T* b, int k, T* dst, T* tmp, T* tmp2,
+                bool dot, int* n_part, bool first_done = false) {
+  return tsmooth_step<T>(LevelTarget{s, li}, u_in, b, k, dst, tmp, tmp2, dot, n_part, first_done);
 }
 
 // the step coefficients of k smoothing steps (k <= COARSE_MAX_STEPS), the
@@ -1273,23 +1309,13 @@ void vx_element_apply_host(double nu, const double* u24, double* y24) {
     for (int c = 0; c < 3; ++c) y24[3 * k + c] = y[ref_to_bin(k)][c];
 }
 
-int vx_create(int32_t nx, int32_t ny, int32_t nz, double nu, const vx_options* opt,
-              vx_handle* out) {
-  if (!out) return VX_ERR_ARG;
-  *out = nullptr;
-  if (nx < 1 || ny < 1 || nz < 1 || !(nu > -1.0 && nu < 0.5)) return VX_ERR_ARG;
-  vx_options o;
-  if (opt) {
-    if (opt->struct_size != (int32_t)sizeof(vx_options)) return VX_ERR_ARG;
-    o = *opt;
-  } else {
-    vx_default_options(&o);
-  }
-  if (o.max_batch < 1) o.max_batch = 1;
-  if (o.n_smooth < 1 || o.coarse_sweeps < 1 || o.max_levels < 1 ||
-      (o.coarse_mode != VX_COARSE_SWEEPS && o.coarse_mode != VX_COARSE_DIRECT))
-    return VX_ERR_ARG;
-  // hierarchy rule of core/multigrid.py:141-148
+}  // extern "C"
+
+namespace {
+
+// hierarchy rule of core/multigrid.py:141-148: halve while every dim is even,
+// the smallest stays >= min_elems / 2 and max_levels is not reached
+std::vector<std::array<int, 3>> level_dims_rule(int nx, int ny, int nz, const vx_options& o) {
   std::vector<std::array<int, 3>> dims;
   dims.push_back({nx, ny, nz});
   while ((int)dims.size() < o.max_levels) {
@@ -1299,18 +1325,49 @@ int vx_create(int32_t nx, int32_t ny, int32_t nz, double nu, const vx_options* o
     if (!(mn / 2 >= o.min_elems / 2 && mn >= 2)) break;
     dims.push_back({d[0] / 2, d[1] / 2, d[2] / 2});
   }
-  // every Chebyshev pass needs an interval [r lambda, lambda] with 0 < r < 1
-  // (r = 1 makes its width 0 and the step coefficients divide by it): check
-  // the ratio of each level's step count and of the coarsest sweeps
-  if (o.smoother == VX_SMOOTH_CHEBYSHEV) {
+  return dims;
+}
+
+// Copy and check the options (nullptr: defaults).  Every Chebyshev pass needs
+// an interval [r lambda, lambda] with 0 < r < 1 (r = 1 makes its width 0 and
+// the step coefficients divide by it): the ratio of each level's step count
+// and of the coarsest sweeps of a hierarchy with `nlev` levels is checked.
+int check_options(const vx_options* opt, int nlev, vx_options* o) {
+  if (opt) {
+    if (opt->struct_size != (int32_t)sizeof(vx_options)) return VX_ERR_ARG;
+    *o = *opt;
+  } else {
+    vx_default_options(o);
+  }
+  if (o->max_batch < 1) o->max_batch = 1;
+  if (o->n_smooth < 1 || o->coarse_sweeps < 1 || o->max_levels < 1 ||
+      (o->coarse_mode != VX_COARSE_SWEEPS && o->coarse_mode != VX_COARSE_DIRECT))
+    return VX_ERR_ARG;
+  if (o->smoother == VX_SMOOTH_CHEBYSHEV) {
     vx_solver probe;
-    probe.opt = o;
-    for (size_t li = 0; li < dims.size(); ++li) {
-      const int k = li + 1 == dims.size() ? o.coarse_sweeps : level_ns(&probe, (int)li);
+    probe.opt = *o;
+    for (int li = 0; li < nlev; ++li) {
+      const int k = li + 1 == nlev ? o->coarse_sweeps : level_ns(&probe, li);
       const double r = cheb_ratio_for(&probe, k);
       if (!(r > 0.0 && r < 1.0)) return VX_ERR_ARG;
     }
   }
+  return VX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vx_create(int32_t nx, int32_t ny, int32_t nz, double nu, const vx_options* opt,
+              vx_handle* out) {
+  if (!out) return VX_ERR_ARG;
+  *out = nullptr;
+  if (nx < 1 || ny < 1 || nz < 1 || !(nu > -1.0 && nu < 0.5)) return VX_ERR_ARG;
+  vx_options o;
+  if (check_options(opt, 0, &o) != VX_OK) return VX_ERR_ARG;
+  const std::vector<std::array<int, 3>> dims = level_dims_rule(nx, ny, nz, o);
+  if (check_options(opt, (int)dims.size(), &o) != VX_OK) return VX_ERR_ARG;
   if (vx_device_count() < 1) return VX_ERR_NO_DEVICE;
   vx_solver* s = new vx_solver();
   s->nx = nx; s->ny = ny; s->nz = nz; s->nu = nu;
@@ -2647,3 +2704,5 @@ int vx_corrector_train(vx_handle s, int32_t nsamp, const float* T, const float* 
 }
 
 }  // extern "C"
+
+#include "parts.h"
