@@ -35,6 +35,11 @@ HBM_PEAK_GBS = 8000.0
 # measured as `value`; OPT_PLAIN (plain 7-level cycle) runs the single-case
 # workloads that converge in 19-100 iterations
 from core.multigrid import PLAIN_CYCLE as OPT_PLAIN, VARIABLE_CYCLE as OPT  # noqa: E402
+# the uneven parts in one vx_solve_parts call: the plain cycle with 16
+# coarsest sweeps (the one-work-group coarsest smoothing of their 1000-4000-DOF
+# coarsest levels is serial work on a few CUs; 16 sweeps cost 5 % more
+# iterations and save 9 % of the time against 30)
+OPT_PARTS = dict(OPT_PLAIN, coarse_sweeps=16)
 
 
 def log(msg):
@@ -220,6 +225,77 @@ def _oracle_solve(p, max_levels=6, active_system=False):
     return u, time.perf_counter() - t0, vf.last_iters, vf.last_converged
 
 
+def uneven_parts_record():
+    """c4: 16 uneven ABS parts on void-padded grids (plain cycle) in one
+    vx_solve_parts call, beside the same parts solved one after another;
+    error against FP64 Jacobi-PCG of each part to 1e-10."""
+    import numpy as np
+    import torch
+    from core import workloads as W
+    from core.multigrid import MGSolver, PartsSolver
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    def used_of(q):
+        return np.repeat(W.element_nodes_mask(q.active.reshape(q.nx, q.ny, q.nz)).ravel(), 3)
+
+    def rel(u, ref, used):
+        return float(np.abs(u[..., used] - ref[..., used]).max() / np.abs(ref[..., used]).max())
+
+    def dev(q):
+        return (torch.tensor(q.E_stiffness, device="cuda"),
+                torch.tensor(q.active.astype(np.uint8), device="cuda"),
+                torch.tensor(q.force, device="cuda"))
+
+    # c4: 16 uneven ABS parts on void-padded grids (plain cycle), all in one
+    # vx_solve_parts call; beside it the same parts solved one after another
+    log("configs: c4 uneven parts")
+    parts = W.uneven_parts()
+    devs = [dev(q) for q in parts]
+    exact, errs, single_total = [], [], 0.0
+    for q, (E, A, f) in zip(parts, devs):
+        # FP64 Jacobi-PCG on the same active-DOF system (the reference's
+        # fallback, core/fea.py:497-593): the FP64 MG form's stall guard can
+        # fire on these parts' oscillating residual
+        u = torch.empty_like(f)
+        ref = MGSolver(q.nx, q.ny, q.nz, q.fixed_dofs, nu=q.nu, compute_mode="GPU", max_levels=1)
+        ref.solve_jacobi_device(E, A, f, u, tol=1e-10)
+        exact.append(u.cpu().numpy())
+        ref.close()
+        mg = MGSolver(q.nx, q.ny, q.nz, q.fixed_dofs, nu=q.nu, compute_mode="GPU", pad=True,
+                      **OPT_PLAIN)
+        single_total += timed(lambda: (mg.set_density(E, active=A),
+                                       mg.solve_device(f, u, tol=q.tol, max_cg=5000)))
+        mg.close()
+    ps = PartsSolver([(q.nx, q.ny, q.nz) for q in parts], [q.fixed_dofs for q in parts],
+                     nu=parts[0].nu, compute_mode="GPU", pad=True, **OPT_PARTS)
+    got = {}
+
+    def run_parts():
+        ps.set_density([d[0] for d in devs], [d[1] for d in devs])
+        got["U"], _ = ps.solve_device([d[2] for d in devs], tol=parts[0].tol, max_cg=5000)
+    t = timed(run_parts)
+    for q, u, ue in zip(parts, got["U"], exact):
+        errs.append(rel(u.cpu().numpy(), ue, used_of(q)))
+    rec = {
+        "time_s": t, "form": "one vx_solve_parts call (per-part recurrences)",
+        "solver": OPT_PARTS,
+        "one_solve_per_part_time_s": single_total, "speedup_vs_one_solve_per_part": single_total / t,
+        "iters": [i["iters"] for i in ps.last_parts],
+        "levels": [i["levels"] for i in ps.last_parts],
+        "true_resid_max": max(i["true_resid_ratio"] for i in ps.last_parts),
+        "all_converged": all(i["converged"] for i in ps.last_parts),
+        "rel_disp_err_vs_fp64_jacobi_1e-10": max(errs)}
+    ps.close()
+    return rec
+
+
 def config_records():
     """Compact records of the other four configurations for the headline line:
     time to tolerance with inputs resident in HBM (second of two runs; the
@@ -312,32 +388,7 @@ def config_records():
         "true_resid_max": max(b["true_resid_ratio"] for b in mg.last_batch),
         "rel_disp_err_vs_fp64_1e-10": rel(U.cpu().numpy(), U_ex, used_of(q))}
     mg.close()
-    # c4: 16 uneven ABS parts, each on its void-padded grid (plain cycle)
-    log("configs: c4 uneven parts")
-    parts = W.uneven_parts()
-    total, iters, errs, trues = 0.0, [], [], []
-    for q in parts:
-        E, A, f = dev(q)
-        u = torch.empty_like(f)
-        # FP64 Jacobi-PCG on the same active-DOF system (the reference's
-        # fallback, core/fea.py:497-593): the FP64 MG form's stall guard can
-        # fire on these parts' oscillating residual
-        exact = MGSolver(q.nx, q.ny, q.nz, q.fixed_dofs, nu=q.nu, compute_mode="GPU",
-                         max_levels=1)
-        exact.solve_jacobi_device(E, A, f, u, tol=1e-10)
-        u_ex = u.cpu().numpy()
-        exact.close()
-        mg = MGSolver(q.nx, q.ny, q.nz, q.fixed_dofs, nu=q.nu, compute_mode="GPU", pad=True,
-                      **OPT_PLAIN)
-        total += timed(lambda: (mg.set_density(E, active=A),
-                                mg.solve_device(f, u, tol=q.tol, max_cg=5000)))
-        iters.append(mg.last_iters)
-        trues.append(mg.last_true_resid_ratio)
-        errs.append(rel(u.cpu().numpy(), u_ex, used_of(q)))
-        mg.close()
-    out["c4_uneven_parts_16"] = {"time_s": total, "form": "one padded solve per part",
-                                 "iters": iters, "true_resid_max": max(trues),
-                                 "rel_disp_err_vs_fp64_jacobi_1e-10": max(errs)}
+    out["c4_uneven_parts_16"] = uneven_parts_record()
     return out
 
 
@@ -438,28 +489,8 @@ def other_workload(args):
             ((Ub - U).abs().amax(dim=1) / U.abs().amax(dim=1)).max().item())
         out["load_cases_per_s"] = nrhs / t_b
     elif args.workload == "uneven_parts":
-        parts = W.uneven_parts()
-        rows = []
-        tot = {}
-        for q in parts:
-            E, A, f = dev(q.E_stiffness), dev(q.active.astype(np.uint8)), dev(q.force)
-            u = torch.empty_like(f)
-            row = {"dims": [q.nx, q.ny, q.nz]}
-            for tag, pad in (("as_is", False), ("padded", True)):
-                mg = MGSolver(q.nx, q.ny, q.nz, q.fixed_dofs, nu=q.nu, compute_mode="GPU",
-                              pad=pad, **OPT_PLAIN)
-                run = lambda: (mg.set_density(E, active=A), mg.solve_device(f, u, tol=q.tol,
-                                                                             max_cg=5000))
-                timed(run)
-                t, _ = timed(run)
-                tot[tag] = tot.get(tag, 0.0) + t
-                row[tag] = {"grid": list(mg._pdims), "levels": len(mg.levels), "time_s": t,
-                            "iters": mg.last_iters, "converged": mg.last_converged}
-                mg.close()
-            rows.append(row)
-        out["parts"] = rows
-        out["total_time_s"] = tot
-        out["padded_speedup"] = tot["as_is"] / tot["padded"]
+        out["solver"] = OPT_PARTS
+        out.update(uneven_parts_record())
     elif args.workload == "inside":
         # voxelisation inside test: a 81,920-triangle sphere in a 256^3 grid
         from core import inside

@@ -547,16 +547,16 @@ class PartsSolver:
 
     ``parts``: a list of (nx, ny, nz); ``fixed``: a list of each part's fixed
     DOF ids.  Options as ``MGSolver`` (``pad=True`` grows each part with void
-    elements to coarsenable dims, which needs ``system="active"``).  The tile
-    kernels' z-chunks follow one rule over the whole group (``z_chunk`` fixes
-    them); ``autotune`` and ``graphs`` are accepted for ``MGSolver``'s option
-    sets and have no effect here."""
+    elements to coarsenable dims, which needs ``system="active"``; at most
+    ``pad_overhead`` times a part's volume).  The tile kernels' z-chunks follow
+    one rule over the whole group (``z_chunk`` fixes them); ``autotune`` is
+    accepted for ``MGSolver``'s option sets and has no effect here."""
 
     def __init__(self, parts, fixed, nu=0.3, n_smooth=2, omega=0.6, min_elems=4, max_levels=6,
                  compute_mode="AUTO", cpu_threads=0, verbose=False, precision="fp64",
                  coarse_sweeps=30, inner_tol=1e-5, max_outer=20, z_chunk=0, smoother="jacobi",
                  cheb_ratio=0.3, system="full", e_coarsen="mean", pad=False, autotune=False,
-                 level_smooth=None, graphs=True, cheb_ratio_power=0):
+                 level_smooth=None, graphs=True, cheb_ratio_power=0, pad_overhead=1.3):
         self.dims = [tuple(int(v) for v in d) for d in parts]
         if len(fixed) != len(self.dims):
             raise ValueError("one fixed-DOF list per part")
@@ -565,7 +565,7 @@ class PartsSolver:
         if pad and system != "active":
             raise ValueError("pad=True needs system='active'")
         self.system, self.precision = system, precision
-        self.pdims = [padded_dims(d, max_levels) if pad else d for d in self.dims]
+        self.pdims = [padded_dims(d, max_levels, pad_overhead) if pad else d for d in self.dims]
         ndof = sum(3 * (d[0] + 1) * (d[1] + 1) * (d[2] + 1) for d in self.pdims)
         self.plan = compute_plan.choose(ndof, mode=compute_mode, cpu_threads=cpu_threads,
                                         verbose=verbose)
@@ -576,7 +576,8 @@ class PartsSolver:
             coarse_sweeps=coarse_sweeps, precision=_PRECISION[precision], inner_tol=inner_tol,
             max_outer=max_outer, z_chunk=z_chunk, smoother=_SMOOTHER[smoother],
             cheb_ratio=cheb_ratio, e_coarsen={"mean": 0, "geometric": 1, "harmonic": 2}[e_coarsen],
-            level_smooth=_level_smooth(level_smooth), cheb_ratio_power=int(cheb_ratio_power))
+            level_smooth=_level_smooth(level_smooth), cheb_ratio_power=int(cheb_ratio_power),
+            eager=0 if graphs else 1)
         dims = np.ascontiguousarray(np.asarray(self.pdims, dtype=np.int32).reshape(-1))
         h = ctypes.c_void_p()
         rc = self._lib.vx_parts_create(len(self.pdims), dims.ctypes.data_as(ctypes.c_void_p),

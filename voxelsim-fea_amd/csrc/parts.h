@@ -227,7 +227,7 @@ int pvcycle(vx_parts* P, const T* b0, T* z0, T* t0, T* w0, bool dot, bool first_
 // pending deferred update u += alpha p, which the next direction update
 // still applies.
 template <typename T>
-int ppcg(vx_parts* P, T* u, T* r, T* z, T* p, T* Ap, T* t0, T* w0, double tol_rel, int max_iter,
+int ppcg(vx_parts* P, T* u, T* r, T* z, T* p, T* Ap, T* t0, T* w0, const std::vector<double>& tol_rel, int max_iter,
          bool stall_guard, const std::vector<double>& ratio_scale, double cb_tol,
          const std::vector<int>& iter_offset, vx_progress_cb cb, void* user, double* hist,
          int hist_cap, std::vector<PcgResult>& res, const std::vector<int>& skip) {
@@ -256,6 +256,11 @@ int ppcg(vx_parts* P, T* u, T* r, T* z, T* p, T* Ap, T* t0, T* w0, double tol_re
     return fail(s, VX_ERR_HIP, "memset p");
   }
   const PartsTarget all{P, 0, 0, nb, s->dscal};
+  bool graphable = !s->opt.eager;
+  for (int l = 0; l < (int)P->lv.size() && graphable; ++l) {
+    const int k = level_ns(s, l);   // smooth_step copies for these (copy_vec)
+    if (s->opt.smoother == VX_SMOOTH_CHEBYSHEV ? k == 1 : k % 2 == 1) graphable = false;
+  }
   std::vector<double> best(nb, INFINITY), prev(nb, NAN), prev2(nb, NAN);
   for (int it = 0; it < max_iter; ++it) {
     if (lazy_u)
@@ -285,12 +290,20 @@ int ppcg(vx_parts* P, T* u, T* r, T* z, T* p, T* Ap, T* t0, T* w0, double tol_re
       if (frozen[q]) continue;
       bool d = false;
       if (!std::isnan(prev[q]) && !std::isnan(prev2[q]) && prev2[q] > 0)
-        d = prev[q] * (prev[q] / prev2[q]) < 3.0 * tol_rel;
+        d = prev[q] * (prev[q] / prev2[q]) < 3.0 * tol_rel[q];
       predicted_done = predicted_done && d;
     }
-    auto queue_next = [&]() -> int {
+    // the V-cycle and its beta: the same launches every step, replayed from
+    // a captured graph when no smoothing pass of the cycle holds a copy
+    auto cycle_and_beta = [&]() -> int {
       CHK(pvcycle<T>(P, r, z, t0, w0, true, fuse_x1));
       return pfinalize(P, true, F_BETA);
+    };
+    auto queue_next = [&]() -> int {
+      if (!graphable) return cycle_and_beta();
+      const void* const key[4] = {r, z, t0, Ap};
+      return run_graphed(s, key, std::is_same<T, float>::value ? 3 : 2, 0, fuse_x1 ? 1 : 0, 0,
+                         cycle_and_beta);
     };
     bool queued = false;
     if (!predicted_done) {
@@ -306,7 +319,7 @@ int ppcg(vx_parts* P, T* u, T* r, T* z, T* p, T* Ap, T* t0, T* w0, double tol_re
       const Scalars sc = s->hscal[q];
       PcgResult& R = res[q];
       if (sc.pAp == 0.0) {   // nothing left along p (core/multigrid.py:500-511)
-        R.converged = R.ratio < tol_rel;
+        R.converged = R.ratio < tol_rel[q];
         frozen[q] = 1;
         changed = true;
         continue;
@@ -317,7 +330,7 @@ int ppcg(vx_parts* P, T* u, T* r, T* z, T* p, T* Ap, T* t0, T* w0, double tol_re
       const int gi = iter_offset[q] + it;
       if (hist && gi < hist_cap) hist[(long long)q * hist_cap + gi] = ratio * ratio_scale[q];
       worst = std::max(worst, ratio * ratio_scale[q]);
-      if (ratio < tol_rel) {
+      if (ratio < tol_rel[q]) {
         R.converged = true;
         frozen[q] = 1;
         changed = true;
@@ -569,6 +582,9 @@ void vx_parts_destroy(vx_parts_handle P) {
   vx_solver* s = &P->s;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   else (void)hipDeviceSynchronize();
+  drop_graphs(s);
+  if (s->cap_stream) (void)hipStreamDestroy(s->cap_stream);
+  s->cap_stream = nullptr;
   for (void* q : s->allocs) (void)hipFree(q);
   s->allocs.clear();
   if (s->hscal) (void)hipHostFree(s->hscal);
@@ -779,8 +795,8 @@ int vx_solve_parts(vx_parts_handle P, const double* F, double* U, double tol, in
       std::vector<double> one(nb, 1.0);
       std::vector<int> off(nb, 0);
       if (rc == VX_OK)
-        rc = ppcg<double>(P, P->u64, P->r64, P->z64, P->p64, P->Ap64, V0.t64, V0.w64, tol,
-                          max_iter, true, one, tol, off, cb, user, hp, hist_cap, pr, zero);
+        rc = ppcg<double>(P, P->u64, P->r64, P->z64, P->p64, P->Ap64, V0.t64, V0.w64,
+                          std::vector<double>(nb, tol), max_iter, true, one, tol, off, cb, user, hp, hist_cap, pr, zero);
       for (int q = 0; q < nb; ++q) {
         if (zero[q]) continue;
         inf[q].iters = pr[q].iters;
@@ -798,16 +814,12 @@ int vx_solve_parts(vx_parts_handle P, const double* F, double* U, double tol, in
         if (rc == VX_OK) rc = sync_scalars(s);
         if (rc != VX_OK) break;
         int open = 0, most = 0;
-        double worst = 0.0;
         for (int q = 0; q < nb; ++q) {
           if (zero[q]) continue;
           true_ratio[q] = std::sqrt(s->hscal[q].tmp) / fnorm[q];
           inf[q].outer_iters = outer;
           if (true_ratio[q] < tol) done[q] = 1;
-          if (!done[q]) {
-            ++open;
-            worst = std::max(worst, true_ratio[q]);
-          }
+          if (!done[q]) ++open;
           most = std::max(most, total[q]);
         }
         if (!open || outer == s->opt.max_outer || most >= max_iter) break;
@@ -820,7 +832,11 @@ int vx_solve_parts(vx_parts_handle P, const double* F, double* U, double tol, in
         HIPCHK(hipMemsetAsync(P->d32, 0, sizeof(float) * n0, s->stream));
         hipLaunchKernelGGL(k_set_fnorm, dim3(nb), dim3(1), 0, s->stream, s->dscal, 1.0);
         LAUNCHCHK();
-        const double want = std::max(s->opt.inner_tol, 0.5 * tol / worst);
+        // each part's inner solve stops where its own true residual would
+        // reach half the tolerance, but not beyond inner_tol
+        std::vector<double> want(nb, s->opt.inner_tol);
+        for (int q = 0; q < nb; ++q)
+          if (!done[q]) want[q] = std::max(s->opt.inner_tol, 0.5 * tol / true_ratio[q]);
         rc = ppcg<float>(P, P->d32, P->r32, P->z32, P->p32, P->Ap32, V0.t32, V0.w32, want,
                          max_iter - most, false, true_ratio, tol, total, cb, user, hp, hist_cap,
                          pr, done);

@@ -925,7 +925,7 @@ struct PcgResult {
 // beta = -z_new.A p / p.A p (A-orthogonal to the previous direction), which
 // equals Fletcher-Reeves for a fixed symmetric preconditioner.
 template <typename T>
-int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* Ap, T* t0, double tol_rel, int max_iter,
+int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* Ap, T* t0, const std::vector<double>& tol_rel, int max_iter,
         bool stall_guard, const std::vector<double>& ratio_scale, double cb_tol,
         const std::vector<int>& iter_offset, vx_progress_cb cb, void* user, double* hist,
         int hist_cap, std::vector<PcgResult>& res, const std::vector<int>& skip, int l0 = 0) {
@@ -1000,7 +1000,7 @@ int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* Ap, T* t0, double tol_rel, int 
       if (frozen[q]) continue;
       bool d = false;
       if (!std::isnan(prev[q]) && !std::isnan(prev2[q]) && prev2[q] > 0)
-        d = prev[q] * (prev[q] / prev2[q]) < 3.0 * tol_rel;
+        d = prev[q] * (prev[q] / prev2[q]) < 3.0 * tol_rel[q];
       predicted_done = predicted_done && d;
     }
     // the V-cycle and its beta: the same launches every step, replayed from
@@ -1032,7 +1032,7 @@ int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* Ap, T* t0, double tol_rel, int 
       const Scalars sc = s->hscal[q];
       PcgResult& R = res[q];
       if (sc.pAp == 0.0) {   // nothing left along p (core/multigrid.py:500-511)
-        R.converged = R.ratio < tol_rel;
+        R.converged = R.ratio < tol_rel[q];
         frozen[q] = 1;
         changed = true;
         continue;
@@ -1043,7 +1043,7 @@ int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* Ap, T* t0, double tol_rel, int 
       const int gi = iter_offset[q] + it;
       if (hist && gi < hist_cap) hist[(long long)q * hist_cap + gi] = ratio * ratio_scale[q];
       worst = std::max(worst, ratio * ratio_scale[q]);
-      if (ratio < tol_rel) {
+      if (ratio < tol_rel[q]) {
         R.converged = true;
         frozen[q] = 1;
         changed = true;
@@ -1257,8 +1257,8 @@ static int level_solve(vx_solver* s, int l, const double* b, double* x, double t
   std::vector<double> one(1, 1.0);
   std::vector<int> zero(1, 0);
   std::vector<PcgResult> pr;
-  CHK(pcg<double>(s, x, s->r64, s->z64, s->p64, s->Ap64, L.t64, tol, max_iter, false, one,
-                  tol, zero, nullptr, nullptr, nullptr, 0, pr, zero, l));
+  CHK(pcg<double>(s, x, s->r64, s->z64, s->p64, s->Ap64, L.t64, std::vector<double>(1, tol),
+                  max_iter, false, one, tol, zero, nullptr, nullptr, nullptr, 0, pr, zero, l));
   *out = pr[0];
   return VX_OK;
 }
@@ -1610,8 +1610,9 @@ static int solve_impl(vx_solver* s, int nb, const double* F, double* U, const do
       std::vector<double> one(nb, 1.0);
       std::vector<int> off(nb, 0);
       if (rc == VX_OK)
-        rc = pcg<double>(s, U, s->r64, s->z64, s->p64, s->Ap64, L0.t64, tol, max_iter,
-                         true, one, tol, off, cb, user, hist, hist_cap, pr, zero);
+        rc = pcg<double>(s, U, s->r64, s->z64, s->p64, s->Ap64, L0.t64,
+                         std::vector<double>(nb, tol), max_iter, true, one, tol, off, cb, user,
+                         hist, hist_cap, pr, zero);
       for (int q = 0; q < nb; ++q) {
         if (zero[q]) continue;
         inf[q].iters = pr[q].iters;
@@ -1669,7 +1670,11 @@ static int solve_impl(vx_solver* s, int nb, const double* F, double* U, const do
         // each inner right-hand side has unit norm
         hipLaunchKernelGGL(k_set_fnorm, dim3(nb), dim3(1), 0, s->stream, s->dscal, 1.0);
         LAUNCHCHK();
-        const double want = std::max(s->opt.inner_tol, 0.5 * tol / worst);
+        // each right-hand side's inner solve stops where its own true residual
+        // would reach half the tolerance, but not beyond inner_tol
+        std::vector<double> want(nb, s->opt.inner_tol);
+        for (int q = 0; q < nb; ++q)
+          if (!done[q]) want[q] = std::max(s->opt.inner_tol, 0.5 * tol / true_ratio[q]);
         learned_step = s->corr_on;
         int cap = max_iter - most;
         if (learned_step && s->corr_budget > 0) cap = std::min(cap, s->corr_budget);
