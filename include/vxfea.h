@@ -87,6 +87,15 @@ typedef struct {
   int32_t eager;              /* 0: each PCG step's V-cycle is replayed from a captured HIP
                                  graph (one submission instead of one per kernel; same
                                  kernels, same results); 1: launched kernel by kernel */
+  int32_t block_steps;        /* FP32 smoothing passes of >= 2 steps on the levels below level 0
+                                 with at most 40000 nodes: steps per launch of the LDS-blocked
+                                 smoother, whose work-groups repeat the halo work of the steps
+                                 they fuse (those levels' launches are latency-bound).  0: the
+                                 form vx_autotune picked, else blocked on levels of at most
+                                 12000 nodes with a size rule; 1: one tile launch per step;
+                                 2-8: blocked with that many (at most) */
+  int32_t block_edge;         /* owned box edge (nodes) of the blocked smoother's work-groups,
+                                 2-16; 0: chosen as block_steps is */
 } vx_options;
 
 /* Per-solve statistics (mirrors MGSolver.last_* of core/multigrid.py:482-517). */
@@ -395,11 +404,22 @@ const char* vx_inside_last_error(void);
  * direction-update form), the smoother / residual forms (class 1) and the
  * smoother forms with dot-product partials (class 2), in the given
  * precision, timed with the handle's max_batch right-hand sides per launch,
- * and keep the fastest for the handle's later launches.  Needs vx_set_fixed and vx_set_density; overwrites the
- * handle's work vectors.  opt.z_chunk > 0 overrides it. */
+ * and keep the fastest for the handle's later launches.  In FP32 it also
+ * times, on every level below level 0 with at most 40000 nodes, that level's
+ * smoothing pass as one tile launch per step against the blocked smoother's
+ * box edges and steps per launch, and keeps the fastest form.  Needs
+ * vx_set_fixed and vx_set_density; overwrites the handle's work vectors.
+ * opt.z_chunk > 0 overrides the z-chunks, opt.block_steps / opt.block_edge
+ * > 0 the blocked form. */
 int vx_autotune(vx_handle h, int32_t precision, int32_t reps);
 /* The z-chunk the handle uses for a level, precision and class. */
 int vx_tuned_zchunk(vx_handle h, int32_t level, int32_t precision, int32_t cls);
+/* The blocked smoother's plan for an FP32 pass of `steps` steps on a level
+ * (one right-hand side): owned box edge and steps per launch, or 0 and 0 when
+ * the pass runs one tile launch per step.  No reference counterpart (an
+ * MI355X launch-shape choice). */
+int vx_block_plan(vx_handle h, int32_t level, int32_t steps, int32_t* edge,
+                  int32_t* per_launch);
 
 /* Host-side element operator (the kernels' own code compiled for the CPU):
  * y24 = KE(nu) * u24 for one unit element in the reference's local DOF order

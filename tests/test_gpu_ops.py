@@ -252,3 +252,104 @@ def test_tile_results_independent_of_z_chunk():
             assert np.array_equal(op(ms[zc]), ref), zc
     for m in ms.values():
         m.close()
+
+
+@pytest.mark.parametrize("smoother,start", [("chebyshev", "zero"), ("chebyshev", "u"),
+                                            ("jacobi", "zero"), ("jacobi", "u")])
+def test_blocked_smoother_matches_step_launches(smoother, start):
+    """FP32 smoothing passes on the levels below level 0 with at most 40000
+    nodes run several steps per launch (k_smooth_block; each work-group
+    repeats the halo work of the steps it fuses).  They agree with one tile
+    launch per step (block_steps=1) and with FP64 (the oracle's steps) to FP32
+    rounding: the two kernels sum a node's element forces in other orders.
+    Between box edges and steps per launch the result is bitwise the same,
+    since every node sums the same terms in the same order."""
+    from core.multigrid import MGSolver
+    print("case blocked smoother", smoother, start, flush=True)
+    dims = (40, 24, 32)
+    E, fixed, ndof, rng = _problem(*dims, seed=31)
+    kw = dict(smoother=smoother, cheb_ratio=0.05, max_levels=3)
+    if smoother == "jacobi":
+        kw["omega"] = 0.35
+    variants = [(1, 0), (0, 0), (2, 0), (5, 0), (8, 0), (3, 2), (3, 5), (4, 7)]
+    ms = {v: MGSolver(*dims, fixed, nu=0.3, compute_mode="GPU", block_steps=v[0],
+                      block_edge=v[1], **kw) for v in variants}
+    for m in ms.values():
+        m.set_density(E)
+    m0 = ms[(1, 0)]
+    for li in (1, 2):
+        n = m0.levels[li]["ndof"]
+        b = rng.standard_normal(n)
+        u = None if start == "zero" else rng.standard_normal(n)
+        for k in (2, 3, 7, 12):
+            ref64 = m0._smooth(li, u, b, k, "fp64")
+            tile = m0._smooth(li, u, b, k, "mixed")
+            got = {v: ms[v]._smooth(li, u, b, k, "mixed") for v in variants[1:]}
+            tol = 5e-6 * k
+            assert _rel(tile, ref64) < tol, (li, k)
+            blocked = got[(0, 0)]
+            assert _rel(blocked, ref64) < tol, (li, k, _rel(blocked, ref64))
+            assert _rel(blocked, tile) < tol, (li, k)
+            for v, g in got.items():
+                assert np.array_equal(g, blocked), (li, k, v)
+    for m in ms.values():
+        m.close()
+
+
+@pytest.mark.parametrize("kw", [
+    dict(max_levels=5, level_smooth=[2, 2, 6, 12], coarse_sweeps=48, cheb_ratio_power=2),
+    dict(max_levels=4, level_smooth=[2, 3, 9], coarse_sweeps=24),
+])
+def test_vcycle_blocked_coarse_levels(kw):
+    """A mixed V-cycle whose coarse passes run blocked equals the one launched
+    step by step to FP32 rounding, and the FP64 cycle to the usual mixed bar."""
+    from core.multigrid import MGSolver
+    print("case vcycle blocked", kw, flush=True)
+    dims = (64, 32, 48)
+    E, fixed, ndof, rng = _problem(*dims, seed=32)
+    common = dict(smoother="chebyshev", cheb_ratio=0.05, **kw)
+    mb = MGSolver(*dims, fixed, nu=0.3, compute_mode="GPU", **common)
+    mt = MGSolver(*dims, fixed, nu=0.3, compute_mode="GPU", block_steps=1, **common)
+    for m in (mb, mt):
+        m.set_density(E)
+    b = np.where(rng.random(ndof) < 0.9, rng.standard_normal(ndof), 0.0)
+    zb = mb._vcycle(0, b, precision="mixed")
+    zt = mt._vcycle(0, b, precision="mixed")
+    z64 = mt._vcycle(0, b)
+    assert _rel(zb, zt) < 1e-5, _rel(zb, zt)
+    assert _rel(zb, z64) < 1e-4
+    mb.close()
+    mt.close()
+
+
+def test_autotuned_block_plans_solve_like_tile_launches():
+    """vx_autotune times each small level's FP32 pass as tile launches and as
+    blocked launches and keeps the faster form; the solve with the tuned
+    forms converges like the one with tile launches only (the two differ by
+    the rounding of a node's force sum)."""
+    import torch
+    from core.multigrid import MGSolver
+    print("case autotuned block plans", flush=True)
+    dims = (64, 64, 32)
+    E, fixed, ndof, rng = _problem(*dims, seed=33)
+    kw = dict(precision="mixed", smoother="chebyshev", cheb_ratio=0.05, max_levels=5,
+              level_smooth=[2, 2, 6, 12], coarse_sweeps=48, cheb_ratio_power=2)
+    f = torch.tensor(np.where(rng.random(ndof) < 0.5, rng.standard_normal(ndof), 0.0),
+                     device="cuda")
+    res = {}
+    for name, extra in (("tuned", dict(autotune=True)), ("tile", dict(block_steps=1))):
+        m = MGSolver(*dims, fixed, nu=0.3, compute_mode="GPU", **kw, **extra)
+        m.set_density(E)
+        u = torch.empty_like(f)
+        m.solve_device(f, u, tol=1e-8)
+        plans = m.block_plans()
+        for li, (edge, per) in plans.items():
+            assert li >= 1 and 2 <= edge <= 8 and 2 <= per <= 4
+        res[name] = (u.cpu().numpy(), m.last_iters, m.last_true_resid_ratio, plans)
+        m.close()
+    assert res["tile"][3] == {}
+    (ut, it_t, tr_t, _), (u1, it_1, tr_1, _) = res["tuned"], res["tile"]
+    print("  tuned plans", res["tuned"][3], "iters", it_t, it_1, flush=True)
+    assert tr_t < 1e-8 and tr_1 < 1e-8
+    assert abs(it_t - it_1) <= 2
+    assert _rel(ut, u1) < 1e-6

@@ -272,6 +272,19 @@ def test_create_rejects_degenerate_chebyshev_intervals():
                           cheb_ratio_power=2, coarse_sweeps=2) == bad
 
 
+def test_create_rejects_bad_block_options():
+    """block_steps (steps per launch of the blocked FP32 smoother) is 0-8 and
+    block_edge (its owned box edge) 0 or 2-16; vx_create checks both before it
+    looks for a device."""
+    from core import _native
+    ok = (_native.VX_OK, _native.VX_ERR_NO_DEVICE)
+    bad = _native.VX_ERR_ARG
+    for steps, edge in ((-1, 0), (9, 0), (0, 1), (0, 17), (0, -2)):
+        assert _create_status(block_steps=steps, block_edge=edge) == bad, (steps, edge)
+    for steps, edge in ((0, 0), (1, 0), (8, 16), (3, 2)):
+        assert _create_status(block_steps=steps, block_edge=edge) in ok, (steps, edge)
+
+
 # ---------------------------------------------------------------------------
 # element_strain_energy (core/fea.py:595-602)
 # ---------------------------------------------------------------------------
@@ -339,3 +352,16 @@ def test_strain_energy_closed_forms(nu):
     # a rigid translation plus rotation carries no energy
     ce = ref.element_strain_energy(field(1.0 + 0.2 * iy, -0.2 * ix, 0.5 + 0 * iz))
     assert np.abs(ce).max() < 1e-14
+
+
+def test_blocked_smoother_small_division():
+    """The blocked smoother splits flat indices with q = (n + 0.5) * fl(1/d)
+    in FP32 instead of an integer division.  Exhaustively for n < 2^16 and
+    d <= 2048 (its indices stay below 2^13 and its divisors below 2^10),
+    the truncated product equals n // d."""
+    n = np.arange(1 << 16, dtype=np.int64)
+    nf = n.astype(np.float32) + np.float32(0.5)
+    for d in range(1, 2049):
+        rd = np.float32(1.0) / np.float32(d)
+        q = (nf * rd).astype(np.int64)
+        assert np.array_equal(q, n // d), d

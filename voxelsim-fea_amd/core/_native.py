@@ -54,6 +54,8 @@ class vx_options(ctypes.Structure):
         ("flexible", ctypes.c_int32),
         ("level_smooth", ctypes.c_uint8 * 16),
         ("eager", ctypes.c_int32),
+        ("block_steps", ctypes.c_int32),
+        ("block_edge", ctypes.c_int32),
     ]
 
 
@@ -124,6 +126,7 @@ SIGNATURES = {
     "vx_element_apply_host": (None, [_D, _P, _P]),
     "vx_autotune": (ctypes.c_int, [_P, _I32, _I32]),
     "vx_tuned_zchunk": (ctypes.c_int, [_P, _I32, _I32, _I32]),
+    "vx_block_plan": (ctypes.c_int, [_P, _I32, _I32, _P, _P]),
     "vx_set_corrector": (ctypes.c_int, [_P, _P, _I32, ctypes.c_float, _I32, _I32]),
     "vx_corrector_eval": (ctypes.c_int, [_P, _I32, _P, _P, _P]),
     "vx_capture": (ctypes.c_int, [_P, _I32, _P, _I32, _I32]),

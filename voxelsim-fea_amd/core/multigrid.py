@@ -140,7 +140,10 @@ class MGSolver:
     ``cheb_ratio_power=p`` a Chebyshev pass of k steps uses the interval
     [cheb_ratio * (n_smooth / k)**p, 1] * lambda_max.  ``graphs=False``
     launches each PCG step's V-cycle kernel by kernel instead of replaying it
-    from a captured HIP graph (same kernels, same results)."""
+    from a captured HIP graph (same kernels, same results).  ``block_steps``
+    sets how many FP32 smoothing steps one launch of the LDS-blocked smoother
+    does on the small levels below level 0 (0: 3, 1: one launch per step) and
+    ``block_edge`` its owned box edge in nodes (0: from the level size)."""
 
     def __init__(self, nx, ny, nz, fixed_dofs, nu=0.3, xp=None, n_smooth=2, omega=0.6,
                  min_elems=4, max_levels=6, compute_mode="AUTO", cpu_threads=0,
@@ -148,7 +151,7 @@ class MGSolver:
                  max_outer=20, z_chunk=0, smoother="jacobi", cheb_ratio=0.3,
                  system="full", e_coarsen="mean", max_batch=1, flexible=False, pad=False,
                  autotune=False, coarse="sweeps", level_smooth=None, graphs=True,
-                 cheb_ratio_power=0):
+                 cheb_ratio_power=0, block_steps=0, block_edge=0):
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
         self.nelem = self.nx * self.ny * self.nz
         self.ndof = 3 * (self.nx + 1) * (self.ny + 1) * (self.nz + 1)
@@ -180,8 +183,11 @@ class MGSolver:
             coarse_mode={"sweeps": _native.VX_COARSE_SWEEPS,
                          "direct": _native.VX_COARSE_DIRECT}[coarse],
             level_smooth=_level_smooth(level_smooth), eager=0 if graphs else 1,
-            cheb_ratio_power=int(cheb_ratio_power))
+            cheb_ratio_power=int(cheb_ratio_power), block_steps=int(block_steps),
+            block_edge=int(block_edge))
         self.max_batch = max_batch
+        self._coarse_sweeps = int(coarse_sweeps)
+        self._steps = [int(k) for k in (level_smooth or [])]
         h = ctypes.c_void_p()
         px, py, pz = self._pdims
         _native.check(self._lib.vx_create(px, py, pz, float(nu), ctypes.byref(opt),
@@ -394,6 +400,23 @@ class MGSolver:
         self._check(self._lib.vx_autotune(self._h, prec, int(reps)))
         return {li: tuple(self._lib.vx_tuned_zchunk(self._h, li, prec, c) for c in range(3))
                 for li in range(len(self.levels))}
+
+    def _level_steps(self, li):
+        return self._steps[li] if li < len(self._steps) and self._steps[li] else self.n_smooth
+
+    def block_plans(self):
+        """{level: (box edge, steps per launch)} of the FP32 smoothing passes
+        that run blocked (levels below 0 whose pass would otherwise be one
+        tile launch per step); levels with tile launches are left out."""
+        out = {}
+        for li in range(1, len(self.levels)):
+            k = self._coarse_sweeps if li == len(self.levels) - 1 else self._level_steps(li)
+            e, s = ctypes.c_int32(), ctypes.c_int32()
+            self._check(self._lib.vx_block_plan(self._h, li, int(k), ctypes.byref(e),
+                                                ctypes.byref(s)))
+            if e.value > 0:
+                out[li] = (e.value, s.value)
+        return out
 
     # -- learned coarse-grid corrector --------------------------------------
     def level_nodes(self, li):

@@ -840,6 +840,236 @@ __global__ __launch_bounds__(BOT_THREADS) void k_bottom(BottomArgs a, CoarseCoef
   for (int i = tid; i < 3 * Nn; i += nt) xo[i] = sx[i];
 }
 
+// ---------------------------------------------------------------------------
+// Several smoothing steps of a small level in one launch (FP32).  A launch on
+// a level of a few tens of nodes per edge is latency-bound: one step costs
+// the dispatch and a few dependent memory round trips, not its bytes.  Here
+// each work-group owns a box of B^3 nodes and loads the iterate on that box
+// grown by `steps` nodes per side (clipped to the grid) into LDS.  Step j
+// (0-based) updates the box grown by steps-1-j nodes: one thread per element
+// of the elements touching those nodes applies the element operator into the
+// element's 24 LDS slots, then one thread per node sums its <= 8 slots in
+// corner order q = x + 2y + 4z (as k_bottom) and updates the node.  After the
+// last step the owned box holds the result, written out together with the
+// iterate before it (the next launch's previous iterate).  Every node's sum
+// has the same terms in the same order whatever B and the step count, so the
+// result does not depend on them; the halo work is repeated by neighbouring
+// work-groups.  Step forms:
+//   BLK_ZERO: x = mask ? w D^-1 b : 0                  (start from zero)
+//   BLK_JAC : x = x + (mask ? w D^-1 (b - K x) : 0)
+//   BLK_CHEB: x = x + (mask ? cu (x - x_prev) + cr D^-1 (b - K x) : 0)
+// LDS: 3 values per region node, E per region element and 24 force slots per
+// element of the first step (at most 160 KiB; the host sizes B and steps).
+// ---------------------------------------------------------------------------
+constexpr int BLK_THREADS = 1024;
+constexpr int BLK_NPT = 2;            // region nodes per thread (region <= 2048 nodes)
+constexpr int BLK_MAX_STEPS = 8;
+constexpr int BLK_LDS_BYTES = 160 * 1024;
+enum { BLK_ZERO = 0, BLK_JAC = 1, BLK_CHEB = 2 };
+
+struct BlockArgs {
+  int nx, ny, nz, Lx, Ly, Lz;
+  int B;                      // owned box edge in nodes
+  int gx, gy, gz;             // boxes per axis
+  int steps;
+  long long vstride;          // values per right-hand side (3 Nn)
+  const float* E;             // scaled E per element
+  const uint8_t* mask;        // free-DOF bits per node
+  const float* minv;          // D^-1 per node
+  const float* b;
+  const float* x;             // iterate before the launch (nullptr: 0)
+  const float* xp;            // the one before it (nullptr: 0)
+  float* xo;                  // iterate after the launch
+  float* xpo;                 // the one before it (nullptr: not written)
+  ElemConst<float> K;
+  int form[BLK_MAX_STEPS];
+  float c0[BLK_MAX_STEPS];    // BLK_ZERO, BLK_JAC: w; BLK_CHEB: cu
+  float c1[BLK_MAX_STEPS];    // BLK_CHEB: cr
+};
+
+// q = n / d for 0 <= n < 2^21 and d >= 1, from rd = 1/d in FP32: (n + 0.5) / d
+// lies at least 0.5 / d from an integer, and the two roundings (of 1/d and of
+// the product) move it by at most 1.2e-7 (n + 0.5) / d < 0.5 / d
+__device__ __forceinline__ int qdiv(int n, float rd) { return (int)(((float)n + 0.5f) * rd); }
+
+__global__ __launch_bounds__(BLK_THREADS) void k_smooth_block(BlockArgs a) {
+  extern __shared__ float blk_lds[];
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int blk = blockIdx.x;
+  const int S = a.steps, B = a.B;
+  const int bq = blk / a.gx, bz = bq / a.gy;
+  const int o0x = (blk - bq * a.gx) * B, o0y = (bq - bz * a.gy) * B, o0z = bz * B;
+  const int o1x = min(o0x + B, a.Lx), o1y = min(o0y + B, a.Ly), o1z = min(o0z + B, a.Lz);
+  const int rx0 = max(0, o0x - S), ry0 = max(0, o0y - S), rz0 = max(0, o0z - S);
+  const int RX = min(a.Lx, o1x + S) - rx0, RY = min(a.Ly, o1y + S) - ry0,
+            RZ = min(a.Lz, o1z + S) - rz0;
+  const int NR = RX * RY * RZ;
+  const int QX = RX - 1, QY = RY - 1;       // region elements per row / column
+  const int NER = QX * QY * (RZ - 1);
+  float* sx = blk_lds;                      // [3 NR]
+  float* sE = sx + 3 * NR;                  // [NER]
+  float* sye = sE + NER;                    // [24 NE] of the current step
+  const long long voff = (long long)blockIdx.y * a.vstride;
+  const float* b = a.b + voff;
+  const float r3x = 1.f / (float)(3 * RX), rry = 1.f / (float)RY, rrx = 1.f / (float)RX;
+  const float rqx = 1.f / (float)QX, rqy = 1.f / (float)QY;
+
+  // Every global load of the set-up is issued before the first wait (one
+  // memory round trip), as raw buffer loads whose out-of-range offset reads 0
+  // (no branches; the level's vectors are far below 2 GiB): the iterate on
+  // the region (rows of 3 RX contiguous values; 3 NR <= 3 BLK_NPT nt), E on
+  // its elements (NER < NR), and b, D^-1, the mask and the previous iterate
+  // of this thread's nodes, which stay in registers (only the node's own
+  // thread reads them).
+  const int Nn = a.Lx * a.Ly * a.Lz;
+  const int vbytes = (int)(4 * a.vstride);
+  const __amdgpu_buffer_rsrc_t rX = buf_rsrc(a.x ? a.x + voff : b, a.x ? vbytes : 0);
+  const __amdgpu_buffer_rsrc_t rP = buf_rsrc(a.xp ? a.xp + voff : b, a.xp ? vbytes : 0);
+  const __amdgpu_buffer_rsrc_t rB = buf_rsrc(b, vbytes);
+  const __amdgpu_buffer_rsrc_t rE = buf_rsrc(a.E, 4 * a.nx * a.ny * a.nz);
+  const __amdgpu_buffer_rsrc_t rM = buf_rsrc(a.minv, 4 * Nn);
+  const __amdgpu_buffer_rsrc_t rK = buf_rsrc(a.mask, Nn);
+  constexpr int XL = 3 * BLK_NPT;
+  float vx[XL], ve[BLK_NPT];
+#pragma unroll
+  for (int k = 0; k < XL; ++k) {
+    const int i = k * nt + tid;
+    const int row = qdiv(i, r3x), w = i - row * 3 * RX;
+    const int lz = qdiv(row, rry), ly = row - lz * RY;
+    const int g3 = 3 * (((rz0 + lz) * a.Ly + (ry0 + ly)) * a.Lx + rx0) + w;
+    vx[k] = buf_ld(rX, i < 3 * NR ? 4 * g3 : BUF_OOB, 0.f);
+  }
+#pragma unroll
+  for (int k = 0; k < BLK_NPT; ++k) {
+    const int e = k * nt + tid;
+    const int row = qdiv(e, rqx), lx = e - row * QX;
+    const int lz = qdiv(row, rqy), ly = row - lz * QY;
+    const int ge = ((rz0 + lz) * a.ny + (ry0 + ly)) * a.nx + (rx0 + lx);
+    ve[k] = buf_ld(rE, e < NER ? 4 * ge : BUF_OOB, 0.f);
+  }
+  int gxn[BLK_NPT], gyn[BLK_NPT], gzn[BLK_NPT], gn[BLK_NPT];
+  float rb[BLK_NPT][3], rxp[BLK_NPT][3], rmi[BLK_NPT];
+  unsigned rmk[BLK_NPT];
+#pragma unroll
+  for (int k = 0; k < BLK_NPT; ++k) {
+    const int n = tid + k * nt;
+    const bool in = n < NR;
+    const int row = qdiv(n, rrx), lz = qdiv(row, rry);
+    gxn[k] = in ? rx0 + n - row * RX : -1;
+    gyn[k] = ry0 + row - lz * RY;
+    gzn[k] = rz0 + lz;
+    gn[k] = (gzn[k] * a.Ly + gyn[k]) * a.Lx + (rx0 + n - row * RX);
+    rmi[k] = buf_ld(rM, in ? 4 * gn[k] : BUF_OOB, 0.f);
+    rmk[k] = buf_ld_u8(rK, in ? gn[k] : BUF_OOB);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      rb[k][c] = buf_ld(rB, in ? 4 * (3 * gn[k] + c) : BUF_OOB, 0.f);
+      rxp[k][c] = buf_ld(rP, in ? 4 * (3 * gn[k] + c) : BUF_OOB, 0.f);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < XL; ++k)
+    if (k * nt + tid < 3 * NR) sx[k * nt + tid] = vx[k];
+#pragma unroll
+  for (int k = 0; k < BLK_NPT; ++k)
+    if (k * nt + tid < NER) sE[k * nt + tid] = ve[k];
+  __syncthreads();
+
+  for (int j = 0; j < S; ++j) {
+    const int h = S - 1 - j;   // the step updates the owned box grown by h
+    const int ux0 = max(0, o0x - h), ux1 = min(a.Lx, o1x + h);
+    const int uy0 = max(0, o0y - h), uy1 = min(a.Ly, o1y + h);
+    const int uz0 = max(0, o0z - h), uz1 = min(a.Lz, o1z + h);
+    const int form = a.form[j];
+    // elements touching the updated nodes
+    const int ex0 = max(0, ux0 - 1), ey0 = max(0, uy0 - 1), ez0 = max(0, uz0 - 1);
+    const int EX = min(a.nx, ux1) - ex0, EY = min(a.ny, uy1) - ey0, EZ = min(a.nz, uz1) - ez0;
+    const int NE = EX * EY * EZ;
+    if (form != BLK_ZERO) {
+      const float rex = 1.f / (float)EX, rey = 1.f / (float)EY;
+      for (int e = tid; e < NE; e += nt) {
+        const int row = qdiv(e, rex), ez = qdiv(row, rey);
+        const int lx = ex0 - rx0 + e - row * EX, ly = ey0 - ry0 + row - ez * EY,
+                  lz = ez0 - rz0 + ez;
+        float u[8][3], y[8][3];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int n = ((lz + (q >> 2)) * RY + (ly + ((q >> 1) & 1))) * RX + (lx + (q & 1));
+#pragma unroll
+          for (int c = 0; c < 3; ++c) u[q][c] = sx[3 * n + c];
+        }
+        elem_apply<float>(u, sE[(lz * QY + ly) * QX + lx], a.K, y);
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) sye[(3 * q + c) * NE + e] = y[q][c];
+      }
+      __syncthreads();
+    }
+    const float c0 = a.c0[j], c1 = a.c1[j];
+#pragma unroll
+    for (int k = 0; k < BLK_NPT; ++k) {
+      const int X = gxn[k], Y = gyn[k], Z = gzn[k];
+      if (!(((unsigned)(X - ux0) < (unsigned)(ux1 - ux0)) & ((unsigned)(Y - uy0) < (unsigned)(uy1 - uy0)) &
+            ((unsigned)(Z - uz0) < (unsigned)(uz1 - uz0))))
+        continue;
+      const int n = tid + k * nt;
+      // the node's <= 8 element slots, read without branches (a missing
+      // element reads slot 0 and adds +0)
+      int sl[8];
+      bool ok[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int ex = X - (q & 1) - ex0, ey = Y - ((q >> 1) & 1) - ey0, ez = Z - (q >> 2) - ez0;
+        ok[q] = ((unsigned)ex < (unsigned)EX) & ((unsigned)ey < (unsigned)EY) &
+                ((unsigned)ez < (unsigned)EZ);
+        sl[q] = ok[q] ? 3 * q * NE + (ez * EY + ey) * EX + ex : 0;
+      }
+      float kx[3] = {0.f, 0.f, 0.f};
+      if (form != BLK_ZERO) {
+        float v[8][3];
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) v[q][c] = sye[sl[q] + c * NE];
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) kx[c] += ok[q] ? v[q][c] : 0.f;
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const float x = sx[3 * n + c];
+        const bool fr = (rmk[k] >> c) & 1u;
+        float xn;
+        if (form == BLK_ZERO) {
+          xn = fr ? c0 * rmi[k] * rb[k][c] : 0.f;
+        } else {
+          const float r = rmi[k] * (rb[k][c] - kx[c]);
+          const float t = form == BLK_JAC ? fmaT(c0, r, x) : x + fmaT(c0, x - rxp[k][c], c1 * r);
+          xn = fr ? t : x;
+        }
+        rxp[k][c] = x;
+        sx[3 * n + c] = xn;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < BLK_NPT; ++k) {
+    const int X = gxn[k], Y = gyn[k], Z = gzn[k];
+    if (!(((unsigned)(X - o0x) < (unsigned)(o1x - o0x)) & ((unsigned)(Y - o0y) < (unsigned)(o1y - o0y)) &
+          ((unsigned)(Z - o0z) < (unsigned)(o1z - o0z))))
+      continue;
+    const int n = tid + k * nt;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      a.xo[voff + 3 * gn[k] + c] = sx[3 * n + c];
+      if (a.xpo) a.xpo[voff + 3 * gn[k] + c] = rxp[k][c];
+    }
+  }
+}
+
 // per-node values -> per-DOF (3 copies)
 __global__ void k_expand3(long long Nn, const double* __restrict__ a, double* __restrict__ b) {
   for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < Nn;

@@ -39,6 +39,10 @@ struct Level {
   // V-cycle work vectors (levels >= 1 own b; x/t at every level)
   double *b64 = nullptr, *x64 = nullptr, *t64 = nullptr, *w64 = nullptr;
   float *b32 = nullptr, *x32 = nullptr, *t32 = nullptr, *w32 = nullptr;
+  float *v32a = nullptr, *v32b = nullptr;   // blocked-smoother iterates (small levels)
+  // blocked smoother plan from vx_autotune: box edge and steps per launch,
+  // blk_B = -1: one tile launch per step; 0: not tuned (the size rule)
+  int blk_B = 0, blk_S = 0;
   uint8_t* mask_fixed = nullptr;   // injected fixed-DOF mask (set_fixed)
   double* mi64 = nullptr;          // inverse Jacobi diagonal per node
   float* mi32 = nullptr;
@@ -134,6 +138,7 @@ struct vx_solver {
   std::vector<CycleGraph> graphs;
   hipStream_t cap_stream = nullptr;
   bool graphs_failed = false;
+  int n_cu = 256;             // compute units of the device
   std::vector<void*> allocs;
 };
 
@@ -451,6 +456,15 @@ int tsmooth_n(const Tgt& tg, const T* u_in, const T* b, int k, T* dst, T* tmp, b
   return VX_OK;
 }
 
+// FP32 smoothing passes of levels li > 0 with at most BLK_LEVEL_MAX_NODES
+// nodes may run blocked (k_smooth_block): a tile launch there costs its
+// latency, not its bytes.  Untuned, the blocked form is used up to
+// BLK_RULE_MAX_NODES (measured on MI355X: a step of a 17^3-node level takes
+// 3.6 us blocked against 4.9 us per tile launch; at 33^3 both take 5.6 us);
+// vx_autotune times both forms on every level up to the larger bound.
+constexpr long long BLK_LEVEL_MAX_NODES = 40000;
+constexpr long long BLK_RULE_MAX_NODES = 12000;
+
 // level work vectors for max_batch right-hand sides
 template <typename T>
 int alloc_level_vectors(vx_solver* s) {
@@ -461,6 +475,10 @@ int alloc_level_vectors(vx_solver* s) {
     CHK(dalloc(s, &Sel<T>::x(L), n));
     CHK(dalloc(s, &Sel<T>::t(L), n));
     CHK(dalloc(s, &Sel<T>::w(L), n));
+    if (std::is_same<T, float>::value && li > 0 && L.Nn <= BLK_LEVEL_MAX_NODES) {
+      CHK(dalloc(s, &L.v32a, n));
+      CHK(dalloc(s, &L.v32b, n));
+    }
   }
   return VX_OK;
 }
@@ -518,9 +536,18 @@ int tsmooth_step(const Tgt& tg, const T* u_in, const T* b, int k, T* dst, T* tmp
   return tsmooth_n<T>(tg, u_in, b, k, dst, tmp, dot, n_part);
 }
 
+static bool block_eligible(const vx_solver* s, int li, int k, bool dot);
+static int smooth_blocked(vx_solver* s, int li, const float* u_in, const float* b, int k,
+                          float* dst, float* tmp, float* tmp2, bool first_done);
+
 template <typename T>
 int smooth_step(vx_solver* s, int li, const T* u_in, const T* b, int k, T* dst, T* tmp, T* tmp2,
                 bool dot, int* n_part, bool first_done = false) {
+  if (std::is_same<T, float>::value && block_eligible(s, li, k, dot))
+    return smooth_blocked(s, li, reinterpret_cast<const float*>(u_in),
+                          reinterpret_cast<const float*>(b), k, reinterpret_cast<float*>(dst),
+                          reinterpret_cast<float*>(tmp), reinterpret_cast<float*>(tmp2),
+                          first_done);
   return tsmooth_step<T>(LevelTarget{s, li}, u_in, b, k, dst, tmp, tmp2, dot, n_part, first_done);
 }
 
@@ -547,6 +574,198 @@ static CoarseCoef smooth_coef(const vx_solver* s, int k) {
     cc.w0 = s->opt.omega;
   }
   return cc;
+}
+
+// ---- blocked smoothing passes (k_smooth_block) ----------------------------
+
+static bool block_eligible(const vx_solver* s, int li, int k, bool dot) {
+  if (dot || li <= 0 || li >= (int)s->lv.size() || s->opt.block_steps == 1 || k < 2 ||
+      k > COARSE_MAX_STEPS)
+    return false;
+  const Level& L = s->lv[li];
+  if (L.Nn > BLK_LEVEL_MAX_NODES || !L.v32a || !L.v32b) return false;
+  if (s->opt.block_steps > 0 || s->opt.block_edge > 0) return true;   // requested
+  if (L.blk_B != 0) return L.blk_B > 0;                               // tuned
+  return L.Nn <= BLK_RULE_MAX_NODES;
+}
+
+// region extent per axis of a box of edge B grown by S (its largest, interior form)
+static void block_region(const Level& L, int B, int S, long long* nr, long long* ne) {
+  const long long rx = std::min<long long>(L.Lx, B + 2 * S);
+  const long long ry = std::min<long long>(L.Ly, B + 2 * S);
+  const long long rz = std::min<long long>(L.Lz, B + 2 * S);
+  *nr = rx * ry * rz;
+  *ne = (rx - 1) * (ry - 1) * (rz - 1);
+}
+
+static size_t block_lds_bytes(const Level& L, int B, int S) {
+  long long nr, ne;
+  block_region(L, B, S, &nr, &ne);
+  return sizeof(float) * (size_t)(3 * nr + 25 * ne);
+}
+
+static bool block_fits(const Level& L, int B, int S) {
+  long long nr, ne;
+  block_region(L, B, S, &nr, &ne);
+  return nr <= (long long)BLK_NPT * BLK_THREADS && block_lds_bytes(L, B, S) <= BLK_LDS_BYTES;
+}
+
+static long long block_count(const Level& L, int B) {
+  return (long long)((L.Lx + B - 1) / B) * ((L.Ly + B - 1) / B) * ((L.Lz + B - 1) / B);
+}
+
+// threads of a blocked work-group: one element per thread in the first step
+// (whole waves), at least half the region's nodes (BLK_NPT per thread)
+static int block_threads(const Level& L, int B, int S) {
+  long long nr, ne;
+  block_region(L, B, S, &nr, &ne);
+  const long long t = std::max(ne, (nr + BLK_NPT - 1) / BLK_NPT);
+  return (int)std::min<long long>(BLK_THREADS, std::max<long long>(64, (t + 63) / 64 * 64));
+}
+
+// Box edge B and steps per launch S for a pass on level L: the options, the
+// autotuned pair, or the pair of least estimated time per step.  Fitted to
+// MI355X timings of 17^3- and 33^3-node levels: a launch costs about 3 us
+// (dispatch, loads, stores) per round of co-resident work-groups, and each
+// step about 1.3 us plus 4.5 ns per element application of the CU's
+// work-groups.
+struct BlockPlan {
+  int B = 0, S = 0, nt = 0;
+};
+
+static BlockPlan block_plan(const vx_solver* s, const Level& L, int k) {
+  BlockPlan best;
+  double best_cost = 0;
+  if (s->opt.block_steps <= 0 && s->opt.block_edge <= 0 && L.blk_B > 0 &&
+      block_fits(L, L.blk_B, std::min(L.blk_S, k))) {
+    best.B = L.blk_B;
+    best.S = std::min(L.blk_S, k);
+    best.nt = block_threads(L, best.B, best.S);
+    return best;
+  }
+  // requested steps per launch: the most of at most that many that fit
+  int s_lo = 2, s_hi = std::min(4, k);
+  if (s->opt.block_steps > 0) {
+    s_hi = std::min<int>(s->opt.block_steps, k);
+    while (s_hi > 1) {
+      bool any = false;
+      for (int B = 2; B <= 16 && !any; ++B)
+        any = (s->opt.block_edge <= 0 || B == s->opt.block_edge) && block_fits(L, B, s_hi);
+      if (any) break;
+      --s_hi;
+    }
+    s_lo = s_hi;
+  }
+  for (int S = std::min(s_lo, s_hi); S <= s_hi; ++S) {
+    for (int B = 2; B <= 16; ++B) {
+      if (s->opt.block_edge > 0 && B != s->opt.block_edge) continue;
+      if (!block_fits(L, B, S)) continue;
+      const long long nblk = block_count(L, B) * s->nb;
+      const long long w = (nblk + s->n_cu - 1) / s->n_cu;   // work-groups per CU
+      const long long conc = std::max<long long>(
+          1, std::min<long long>(2048 / block_threads(L, B, S),
+                                 BLK_LDS_BYTES / (long long)block_lds_bytes(L, B, S)));
+      double t = 3.0 * (double)((w + conc - 1) / conc);
+      for (int j = 0; j < S; ++j) {
+        const int e = B + 2 * (S - 1 - j) + 1;
+        const double ne = (double)std::min(L.nx, e) * std::min(L.ny, e) * std::min(L.nz, e);
+        t += 1.3 + (double)w * ne * 4.5e-3;
+      }
+      const double cost = t / S;
+      if (!best.B || cost < best_cost) {
+        best.B = B;
+        best.S = S;
+        best_cost = cost;
+      }
+    }
+  }
+  if (best.B) best.nt = block_threads(L, best.B, best.S);
+  return best;
+}
+
+// k smoothing steps of level li (the step sequence of smooth_step: from zero,
+// from u_in with a first Jacobi-form step, or after a first step from zero
+// already in tmp), `block_steps` per k_smooth_block launch.  A launch reads
+// the iterate and the one before it and writes both anew: they rotate over
+// tmp, tmp2 and the level's two blocked vectors, and the last launch writes
+// dst, which no launch before it reads or writes unless it is u_in.
+static int smooth_blocked(vx_solver* s, int li, const float* u_in, const float* b, int k,
+                          float* dst, float* tmp, float* tmp2, bool first_done) {
+  Level& L = s->lv[li];
+  const CoarseCoef cc = smooth_coef(s, k);
+  const BlockPlan plan = block_plan(s, L, k);
+  if (!plan.B)   // no box of the requested edge fits the LDS: one launch per step
+    return tsmooth_step<float>(LevelTarget{s, li}, u_in, b, k, dst, tmp, tmp2, false, nullptr,
+                               first_done);
+  const int S = plan.S, B = plan.B;
+  int j = 0;
+  const float* cur = u_in;
+  const float* prev = nullptr;
+  if (first_done) {
+    if (u_in) return fail(s, VX_ERR_ARG, "a precomputed first step needs a start from zero");
+    cur = tmp;   // x1 from zero; x0 = 0
+    j = 1;
+  }
+  float* pool[4] = {tmp, tmp2, L.v32a, L.v32b};
+  BlockArgs a;
+  a.nx = L.nx; a.ny = L.ny; a.nz = L.nz;
+  a.Lx = L.Lx; a.Ly = L.Ly; a.Lz = L.Lz;
+  a.B = B;
+  a.gx = (L.Lx + B - 1) / B; a.gy = (L.Ly + B - 1) / B; a.gz = (L.Lz + B - 1) / B;
+  a.vstride = 3 * L.Nn;
+  a.E = L.E32; a.mask = L.mask; a.minv = L.mi32; a.b = b;
+  a.K = kconst<float>(s);
+  const unsigned nblk = (unsigned)(a.gx * a.gy * a.gz);
+  while (j < k) {
+    int n = std::min(S, k - j);
+    bool last = j + n == k;
+    if (last && cur == dst) {   // the first launch reads dst (u_in): finish in a second one
+      n = (k - j + 1) / 2;
+      last = false;
+    }
+    float* o = dst;
+    float* op = nullptr;
+    if (!last) {
+      o = nullptr;
+      for (float* p : pool) {
+        if (p == cur || p == prev || p == dst || p == b) continue;
+        if (!o) {
+          o = p;
+        } else {
+          op = p;
+          break;
+        }
+      }
+      if (!o || !op) return fail(s, VX_ERR_ARG, "blocked smoother: aliased buffers");
+    }
+    a.steps = n;
+    for (int t = 0; t < n; ++t) {
+      const int st = j + t;
+      a.c1[t] = 0.f;
+      if (st == 0) {
+        a.form[t] = u_in ? BLK_JAC : BLK_ZERO;
+        a.c0[t] = (float)cc.w0;
+      } else if (cc.cheb) {
+        a.form[t] = BLK_CHEB;
+        a.c0[t] = (float)cc.cu[st];
+        a.c1[t] = (float)cc.cr[st];
+      } else {
+        a.form[t] = BLK_JAC;
+        a.c0[t] = (float)cc.w0;
+      }
+    }
+    a.x = cur;
+    a.xp = prev;
+    a.xo = o;
+    a.xpo = op;
+    hipLaunchKernelGGL(k_smooth_block, dim3(nblk, s->nb), dim3(plan.nt),
+                       block_lds_bytes(L, B, n), s->stream, a);
+    LAUNCHCHK();
+    prev = op;
+    cur = o;
+    j += n;
+  }
+  return VX_OK;
 }
 
 // The two bottom levels of the cycle in one launch (k_bottom): FP32, the
@@ -1343,6 +1562,9 @@ int check_options(const vx_options* opt, int nlev, vx_options* o) {
   if (o->n_smooth < 1 || o->coarse_sweeps < 1 || o->max_levels < 1 ||
       (o->coarse_mode != VX_COARSE_SWEEPS && o->coarse_mode != VX_COARSE_DIRECT))
     return VX_ERR_ARG;
+  if (o->block_steps < 0 || o->block_steps > BLK_MAX_STEPS || o->block_edge < 0 ||
+      o->block_edge == 1 || o->block_edge > 16)
+    return VX_ERR_ARG;
   if (o->smoother == VX_SMOOTH_CHEBYSHEV) {
     vx_solver probe;
     probe.opt = *o;
@@ -1410,6 +1632,21 @@ int vx_create(int32_t nx, int32_t ny, int32_t nz, double nu, const vx_options* o
   if (rc == VX_OK &&
       hipHostMalloc((void**)&s->hflags, sizeof(int) * s->opt.max_batch) != hipSuccess)
     rc = VX_ERR_NOMEM;
+  if (rc == VX_OK) {
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        ncu > 0)
+      s->n_cu = ncu;
+    (void)hipGetLastError();
+    // the blocked smoother's LDS exceeds the default dynamic limit
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_smooth_block),
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            BLK_LDS_BYTES) != hipSuccess) {
+      (void)hipGetLastError();
+      rc = VX_ERR_HIP;
+    }
+  }
   if (rc != VX_OK) {
     vx_destroy(s);
     return rc;
@@ -2442,6 +2679,50 @@ extern "C" int vx_autotune(vx_handle s, int32_t precision, int32_t reps) {
         }
       L.zc_tuned[pi][cls] = best_zc;
     }
+    // FP32 smoothing passes of a small level: one tile launch per step
+    // against the blocked smoother's box edges and steps per launch, each
+    // timed on the level's own pass (its steps from zero)
+    const int nl = (int)s->lv.size();
+    const int kp = li + 1 == (size_t)nl ? s->opt.coarse_sweeps : level_ns(s, (int)li);
+    if (pi == 1 && rc == VX_OK && li > 0 && L.v32a && kp >= 2 && kp <= COARSE_MAX_STEPS) {
+      std::vector<std::pair<int, int>> cand{{-1, 0}};
+      for (int S = 2; S <= std::min(4, kp); ++S)
+        for (int B = 2; B <= 8; ++B)
+          if (block_fits(L, B, S)) cand.push_back({B, S});
+      std::vector<float> tms;
+      float best = INFINITY;
+      for (const auto& c : cand) {
+        L.blk_B = c.first;
+        L.blk_S = c.second;
+        auto pass = [&]() {
+          return smooth_step<float>(s, (int)li, nullptr, L.b32, kp, L.x32, L.t32, L.w32, false,
+                                    nullptr);
+        };
+        rc = pass();   // warm-up
+        if (rc != VX_OK) break;
+        if (hipEventRecord(a, s->stream) != hipSuccess) { rc = fail(s, VX_ERR_HIP, "event"); break; }
+        for (int r = 0; r < reps && rc == VX_OK; ++r) rc = pass();
+        if (rc != VX_OK) break;
+        float ms = 0.f;
+        if (hipEventRecord(b, s->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+            hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+          rc = fail(s, VX_ERR_HIP, "event timing");
+          break;
+        }
+        tms.push_back(ms);
+        best = std::min(best, ms);
+      }
+      L.blk_B = 0;
+      L.blk_S = 0;
+      // the first candidate (the tile launches, then ascending S and B) within
+      // TUNE_TOL of the fastest
+      for (size_t i = 0; i < tms.size() && rc == VX_OK; ++i)
+        if (tms[i] <= best * (1.0f + TUNE_TOL)) {
+          L.blk_B = cand[i].first;
+          L.blk_S = cand[i].second;
+          break;
+        }
+    }
   }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
@@ -2453,6 +2734,22 @@ extern "C" int vx_tuned_zchunk(vx_handle s, int32_t level, int32_t precision, in
   if (!s || level < 0 || level >= (int)s->lv.size() || cls < 0 || cls > 2) return VX_ERR_ARG;
   const Level& L = s->lv[level];
   return zchunk(s, L, precision == VX_FP64 ? 0 : 1, cls);
+}
+
+extern "C" int vx_block_plan(vx_handle s, int32_t level, int32_t steps, int32_t* edge,
+                             int32_t* per_launch) {
+  if (!s || level < 0 || level >= (int)s->lv.size() || !edge || !per_launch) return VX_ERR_ARG;
+  *edge = 0;
+  *per_launch = 0;
+  const int saved = s->nb;
+  s->nb = 1;
+  if (block_eligible(s, level, steps, false)) {
+    const BlockPlan p = block_plan(s, s->lv[level], steps);
+    *edge = p.B;
+    *per_launch = p.S;
+  }
+  s->nb = saved;
+  return VX_OK;
 }
 
 // ===========================================================================
