@@ -193,15 +193,17 @@ def copy_bandwidth(nbytes=2 << 30, reps=10):
 
 
 def kernel_only(p, E_dev, act_dev, MGSolver, z_chunk):
-    """The level-0 FP32 Chebyshev tile kernel 20 times, then a plain 1 GiB
-    FP32 copy 5 times (the reference dispatch for counter scaling).  The
-    z-chunk is fixed (the autotuner's usual pick at 256^3) so that every
-    counter pass sees the same launch."""
+    """The level-0 FP32 Chebyshev tile kernel 20 times, the level-0 FP32
+    operator (masked K u) 20 times, then a plain 1 GiB FP32 copy 5 times (the
+    reference dispatch for counter scaling).  The z-chunk is fixed (the
+    autotuner's usual pick at 256^3) so that every counter pass sees the same
+    launches."""
     import torch
     opt = MGSolver(p.nx, p.ny, p.nz, p.fixed_dofs, nu=p.nu, compute_mode="GPU",
                    **dict(OPT, autotune=False, z_chunk=z_chunk))
     opt.set_density(E_dev, active=act_dev)
     ms = opt.time_kernel("chebyshev", 0, "mixed", 20)
+    ms_mv = opt.time_kernel("matvec", 0, "mixed", 20)
     a = torch.ones((1 << 30) // 4, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
     for _ in range(5):
@@ -210,6 +212,8 @@ def kernel_only(p, E_dev, act_dev, MGSolver, z_chunk):
     nn = (p.nx + 1) * (p.ny + 1) * (p.nz + 1)
     print(json.dumps({"kernel": "chebyshev_fp32_level0", "ms": ms,
                       "algorithmic_bytes": 48 * nn + nn + 4 * p.nelem,
+                      "matvec_fp32_level0_ms": ms_mv,
+                      "matvec_algorithmic_bytes": 24 * nn + nn + 4 * p.nelem,
                       "copy_bytes_read": 1 << 30, "copy_bytes_written": 1 << 30}), flush=True)
 
 
