@@ -229,6 +229,19 @@ def _oracle_solve(p, max_levels=6, active_system=False):
     return u, time.perf_counter() - t0, vf.last_iters, vf.last_converged
 
 
+def vm_rel_err(q, u, ref):
+    """Largest element von Mises difference over the largest von Mises of the
+    reference displacement (both with the part's E; the GPU kernel of
+    core/fea.py:604-626)."""
+    import numpy as np
+    from core.fea import VoxelFEA
+    vf = VoxelFEA(q.nx, q.ny, q.nz, nu=q.nu, active_elems=q.active, compute_mode="GPU")
+    a = vf.element_von_mises_stress(u, q.E_stiffness)
+    b = vf.element_von_mises_stress(ref, q.E_stiffness)
+    vf.close()
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
 def uneven_parts_record():
     """c4: 16 uneven ABS parts on void-padded grids (plain cycle) in one
     vx_solve_parts call, beside the same parts solved one after another;
@@ -285,8 +298,10 @@ def uneven_parts_record():
         ps.set_density([d[0] for d in devs], [d[1] for d in devs])
         got["U"], _ = ps.solve_device([d[2] for d in devs], tol=parts[0].tol, max_cg=5000)
     t = timed(run_parts)
+    vm_errs = []
     for q, u, ue in zip(parts, got["U"], exact):
         errs.append(rel(u.cpu().numpy(), ue, used_of(q)))
+        vm_errs.append(vm_rel_err(q, u.cpu().numpy(), ue))
     rec = {
         "time_s": t, "form": "one vx_solve_parts call (per-part recurrences)",
         "solver": OPT_PARTS,
@@ -295,7 +310,8 @@ def uneven_parts_record():
         "levels": [i["levels"] for i in ps.last_parts],
         "true_resid_max": max(i["true_resid_ratio"] for i in ps.last_parts),
         "all_converged": all(i["converged"] for i in ps.last_parts),
-        "rel_disp_err_vs_fp64_jacobi_1e-10": max(errs)}
+        "rel_disp_err_vs_fp64_jacobi_1e-10": max(errs),
+        "rel_vm_err_vs_fp64_jacobi_1e-10": max(vm_errs)}
     ps.close()
     return rec
 
@@ -305,7 +321,7 @@ def config_records():
     time to tolerance with inputs resident in HBM (second of two runs; the
     first builds the autotuned launches and graphs), iterations, true
     residual and the relative displacement error on the nodes of active
-    elements -- against the reference's own solution where this exact
+    elements and the relative element von Mises error -- against the reference's own solution where this exact
     problem has one (c1: tests/golden/cantilever.npz, written by the
     reference), else against an FP64 solve to 1e-10 (c2, c3: with the same
     preconditioner; c4: Jacobi-PCG, the reference's fallback) at full size;
@@ -350,7 +366,8 @@ def config_records():
                            mg.solve_device(f, u, tol=q.tol)))
         rec[name] = {"time_s": t, "iters": mg.last_iters,
                      "true_resid": mg.last_true_resid_ratio,
-                     "rel_disp_err_vs_reference": rel(u.cpu().numpy(), ref_u, used_of(q))}
+                     "rel_disp_err_vs_reference": rel(u.cpu().numpy(), ref_u, used_of(q)),
+                     "rel_vm_err_vs_reference": vm_rel_err(q, u.cpu().numpy(), ref_u)}
         mg.close()
     out["c1_cantilever_64x16x16"] = rec
     # c2: 128^3 aluminium block with 40 spherical voids, plain 7-level cycle
@@ -369,7 +386,9 @@ def config_records():
     out["c2_voids_block_128"] = {"time_s": t, "iters": mg.last_iters,
                                  "true_resid": mg.last_true_resid_ratio,
                                  "rel_disp_err_vs_fp64_1e-10": rel(u.cpu().numpy(), u_ex,
-                                                                   used_of(q))}
+                                                                   used_of(q)),
+                                 "rel_vm_err_vs_fp64_1e-10": vm_rel_err(q, u.cpu().numpy(),
+                                                                        u_ex)}
     mg.close()
     # c3: 96^3 part, 32 load cases in one batched solve (variable cycle)
     log("configs: c3 multi-RHS part")
@@ -390,7 +409,9 @@ def config_records():
         "time_s": t, "load_cases_per_s": nrhs / t,
         "iters_max": max(b["iters"] for b in mg.last_batch),
         "true_resid_max": max(b["true_resid_ratio"] for b in mg.last_batch),
-        "rel_disp_err_vs_fp64_1e-10": rel(U.cpu().numpy(), U_ex, used_of(q))}
+        "rel_disp_err_vs_fp64_1e-10": rel(U.cpu().numpy(), U_ex, used_of(q)),
+        "rel_vm_err_vs_fp64_1e-10": max(vm_rel_err(q, Ur, Ue)
+                                        for Ur, Ue in zip(U.cpu().numpy(), U_ex))}
     mg.close()
     out["c4_uneven_parts_16"] = uneven_parts_record()
     return out
