@@ -452,6 +452,10 @@ constexpr int coarse_max_elems() { return sizeof(T) == 8 ? 512 : 1024; }
 template <typename T>
 constexpr int coarse_max_nodes() { return sizeof(T) == 8 ? 736 : 1400; }
 constexpr int COARSE_MAX_STEPS = 64;
+// elements and nodes per thread of the one-work-group smoother: FP32 levels
+// up to 1400 nodes take two per thread; FP64 ones (at most 736 nodes) one
+template <typename T>
+constexpr int cs_slots() { return sizeof(T) == 8 ? 1 : 2; }
 
 struct CoarseCoef {
   int steps;
@@ -490,62 +494,92 @@ __device__ __forceinline__ void coarse_smooth_run(int nx, int ny, int nz, const 
     sxp[i] = T(0);
   }
   __syncthreads();
-  if (Ne <= nt && 3 * Nn <= nt) {
-    // one element and one DOF per thread: their LDS indices are fixed, so
-    // they are computed once instead of once per step
-    int en[8];
-    T Ee = T(0);
-    if (tid < Ne) {
-      const int ex = tid % nx, ey = (tid / nx) % ny, ez = tid / (nx * ny);
+  constexpr int CSN = cs_slots<T>();
+  if (Ne <= CSN * nt && Nn <= CSN * nt) {
+    // at most CSN elements and nodes per thread: their LDS indices are
+    // computed once instead of once per step.  The forces are stored
+    // component-major (slot (3q + c) * MAXE + e: consecutive elements in
+    // consecutive banks), and a node reads its 8 slots without branches (a
+    // missing element reads slot 0 and adds 0); b, D^-1, the mask and the
+    // previous iterate of a node stay in its thread's registers.
+    int en[CSN][8];
+    T Ee[CSN];
+    int sl[CSN][8];
+    unsigned okm[CSN], mk[CSN];
+    T mi[CSN], bi[CSN][3], xp[CSN][3];
+#pragma unroll
+    for (int k = 0; k < CSN; ++k) {
+      const int e = tid + k * nt;
+      const int ee = e < Ne ? e : 0;
+      const int ex = ee % nx, ey = (ee / nx) % ny, ez = ee / (nx * ny);
 #pragma unroll
       for (int q = 0; q < 8; ++q)
-        en[q] = 3 * (((ez + (q >> 2)) * Ly + (ey + ((q >> 1) & 1))) * Lx + (ex + (q & 1)));
-      Ee = sE[tid];
-    }
-    int slot[8];
-    bool freed = false;
-    T mi = T(0), bi = T(0);
-    if (tid < 3 * Nn) {
-      const int n = tid / 3, c = tid - 3 * n;
-      const int ix = n % Lx, iy = (n / Lx) % Ly, iz = n / (Lx * Ly);
+        en[k][q] = 3 * (((ez + (q >> 2)) * Ly + (ey + ((q >> 1) & 1))) * Lx + (ex + (q & 1)));
+      Ee[k] = sE[ee];
+      const int n = tid + k * nt;
+      const bool nv = n < Nn;
+      const int nn = nv ? n : 0;
+      const int ix = nn % Lx, iy = (nn / Lx) % Ly, iz = nn / (Lx * Ly);
+      okm[k] = 0u;
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        const int ex = ix - (q & 1), ey = iy - ((q >> 1) & 1), ez = iz - (q >> 2);
-        slot[q] = (ex >= 0 && ex < nx && ey >= 0 && ey < ny && ez >= 0 && ez < nz)
-                      ? ((ez * ny + ey) * nx + ex) * 24 + 3 * q + c
-                      : -1;
+        const int fx = ix - (q & 1), fy = iy - ((q >> 1) & 1), fz = iz - (q >> 2);
+        const bool ok = ((unsigned)fx < (unsigned)nx) & ((unsigned)fy < (unsigned)ny) &
+                        ((unsigned)fz < (unsigned)nz);
+        okm[k] |= ok ? (1u << q) : 0u;
+        sl[k][q] = ok ? 3 * q * MAXE + (fz * ny + fy) * nx + fx : 0;
       }
-      freed = (smk[n] >> c) & 1u;
-      mi = smi[n];
-      bi = sb[tid];
+      mk[k] = nv ? (unsigned)smk[nn] : 0u;
+      mi[k] = smi[nn];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        bi[k][c] = sb[3 * nn + c];
+        xp[k][c] = T(0);
+      }
     }
     for (int st = 1; st < cc.steps; ++st) {
-      if (tid < Ne) {
-        T u[8][3], y[8][3];
 #pragma unroll
-        for (int q = 0; q < 8; ++q)
+      for (int k = 0; k < CSN; ++k) {
+        const int e = tid + k * nt;
+        if (e < Ne) {
+          T u[8][3], y[8][3];
 #pragma unroll
-          for (int c = 0; c < 3; ++c) u[q][c] = sx[en[q] + c];
-        elem_apply<T>(u, Ee, K, y);
+          for (int q = 0; q < 8; ++q)
 #pragma unroll
-        for (int q = 0; q < 8; ++q)
+            for (int c = 0; c < 3; ++c) u[q][c] = sx[en[k][q] + c];
+          elem_apply<T>(u, Ee[k], K, y);
 #pragma unroll
-          for (int c = 0; c < 3; ++c) sye[tid * 24 + 3 * q + c] = y[q][c];
+          for (int q = 0; q < 8; ++q)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) sye[(3 * q + c) * MAXE + e] = y[q][c];
+        }
       }
       __syncthreads();
-      if (tid < 3 * Nn) {
-        T kx = T(0);
+      const T w0 = (T)cc.w0, cu = (T)cc.cu[st], cr = (T)cc.cr[st];
 #pragma unroll
-        for (int q = 0; q < 8; ++q)
-          if (slot[q] >= 0) kx += sye[slot[q]];
-        const T x = sx[tid];
-        T xn = x;
-        if (freed) {
-          const T r = mi * (bi - kx);
-          xn = cc.cheb ? x + (T)cc.cu[st] * (x - sxp[tid]) + (T)cc.cr[st] * r : x + (T)cc.w0 * r;
+      for (int k = 0; k < CSN; ++k) {
+        const int n = tid + k * nt;
+        if (n < Nn) {
+          T v[8][3];
+#pragma unroll
+          for (int q = 0; q < 8; ++q)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) v[q][c] = sye[sl[k][q] + c * MAXE];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            T kx = T(0);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) kx += ((okm[k] >> q) & 1u) ? v[q][c] : T(0);
+            const T x = sx[3 * n + c];
+            T xn = x;
+            if ((mk[k] >> c) & 1u) {
+              const T r = mi[k] * (bi[k][c] - kx);
+              xn = cc.cheb ? x + cu * (x - xp[k][c]) + cr * r : x + w0 * r;
+            }
+            xp[k][c] = x;
+            sx[3 * n + c] = xn;
+          }
         }
-        sxp[tid] = x;
-        sx[tid] = xn;
       }
       __syncthreads();
     }

@@ -470,7 +470,7 @@ int parts_build(vx_parts* P) {
       V.nn += align_up(nn);
       V.ne += align_up(ne);
       if (q >= V.m && q < V.mc) {
-        const long long work = std::max(ne, 3 * nn);
+        const long long work = std::max(ne, nn);
         V.coarse_threads = (int)std::max<long long>(
             V.coarse_threads, std::min<long long>(1024, (work + 63) / 64 * 64));
       }

@@ -814,9 +814,9 @@ template <typename T>
 int coarse_fused(vx_solver* s, Level& L, const T* b, T* x) {
   const int k = s->opt.coarse_sweeps;
   const CoarseCoef cc = smooth_coef(s, k);
-  // as many threads as the larger of the element and DOF loops needs (whole
-  // waves, at most 1024): fewer waves make each step's two barriers cheaper
-  const long long work = std::max<long long>(L.Ne, 3 * L.Nn);
+  // one element and one node per thread where 1024 threads allow (whole
+  // waves; at most cs_slots<T>() of each per thread at the size limits)
+  const long long work = std::max<long long>(L.Ne, L.Nn);
   const int threads = (int)std::min<long long>(1024, std::max<long long>(64, (work + 63) / 64 * 64));
   hipLaunchKernelGGL(k_coarse_smooth<T>, dim3(s->nb), dim3(threads), 0, s->stream, L.nx, L.ny, L.nz,
                      Sel<T>::E(L), L.mask, Sel<T>::mi(L), b, x, kconst<T>(s), cc);
