@@ -707,7 +707,8 @@ __global__ __launch_bounds__(BOT_THREADS) void k_bottom(BottomArgs a, CoarseCoef
   // slots in sye (-1: no element), in corner order q = x + 2y + 4z.
   struct Tab {
     int en[8];
-    int sl[8];
+    int sl[8];     // component-0 slot of the node's element at corner q (0 if none)
+    unsigned ok;   // bit q: that element exists
   };
   auto make_tab = [&](int ex_, int ey_, int ez_) {
     Tab t;
@@ -717,13 +718,15 @@ __global__ __launch_bounds__(BOT_THREADS) void k_bottom(BottomArgs a, CoarseCoef
     const int ex = e % ex_, ey = (e / ex_) % ey_, ez = e / (ex_ * ey_);
     const int n = tid < nn ? tid : 0;
     const int ix = n % lx, iy = (n / lx) % ly, iz = n / (lx * ly);
+    t.ok = 0u;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       t.en[q] = ((ez + (q >> 2)) * ly + (ey + ((q >> 1) & 1))) * lx + (ex + (q & 1));
       const int fx = ix - (q & 1), fy = iy - ((q >> 1) & 1), fz = iz - (q >> 2);
-      t.sl[q] = (fx >= 0 && fx < ex_ && fy >= 0 && fy < ey_ && fz >= 0 && fz < ez_)
-                    ? ((fz * ey_ + fy) * ex_ + fx) * 24 + 3 * q
-                    : -1;
+      const bool ok = ((unsigned)fx < (unsigned)ex_) & ((unsigned)fy < (unsigned)ey_) &
+                      ((unsigned)fz < (unsigned)ez_);
+      t.ok |= ok ? (1u << q) : 0u;
+      t.sl[q] = ok ? 3 * q * BOT_MAXE + (fz * ey_ + fy) * ex_ + fx : 0;
     }
     return t;
   };
@@ -741,16 +744,25 @@ __global__ __launch_bounds__(BOT_THREADS) void k_bottom(BottomArgs a, CoarseCoef
 #pragma unroll
       for (int q = 0; q < 8; ++q)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) sye[tid * 24 + 3 * q + c] = y[q][c];
+        for (int c = 0; c < 3; ++c) sye[(3 * q + c) * BOT_MAXE + tid] = y[q][c];
     }
   };
-  // node n = tid: the sum of its <= 8 element slots for component c
-  auto node_sum = [&](const Tab& t, int c) -> float {
-    float kx = 0.f;
+  // node n = tid: the sums of its <= 8 element slots per component, in corner
+  // order (the slots are component-major, so consecutive elements sit in
+  // consecutive banks; all 24 are read before the sums, a missing element
+  // adds 0)
+  auto node_sums = [&](const Tab& t, float (&kx)[3]) {
+    float v[8][3];
 #pragma unroll
     for (int q = 0; q < 8; ++q)
-      if (t.sl[q] >= 0) kx += sye[t.sl[q] + c];
-    return kx;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) v[q][c] = sye[t.sl[q] + c * BOT_MAXE];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      kx[c] = 0.f;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) kx[c] += ((t.ok >> q) & 1u) ? v[q][c] : 0.f;
+    }
   };
   // k smoothing steps on a level in LDS; from zero (x1 = w0 D^-1 b, as
   // k_zero_step), or from X (first step x + w0 D^-1 r, as M_JACOBI); then
@@ -781,14 +793,15 @@ __global__ __launch_bounds__(BOT_THREADS) void k_bottom(BottomArgs a, CoarseCoef
       if (tid < nn) {
         const unsigned mk = MK[tid];
         const float mi = MI[tid];
+        float kx[3];
+        node_sums(t, kx);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           const int i = 3 * tid + c;
-          const float kx = node_sum(t, c);
           const float x = X[i];
           float xn = x;
           if ((mk >> c) & 1u) {
-            const float r = mi * (B[i] - kx);
+            const float r = mi * (B[i] - kx[c]);
             xn = jac ? fmaT(w0, r, x) : x + fmaT(cu, x - XP[i], cr * r);
           }
           XP[i] = x;
@@ -806,11 +819,10 @@ __global__ __launch_bounds__(BOT_THREADS) void k_bottom(BottomArgs a, CoarseCoef
   __syncthreads();
   if (tid < Nn) {
     const unsigned mk = smk[tid];
+    float kx[3];
+    node_sums(tf, kx);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const float kx = node_sum(tf, c);
-      sxp[3 * tid + c] = sb[3 * tid + c] - (((mk >> c) & 1u) ? kx : 0.f);
-    }
+    for (int c = 0; c < 3; ++c) sxp[3 * tid + c] = sb[3 * tid + c] - (((mk >> c) & 1u) ? kx[c] : 0.f);
   }
   __syncthreads();
   // restriction: taps in (z, y, x) order, weights 1 or 1/2 per axis
