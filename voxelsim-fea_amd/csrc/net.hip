@@ -231,20 +231,34 @@ __global__ void k_reduce_wgrad(const float* __restrict__ partial, int nw, long l
   }
 }
 
-// db[c] = sum_v G[v][c]; one work-group per channel, fixed reduction order
-__global__ __launch_bounds__(256) void k_bias_grad(const uint16_t* __restrict__ G, long long nv,
-                                                   int C, float* __restrict__ db) {
+// db[c] = sum_v G[v][c] in two fixed-order stages.  Stage 1: work-group b
+// sums rows [b R, (b+1) R) of G[v][C]; thread t takes channel t % C and rows
+// t / C, t / C + 256 / C, ... (consecutive threads read consecutive values),
+// then the 256 / C row lanes of each channel are added in order.  C divides
+// 256 (16, 32 or 64 channels).
+__global__ __launch_bounds__(256) void k_bias_part(const uint16_t* __restrict__ G, long long nv,
+                                                   int C, long long R, float* __restrict__ part) {
   __shared__ float red[256];
-  const int c = blockIdx.x;
+  const int t = threadIdx.x, c = t % C, rl = t / C, nrl = 256 / C;
+  const long long r0 = blockIdx.x * R, r1 = min(nv, r0 + R);
   float s = 0.f;
-  for (long long v = threadIdx.x; v < nv; v += 256) s += bf2f(G[v * C + c]);
-  red[threadIdx.x] = s;
+  for (long long v = r0 + rl; v < r1; v += nrl) s += bf2f(G[v * C + c]);
+  red[t] = s;
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-    __syncthreads();
+  if (t < C) {
+    float a = 0.f;
+    for (int k = 0; k < nrl; ++k) a += red[k * C + t];
+    part[(long long)blockIdx.x * C + t] = a;
   }
-  if (threadIdx.x == 0) db[c] = red[0];
+}
+
+// Stage 2: db[c] = sum over the stage-1 blocks b of part[b][c], in order.
+__global__ void k_bias_sum(const float* __restrict__ part, int nb, int C, float* __restrict__ db) {
+  const int c = threadIdx.x;
+  if (c >= C) return;
+  float a = 0.f;
+  for (int b = 0; b < nb; ++b) a += part[(long long)b * C + c];
+  db[c] = a;
 }
 
 // g = bf16(relu ? (h > 0 ? d : 0) : d)
@@ -614,8 +628,18 @@ int layer_bwd(vx_net* n, int i, const float* d, const uint16_t* x, float* grad, 
   // waves past the last k-block wrote zero partials: reduce over all launched waves
   hipLaunchKernelGGL(k_reduce_wgrad, dim3(grid1(wsz)), dim3(256), 0, n->stream, n->partial,
                      nchunk * 4, wsz, grad + l.woff);
-  hipLaunchKernelGGL(k_bias_grad, dim3(l.cout), dim3(256), 0, n->stream, g, n->V[lo] * n->B,
-                     l.cout, grad + l.boff);
+  // bias gradient; its stage-1 partials (at most 256 x cout floats) reuse
+  // n->partial, which holds at least 4 x 27 x cin x cout floats and which
+  // k_reduce_wgrad, ordered before on the stream, has consumed
+  {
+    const long long nv = n->V[lo] * n->B;
+    const long long R = std::max<long long>(1024, (nv + 255) / 256);
+    const int nbb = (int)((nv + R - 1) / R);
+    hipLaunchKernelGGL(k_bias_part, dim3(nbb), dim3(256), 0, n->stream, g, nv, l.cout, R,
+                       n->partial);
+    hipLaunchKernelGGL(k_bias_sum, dim3(1), dim3(64), 0, n->stream, n->partial, nbb, l.cout,
+                       grad + l.boff);
+  }
   NLAUNCH();
   if (!dx) return VX_OK;
   if (acc)
