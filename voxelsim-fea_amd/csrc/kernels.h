@@ -1219,8 +1219,10 @@ __device__ __forceinline__ void restrict_run(int Lcx, int Lcy, int Lcz, const Tr
   const long long fplane = 3LL * Lfx * Lfy;
   const int jz0 = chunk * g.per_chunk;
   const int jz1 = min(jz0 + g.per_chunk, Lcz);
-  // x-y taps of this column (y outer, x inner): fine offsets and weights,
-  // weight 0 for taps outside the grid (skipped)
+  // x-y taps of this column (y outer, x inner): fine offsets and weights;
+  // a tap outside the grid reads the column's own node with weight 0, so
+  // every tap is loaded without a branch (per-lane branches around loads made
+  // each load wait for the one before it)
   int off[9];
   T w[9];
 #pragma unroll
@@ -1230,53 +1232,59 @@ __device__ __forceinline__ void restrict_run(int Lcx, int Lcy, int Lcz, const Tr
       const int k = (dy + 1) * 3 + (dx + 1);
       const int x = 2 * jx + dx, y = 2 * jy + dy;
       const bool in = x >= 0 && x < Lfx && y >= 0 && y < Lfy;
-      off[k] = in ? 3 * (y * Lfx + x) : 0;
+      off[k] = in ? 3 * (y * Lfx + x) : 3 * (2 * jy * Lfx + 2 * jx);
       w[k] = in ? (dx == 0 ? T(1) : T(0.5)) * (dy == 0 ? T(1) : T(0.5)) : T(0);
     }
-  // acc += sum over the taps of plane z, weight scale * w
-  auto add_plane = [&](int z, T sc, T acc[3]) {
+  auto load_plane = [&](int z, T (&q)[9][3]) {
     const T* P = rf + (long long)z * fplane;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
-      if (w[k] == T(0)) continue;
-      const T wk = sc * w[k];
-      const T* q = P + off[k];
-      acc[0] += wk * q[0];
-      acc[1] += wk * q[1];
-      acc[2] += wk * q[2];
-    }
+    for (int k = 0; k < 9; ++k)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) q[k][c] = P[off[k] + c];
   };
   T h[3] = {T(0), T(0), T(0)};   // partial sum of plane 2 jz - 1 (weight 1/2)
-  if (jz0 > 0) add_plane(2 * jz0 - 1, T(0.5), h);
-  for (int jz = jz0; jz < jz1; ++jz) {
-    T a[3] = {h[0], h[1], h[2]};
-    add_plane(2 * jz, T(1), a);
-    if (2 * jz + 1 < Lfz) {
-      const T* P = rf + (long long)(2 * jz + 1) * fplane;
-      h[0] = h[1] = h[2] = T(0);
+  if (jz0 > 0) {
+    T q[9][3];
+    load_plane(2 * jz0 - 1, q);
 #pragma unroll
-      for (int k = 0; k < 9; ++k) {
-        if (w[k] == T(0)) continue;
-        const T wk = T(0.5) * w[k];
-        const T* q = P + off[k];
-        const T q0 = q[0], q1 = q[1], q2 = q[2];
-        a[0] += wk * q0;
-        a[1] += wk * q1;
-        a[2] += wk * q2;
-        h[0] += wk * q0;
-        h[1] += wk * q1;
-        h[2] += wk * q2;
-      }
-    }
+    for (int k = 0; k < 9; ++k)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) h[c] = fmaT(T(0.5) * w[k], q[k][c], h[c]);
+  }
+  for (int jz = jz0; jz < jz1; ++jz) {
+    // the coarse node's mask and D^-1 and both fine planes' taps are loaded
+    // before the first wait.  Past the grid's last plane the odd plane is the
+    // even one again, with weight 0 (the loop's last step: h is not used).
+    const bool odd = 2 * jz + 1 < Lfz;   // uniform
     const long long J = ((long long)jz * Lcy + jy) * Lcx + jx;
     const unsigned mk = cmask[J];
+    const T cmi = x1 ? cminv[J] : T(0);
+    T q0[9][3], q1[9][3];
+    load_plane(2 * jz, q0);
+    load_plane(odd ? 2 * jz + 1 : 2 * jz, q1);
+    T a[3] = {h[0], h[1], h[2]};
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) a[c] = fmaT(w[k], q0[k][c], a[c]);
+    const T so = odd ? T(0.5) : T(0);
+    h[0] = h[1] = h[2] = T(0);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const T wk = so * w[k];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        a[c] = fmaT(wk, q1[k][c], a[c]);
+        h[c] = fmaT(wk, q1[k][c], h[c]);
+      }
+    }
     rc[3 * J] = (mk & 1u) ? a[0] : T(0);
     rc[3 * J + 1] = (mk & 2u) ? a[1] : T(0);
     rc[3 * J + 2] = (mk & 4u) ? a[2] : T(0);
     if (x1) {
       // the coarse level's first smoothing step from zero, as k_zero_step
       // forms it from rc
-      const T wm = w0 * cminv[J];
+      const T wm = w0 * cmi;
       x1[3 * J] = (mk & 1u) ? wm * a[0] : T(0);
       x1[3 * J + 1] = (mk & 2u) ? wm * a[1] : T(0);
       x1[3 * J + 2] = (mk & 4u) ? wm * a[2] : T(0);
@@ -1326,34 +1334,45 @@ __device__ __forceinline__ void prolong_run(int Lcx, int Lcy, int Lcz, const TrG
   const int cx = ix >> 1, cy = iy >> 1;
   const bool ox = ix & 1, oy = iy & 1;
   const long long cplane = 3LL * Lcx * Lcy;
-  // the coarse taps of this value inside a coarse plane, (ky, kx) order
+  // the coarse taps of this value inside a coarse plane, (ky, kx) order; a tap
+  // the value does not use repeats t00 and adds 0, so all four are loaded
+  // without a branch (per-lane branches around the loads made each load wait
+  // for the one before it)
   const int t00 = 3 * (cy * Lcx + cx) + c;
-  const int t01 = t00 + 3, t10 = t00 + 3 * Lcx, t11 = t10 + 3;
-  auto csum = [&](int cz) -> T {
+  const int t01 = t00 + (ox ? 3 : 0), t10 = t00 + (oy ? 3 * Lcx : 0);
+  const int t11 = t10 + (ox ? 3 : 0);
+  const bool oxy = ox && oy;
+  auto taps = [&](int cz, T (&v)[4]) {
     const T* P = ec + (long long)cz * cplane;
-    T sacc = P[t00];
-    if (ox) sacc += P[t01];
-    if (oy) {
-      sacc += P[t10];
-      if (ox) sacc += P[t11];
-    }
+    v[0] = P[t00];
+    v[1] = P[t01];
+    v[2] = P[t10];
+    v[3] = P[t11];
+  };
+  auto cont = [&](const T (&v)[4], T sacc) -> T {   // sacc continued with a plane's taps
+    sacc += v[0];
+    sacc += ox ? v[1] : T(0);
+    sacc += oy ? v[2] : T(0);
+    sacc += oxy ? v[3] : T(0);
     return sacc;
   };
-  auto cont = [&](int cz, T sacc) -> T {   // sacc continued with plane cz's taps
-    const T* P = ec + (long long)cz * cplane;
-    sacc += P[t00];
-    if (ox) sacc += P[t01];
-    if (oy) {
-      sacc += P[t10];
-      if (ox) sacc += P[t11];
-    }
+  auto csum = [&](const T (&v)[4]) -> T {
+    T sacc = v[0];
+    sacc += ox ? v[1] : T(0);
+    sacc += oy ? v[2] : T(0);
+    sacc += oxy ? v[3] : T(0);
     return sacc;
   };
   const T wxy = (ox ? T(0.5) : T(1)) * (oy ? T(0.5) : T(1));
   const int iz0 = chunk * g.per_chunk;
   const int iz1 = min(iz0 + g.per_chunk, Lfz);
   const long long fp3 = 3LL * fplane;
-  T lo = csum(iz0 >> 1);
+  T lo;
+  {
+    T v[4];
+    taps(iz0 >> 1, v);
+    lo = csum(v);
+  }
   // four fine planes (two coarse planes) per step, every load issued before
   // the first store: more loads in flight per wave
   for (int iz = iz0; iz < iz1; iz += 4) {
@@ -1370,13 +1389,16 @@ __device__ __forceinline__ void prolong_run(int Lcx, int Lcy, int Lcz, const TrG
     const T vc = h2 ? x[ia + 2 * fp3] : T(0);
     const T vd = h3 ? x[ia + 3 * fp3] : T(0);
     T s1 = T(0), v1 = T(0), s2 = T(0), v3 = T(0);
+    T p1[4], p2[4];
+    if (h1) taps(cz + 1, p1);
+    if (h3) taps(cz + 2, p2);
     if (h1) {
-      s1 = csum(cz + 1);        // plane cz+1's own sum
-      v1 = cont(cz + 1, lo);    // plane cz's sum continued with plane cz+1
+      s1 = csum(p1);        // plane cz+1's own sum
+      v1 = cont(p1, lo);    // plane cz's sum continued with plane cz+1
     }
     if (h3) {
-      s2 = csum(cz + 2);
-      v3 = cont(cz + 2, s1);
+      s2 = csum(p2);
+      v3 = cont(p2, s1);
     }
     // every value is stored (unchanged where masked): whole cache lines
     if ((ma >> c) & 1u) va += lo * wxy;
