@@ -198,6 +198,32 @@ def test_islands_vs_oracle(rng):
         k, n = drop_unsupported_islands(act, fixed)
         kr, nr = ref_drop(act, fixed)
         assert n == nr and np.array_equal(k, kr)
+    # other support faces and shapes: random grids fixed on the x = 0 face (its
+    # nodes are x-slowest blocks of the node order: every node with ix == 0)
+    for seed, (nx, ny, nz) in ((3, (31, 17, 24)), (4, (9, 40, 15)), (5, (24, 24, 24))):
+        r = np.random.default_rng(seed)
+        act = r.random((nx, ny, nz)) < 0.5
+        ix = np.arange((nx + 1) * (ny + 1) * (nz + 1)) % (nx + 1)
+        fixed = (3 * np.flatnonzero(ix == 0)[:, None] + np.arange(3)).ravel()
+        k, n = drop_unsupported_islands(act, fixed)
+        kr, nr = ref_drop(act, fixed)
+        assert n == nr and np.array_equal(k, kr), seed
+    # a serpentine corridor (rows along x joined alternately at the two ends,
+    # layers joined alternately too): the fill must turn back in x many times
+    nx, ny, nz = 16, 11, 5
+    act = np.zeros((nx, ny, nz), dtype=bool)
+    for z in range(0, nz, 2):
+        for y in range(0, ny, 2):
+            act[:, y, z] = True
+            if y + 2 < ny:
+                act[(nx - 1) if (y // 2) % 2 == 0 else 0, y + 1, z] = True
+        if z + 2 < nz:
+            act[0 if ((ny - 1) // 2) % 2 == 0 else nx - 1, ny - 1, z + 1] = True
+    act[5, 3, 1] = True     # an isolated element: dropped
+    fixed = np.arange(3 * (nx + 1) * (ny + 1))   # z = 0 plane
+    k, n = drop_unsupported_islands(act, fixed)
+    kr, nr = ref_drop(act, fixed)
+    assert n == nr and np.array_equal(k, kr)
     # nothing reaches a support: unchanged
     act = np.zeros((6, 6, 6), dtype=bool)
     act[2:4, 2:4, 3:5] = True

@@ -2086,34 +2086,70 @@ __global__ void k_island_seed(int nx, int ny, int nz, const uint8_t* __restrict_
   }
 }
 
-// One monotone dilation pass, in place: an active element becomes reached
-// when a face neighbour is.  Lines along z are swept forward and back so a
-// pass propagates far; the fixed point (the 6-connected components holding a
-// seed) is independent of the order.
-__global__ void k_island_grow(int nx, int ny, int nz, const uint8_t* __restrict__ active,
-                              uint8_t* reached, int* changed) {
-  const long long lines = (long long)nx * ny;
+// One monotone dilation pass along the lines of one axis, in place: an
+// active element becomes reached when a face neighbour is.  Each thread sweeps
+// one line forward and back, carrying the state of the element behind it in a
+// register, so the only serial dependence is that register.  The line's
+// elements are loaded ISL_U at a time before any of them is stored: none of
+// those loads reads an element this thread stores earlier in the group (an
+// element's own value is loaded before its store; the one ahead is read as it
+// was, which a monotone fill allows).  Neighbours outside the grid, and
+// positions past the line's end, read an element of the line and are masked,
+// so the loads carry no branches.  Consecutive threads take consecutive z (y
+// for lines along z).  The fixed point, the 6-connected components holding a
+// seed, does not depend on the order of the passes; a pass along any axis
+// checks all six neighbours, so a pass that changes nothing ends the fill.
+constexpr int ISL_U = 8;
+
+__global__ void k_island_sweep(int nx, int ny, int nz, int axis, const uint8_t* __restrict__ active,
+                               uint8_t* reached, int* changed) {
+  // element e = iz + nz (iy + ny ix); a = swept axis, b and c the others
+  const long long sX = (long long)ny * nz, sY = nz;
+  long long sa, sb, sc;
+  int na, nb, nc;
+  if (axis == 0) {
+    sa = sX; na = nx; sb = sY; nb = ny; sc = 1; nc = nz;
+  } else if (axis == 1) {
+    sa = sY; na = ny; sb = sX; nb = nx; sc = 1; nc = nz;
+  } else {
+    sa = 1; na = nz; sb = sX; nb = nx; sc = sY; nc = ny;
+  }
+  const long long lines = (long long)nb * nc;
   for (long long l = blockIdx.x * (long long)blockDim.x + threadIdx.x; l < lines;
        l += (long long)gridDim.x * blockDim.x) {
-    const int iy = (int)(l % ny);
-    const int ix = (int)(l / ny);
-    const long long base = ((long long)ix * ny + iy) * nz;
+    const int ic = (int)(l % nc), ib = (int)(l / nc);
+    const long long base = ib * sb + ic * sc;
+    const bool hb0 = ib > 0, hb1 = ib + 1 < nb, hc0 = ic > 0, hc1 = ic + 1 < nc;
+    const long long ob0 = hb0 ? -sb : 0, ob1 = hb1 ? sb : 0, oc0 = hc0 ? -sc : 0,
+                    oc1 = hc1 ? sc : 0;
     int any = 0;
-    for (int pass = 0; pass < 2; ++pass) {
-      for (int k = 0; k < nz; ++k) {
-        const int iz = pass == 0 ? k : nz - 1 - k;
-        const long long e = base + iz;
-        if (!active[e] || reached[e]) continue;
-        bool r = false;
-        if (iz > 0 && reached[e - 1]) r = true;
-        if (iz + 1 < nz && reached[e + 1]) r = true;
-        if (iy > 0 && reached[e - nz]) r = true;
-        if (iy + 1 < ny && reached[e + nz]) r = true;
-        if (ix > 0 && reached[e - (long long)ny * nz]) r = true;
-        if (ix + 1 < nx && reached[e + (long long)ny * nz]) r = true;
-        if (r) {
-          reached[e] = 1;
-          any = 1;
+    for (int dir = 0; dir < 2; ++dir) {
+      const long long step = dir == 0 ? sa : -sa;
+      const long long e0 = dir == 0 ? base : base + (long long)(na - 1) * sa;
+      unsigned prev = 0u;   // reached state of the element behind (none before the line)
+      for (int t0 = 0; t0 < na; t0 += ISL_U) {
+        unsigned act[ISL_U], cur[ISL_U], nbr[ISL_U];
+#pragma unroll
+        for (int j = 0; j < ISL_U; ++j) {
+          const int t = t0 + j;
+          const bool in = t < na, ahead = t + 1 < na;
+          const long long e = e0 + (long long)(in ? t : na - 1) * step;
+          const unsigned va = active[e], vc = reached[e], vf = reached[e + (ahead ? step : 0)];
+          const unsigned x0 = reached[e + ob0], x1 = reached[e + ob1];
+          const unsigned y0 = reached[e + oc0], y1 = reached[e + oc1];
+          act[j] = (in && va) ? 1u : 0u;
+          cur[j] = vc ? 1u : 0u;
+          nbr[j] = ((ahead && vf) | (hb0 && x0) | (hb1 && x1) | (hc0 && y0) | (hc1 && y1)) ? 1u : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < ISL_U; ++j) {
+          if (t0 + j >= na) break;
+          const unsigned nv = cur[j] | (act[j] & (prev | nbr[j]));
+          if (nv && !cur[j]) {
+            reached[e0 + (long long)(t0 + j) * step] = 1;
+            any = 1;
+          }
+          prev = nv;
         }
       }
     }

@@ -2316,36 +2316,70 @@ int vx_von_mises(vx_handle s, const double* u, const double* E, const uint8_t* a
   return VX_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// nf[d / 3] = 1 for each fixed DOF d (already range-checked)
+__global__ void k_mark_fixed_nodes(long long n, const int64_t* __restrict__ fixed,
+                                   uint8_t* __restrict__ nf) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    nf[fixed[i] / 3] = 1;
+}
+
+// one dilation pass along each axis (x, y, z); *changed is set when any
+// element was reached
+int island_sweeps(vx_solver* s, const Level& L, const uint8_t* active, uint8_t* kept,
+                  int* changed) {
+  const long long lines[3] = {(long long)L.ny * L.nz, (long long)L.nx * L.nz,
+                              (long long)L.nx * L.ny};
+  for (int axis = 0; axis < 3; ++axis)
+    hipLaunchKernelGGL(k_island_sweep, dim3(ew_grid(lines[axis])), dim3(EW_THREADS), 0, s->stream,
+                       L.nx, L.ny, L.nz, axis, active, kept, changed);
+  LAUNCHCHK();
+  return VX_OK;
+}
+}  // namespace
+
+extern "C" {
+
 int vx_drop_islands(vx_handle s, const uint8_t* active, const int64_t* fixed, int64_t n_fixed,
                     uint8_t* kept, int64_t* n_removed) {
   if (!s || !active || !kept || (n_fixed > 0 && !fixed))
     return fail(s, VX_ERR_ARG, "bad island arguments");
   Level& L = s->lv[0];
-  std::vector<uint8_t> nf(L.Nn, 0);
-  for (int64_t i = 0; i < n_fixed; ++i) {
+  for (int64_t i = 0; i < n_fixed; ++i)
     if (fixed[i] < 0 || fixed[i] >= 3 * L.Nn) return fail(s, VX_ERR_ARG, "fixed DOF out of range");
-    nf[fixed[i] / 3] = 1;
-  }
+  // the fixed-node flags are marked on the device from the DOF list (a host
+  // array of Nn flags and its copy cost more than the fill at 256^3)
   uint8_t* dnf = nullptr;
+  int64_t* dfix = nullptr;
   int* dchg = nullptr;
   unsigned long long* dcnt = nullptr;
   HIPCHK(hipMalloc(&dnf, L.Nn));
+  HIPCHK(hipMalloc(&dfix, sizeof(int64_t) * std::max<int64_t>(n_fixed, 1)));
   HIPCHK(hipMalloc(&dchg, sizeof(int)));
   HIPCHK(hipMalloc(&dcnt, 2 * sizeof(unsigned long long)));
   auto done = [&]() {
     (void)hipFree(dnf);
+    (void)hipFree(dfix);
     (void)hipFree(dchg);
     (void)hipFree(dcnt);
   };
-  HIPCHK(hipMemcpyAsync(dnf, nf.data(), L.Nn, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipMemsetAsync(dnf, 0, L.Nn, s->stream));
+  if (n_fixed > 0) {
+    HIPCHK(hipMemcpyAsync(dfix, fixed, sizeof(int64_t) * n_fixed, hipMemcpyHostToDevice, s->stream));
+    hipLaunchKernelGGL(k_mark_fixed_nodes, dim3(ew_grid(n_fixed)), dim3(EW_THREADS), 0, s->stream,
+                       (long long)n_fixed, dfix, dnf);
+    LAUNCHCHK();
+  }
   hipLaunchKernelGGL(k_island_seed, dim3(ew_grid(L.Ne)), dim3(EW_THREADS), 0, s->stream, L.nx,
                      L.ny, L.nz, active, dnf, kept);
   int hchg = 1;
   long long passes = 0;
   while (hchg) {
     HIPCHK(hipMemsetAsync(dchg, 0, sizeof(int), s->stream));
-    hipLaunchKernelGGL(k_island_grow, dim3(ew_grid((long long)L.nx * L.ny)), dim3(EW_THREADS), 0,
-                       s->stream, L.nx, L.ny, L.nz, active, kept, dchg);
+    CHK(island_sweeps(s, L, active, kept, dchg));
     HIPCHK(hipMemcpyAsync(&hchg, dchg, sizeof(int), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     if (++passes > 4 * (L.Ne + 1)) {
@@ -2495,9 +2529,7 @@ int vx_drop_islands_noalloc(vx_handle s, const uint8_t* active, const int64_t* f
   long long passes = 0;
   do {
     hipLaunchKernelGGL(k_noalloc_reset_changed, dim3(1), dim3(1), 0, s->stream, sc);
-    hipLaunchKernelGGL(k_island_grow, dim3(ew_grid((long long)L.nx * L.ny)), dim3(EW_THREADS), 0,
-                       s->stream, L.nx, L.ny, L.nz, active, kept, &dh->changed);
-    LAUNCHCHK();
+    CHK(island_sweeps(s, L, active, kept, &dh->changed));
     CHK(noalloc_read_head(s, sc, &h));
     if (++passes > 4 * (L.Ne + 1)) return fail(s, VX_ERR_HIP, "island flood fill did not terminate");
   } while (h.changed);
