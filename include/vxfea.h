@@ -96,6 +96,14 @@ typedef struct {
                                  2-8: blocked with that many (at most) */
   int32_t block_edge;         /* owned box edge (nodes) of the blocked smoother's work-groups,
                                  2-16; 0: chosen as block_steps is */
+  int32_t keep_direction;     /* mixed, without the corrector: 1 = each FP32 inner solve of the
+                                 iterative refinement continues the previous one's search
+                                 direction (rescaled to the new residual) instead of restarting
+                                 CG, a "reliable update" of the recurrence residual by the FP64
+                                 true residual; 0 (default) = restart */
+  double keep_drift;          /* keep_direction: a refresh that moves the residual by more than
+                                 this (relative to the new residual's norm) restarts the inner
+                                 solve all the same; 0: no limit */
 } vx_options;
 
 /* Per-solve statistics (mirrors MGSolver.last_* of core/multigrid.py:482-517). */

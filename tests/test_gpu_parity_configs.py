@@ -42,9 +42,10 @@ def load_config(name):
     return {k: g[k] for k in g.files}
 
 
-def _solver(g, preset, precision, batched, pad):
+def _solver(g, preset, precision, batched, pad, **extra):
     from core.multigrid import MGSolver, PLAIN_CYCLE, VARIABLE_CYCLE
     kw = dict(VARIABLE_CYCLE if preset == "variable" else PLAIN_CYCLE, precision=precision)
+    kw.update(extra)
     nrhs = np.atleast_2d(g["force"]).shape[0]
     return MGSolver(int(g["nx"]), int(g["ny"]), int(g["nz"]), g["fixed_dofs"], nu=float(g["nu"]),
                     compute_mode="GPU", pad=pad, max_batch=nrhs if batched else 1, **kw)
@@ -112,6 +113,26 @@ def test_benchmark_preset_matches_reference_solution(name, precision):
     for q, (eu, ev) in enumerate(_errors(g, U6)):
         print(f"  tol 1e-6 rhs {q}: rel u err {eu:.2e} rel vm err {ev:.2e}", flush=True)
         assert eu < 1e-4 and ev < 1e-4
+    mg.close()
+
+
+@pytest.mark.parametrize("name", ["c5_two_material24", "c3_multi_rhs_part32"])
+def test_keep_direction_matches_reference_solution(name):
+    """Mixed refinement whose FP32 inner solves continue the previous one's
+    search direction across each FP64 residual refresh (inner solves to 0.2,
+    so about a dozen refreshes to 1e-8; the batched case has load cases that
+    stop at different refreshes).  Same bars as the restarting form."""
+    preset, batched, pad = CASES[name]
+    print(f"case keep-direction {name} batched={batched}", flush=True)
+    g = load_config(name)
+    mg = _solver(g, preset, "mixed", batched, pad, inner_tol=0.2, keep_direction=True)
+    U, true = _solve(g, mg, 1e-8, batched)
+    assert all(t < 1e-8 for t in true), true
+    outer = ([b["outer_iters"] for b in mg.last_batch] if batched else [mg.last_outer_iters])
+    assert min(outer) >= 5, outer
+    for q, (eu, ev) in enumerate(_errors(g, U)):
+        print(f"  rhs {q}: outer {outer} rel u err {eu:.2e} rel vm err {ev:.2e}", flush=True)
+        assert eu <= BAR["mixed"] and ev <= BAR["mixed"], (eu, ev)
     mg.close()
 
 

@@ -285,6 +285,18 @@ def test_create_rejects_bad_block_options():
         assert _create_status(block_steps=steps, block_edge=edge) in ok, (steps, edge)
 
 
+def test_create_rejects_bad_keep_direction_options():
+    """keep_direction is a 0/1 switch and keep_drift a limit >= 0 (0: none);
+    a NaN limit is rejected too."""
+    from core import _native
+    ok = (_native.VX_OK, _native.VX_ERR_NO_DEVICE)
+    bad = _native.VX_ERR_ARG
+    for keep, drift in ((2, 0.0), (-1, 0.0), (1, -0.1), (1, float("nan"))):
+        assert _create_status(keep_direction=keep, keep_drift=drift) == bad, (keep, drift)
+    for keep, drift in ((0, 0.0), (1, 0.0), (1, 0.3), (0, 5.0)):
+        assert _create_status(keep_direction=keep, keep_drift=drift) in ok, (keep, drift)
+
+
 # ---------------------------------------------------------------------------
 # element_strain_energy (core/fea.py:595-602)
 # ---------------------------------------------------------------------------

@@ -56,6 +56,8 @@ class vx_options(ctypes.Structure):
         ("eager", ctypes.c_int32),
         ("block_steps", ctypes.c_int32),
         ("block_edge", ctypes.c_int32),
+        ("keep_direction", ctypes.c_int32),
+        ("keep_drift", ctypes.c_double),
     ]
 
 

@@ -143,7 +143,11 @@ class MGSolver:
     from a captured HIP graph (same kernels, same results).  ``block_steps``
     sets how many FP32 smoothing steps one launch of the LDS-blocked smoother
     does on the small levels below level 0 (0: 3, 1: one launch per step) and
-    ``block_edge`` its owned box edge in nodes (0: from the level size)."""
+    ``block_edge`` its owned box edge in nodes (0: from the level size).
+    ``keep_direction=True`` (mixed precision) lets each FP32 inner solve of the
+    iterative refinement continue the previous one's search direction instead
+    of restarting CG; ``keep_drift`` > 0 restarts it all the same where the
+    refreshed residual moved by more than that relative amount."""
 
     def __init__(self, nx, ny, nz, fixed_dofs, nu=0.3, xp=None, n_smooth=2, omega=0.6,
                  min_elems=4, max_levels=6, compute_mode="AUTO", cpu_threads=0,
@@ -151,7 +155,7 @@ class MGSolver:
                  max_outer=20, z_chunk=0, smoother="jacobi", cheb_ratio=0.3,
                  system="full", e_coarsen="mean", max_batch=1, flexible=False, pad=False,
                  autotune=False, coarse="sweeps", level_smooth=None, graphs=True,
-                 cheb_ratio_power=0, block_steps=0, block_edge=0):
+                 cheb_ratio_power=0, block_steps=0, block_edge=0, keep_direction=False, keep_drift=0.0):
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
         self.nelem = self.nx * self.ny * self.nz
         self.ndof = 3 * (self.nx + 1) * (self.ny + 1) * (self.nz + 1)
@@ -184,7 +188,8 @@ class MGSolver:
                          "direct": _native.VX_COARSE_DIRECT}[coarse],
             level_smooth=_level_smooth(level_smooth), eager=0 if graphs else 1,
             cheb_ratio_power=int(cheb_ratio_power), block_steps=int(block_steps),
-            block_edge=int(block_edge))
+            block_edge=int(block_edge), keep_direction=1 if keep_direction else 0,
+            keep_drift=float(keep_drift))
         self.max_batch = max_batch
         self._coarse_sweeps = int(coarse_sweeps)
         self._steps = [int(k) for k in (level_smooth or [])]

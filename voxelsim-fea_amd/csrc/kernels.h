@@ -42,6 +42,10 @@ struct Scalars {
   double tmp;
   double tmp2;
   double cs;       // corrector: sum of squares of its input residual
+  double rz_a;     // the r.z the last step's alpha used
+  double pscale;   // continued direction: (old over new) residual scale of an inner system
+  double drift;    // ||true - recurrence residual|| at a refresh, in units of the new one
+  double drift_max;  // continued direction only while drift <= drift_max (0: no limit)
   int frozen;      // 1: this right-hand side has stopped (alpha = 0)
   int pad;
 };
@@ -1562,7 +1566,11 @@ enum {
   F_BETA = 5,       // s.rz_new = sum; s.beta = rz == 0 ? 0 : sum / rz; s.rz = sum
   F_SET_CS = 6,     // s.cs = sum
   F_RZNEW = 7,      // s.rz_new = sum
-  F_BETA_FLEX = 8   // sum = z.Ap: s.beta = -sum / pAp (flexible CG); s.rz = s.rz_new
+  F_BETA_FLEX = 8,  // sum = z.Ap: s.beta = -sum / pAp (flexible CG); s.rz = s.rz_new
+  F_RZ0_KEEP = 9,   // s.rz = sum; alpha = 0, frozen = 0; beta continues the previous
+                    // direction: sum / (rz_a * pscale) (0 without one or when the
+                    // refresh moved the residual by more than drift_max)
+  F_DRIFT = 10      // s.drift = sqrt(sum)
 };
 
 constexpr int FIN_THREADS = 1024;
@@ -1595,6 +1603,15 @@ __device__ __forceinline__ void finalize_run(const double* __restrict__ partial,
       case F_ALPHA:
         s->pAp = v;
         s->alpha = (v == 0.0 || s->frozen) ? 0.0 : s->rz / v;
+        s->rz_a = s->rz;
+        break;
+      case F_RZ0_KEEP:
+        s->rz = v;
+        s->beta = (s->rz_a > 0.0 && s->pscale > 0.0 && (s->drift_max <= 0.0 || s->drift <= s->drift_max))
+                      ? v / (s->rz_a * s->pscale)
+                      : 0.0;
+        s->alpha = 0.0;
+        s->frozen = 0;
         break;
       case F_RATIO:
         s->rr = v;
@@ -1606,6 +1623,7 @@ __device__ __forceinline__ void finalize_run(const double* __restrict__ partial,
         s->rz = v;
         break;
       case F_SET_CS: s->cs = v; break;
+      case F_DRIFT: s->drift = sqrt(v); break;
       case F_RZNEW: s->rz_new = v; break;
       case F_BETA_FLEX:
         s->beta = (s->pAp == 0.0) ? 0.0 : -v / s->pAp;
@@ -1905,6 +1923,31 @@ __global__ void k_scale_to_f32(long long n, const Scalars* __restrict__ s,
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
     r32[i] = (float)(r64[i] * sc);
+}
+
+// k_scale_to_f32 for a refresh that may continue the previous inner solve's
+// direction: before r32 (the previous inner recurrence residual, in that
+// solve's units) is overwritten, the partial sums of ||r64 / ||r64|| -
+// pscale * r32||^2, the refresh's move in units of the new residual
+__global__ void k_scale_to_f32_drift(long long n, const Scalars* __restrict__ s,
+                                     const double* __restrict__ r64, float* __restrict__ r32,
+                                     double* __restrict__ partial) {
+  __shared__ double sred[4];
+  const double nr = sqrt(s[blockIdx.y].tmp);
+  const double sc = (nr > 0.0 && s[blockIdx.y].tmp2 == 0.0) ? 1.0 / nr : 0.0;
+  const double c = s[blockIdx.y].pscale;
+  r64 += blockIdx.y * n;
+  r32 += blockIdx.y * n;
+  double acc = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double v = r64[i] * sc;
+    const double d = v - c * (double)r32[i];
+    acc += d * d;
+    r32[i] = (float)v;
+  }
+  acc = block_sum(acc, sred);
+  if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = acc;
 }
 
 // u64 += d32 * ||r64|| (undo the inner scaling)

@@ -1147,7 +1147,8 @@ template <typename T>
 int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* Ap, T* t0, const std::vector<double>& tol_rel, int max_iter,
         bool stall_guard, const std::vector<double>& ratio_scale, double cb_tol,
         const std::vector<int>& iter_offset, vx_progress_cb cb, void* user, double* hist,
-        int hist_cap, std::vector<PcgResult>& res, const std::vector<int>& skip, int l0 = 0) {
+        int hist_cap, std::vector<PcgResult>& res, const std::vector<int>& skip, int l0 = 0,
+        bool keep_p = false) {
   Level& L0 = s->lv[l0];
   const bool flexible = (std::is_same<T, float>::value && s->corr_on) || s->opt.flexible;
   const int nb = s->nb;
@@ -1160,7 +1161,9 @@ int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* Ap, T* t0, const std::vector<do
   res.assign(nb, PcgResult());
   std::vector<int> frozen(skip.begin(), skip.end());
   int rc = vcycle<T>(s, r, z, t0, Sel<T>::w(L0), true, &np, l0);
-  if (rc == VX_OK) rc = finalize(s, np, F_RZ0);   // also beta = 0, frozen = 0
+  // also alpha = 0, frozen = 0; beta = 0, or with keep_p the previous
+  // solve's direction continued (p is kept, scaled through beta)
+  if (rc == VX_OK) rc = finalize(s, np, keep_p ? F_RZ0_KEEP : F_RZ0);
   if (rc == VX_OK && std::any_of(frozen.begin(), frozen.end(), [](int f) { return f; }))
     rc = push_frozen(s, frozen);
   if (rc != VX_OK) { cleanup(); return rc; }
@@ -1177,7 +1180,7 @@ int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* Ap, T* t0, const std::vector<do
   const bool lazy_u = fuse_x1;
   const double x1_coef = first_step_coef(s, level_ns(s, l0));
   // p starts at 0 so that the first p = z + 0 * p is exact
-  if (hipMemsetAsync(p, 0, sizeof(T) * n * nb, s->stream) != hipSuccess) {
+  if (!keep_p && hipMemsetAsync(p, 0, sizeof(T) * n * nb, s->stream) != hipSuccess) {
     cleanup();
     return fail(s, VX_ERR_HIP, "memset p");
   }
@@ -1563,7 +1566,8 @@ int check_options(const vx_options* opt, int nlev, vx_options* o) {
       (o->coarse_mode != VX_COARSE_SWEEPS && o->coarse_mode != VX_COARSE_DIRECT))
     return VX_ERR_ARG;
   if (o->block_steps < 0 || o->block_steps > BLK_MAX_STEPS || o->block_edge < 0 ||
-      o->block_edge == 1 || o->block_edge > 16)
+      o->block_edge == 1 || o->block_edge > 16 || o->keep_direction < 0 || o->keep_direction > 1 ||
+      !(o->keep_drift >= 0.0))
     return VX_ERR_ARG;
   if (o->smoother == VX_SMOOTH_CHEBYSHEV) {
     vx_solver probe;
@@ -1865,6 +1869,10 @@ static int solve_impl(vx_solver* s, int nb, const double* F, double* U, const do
       // the plain V-cycle takes over.
       std::vector<int> total(nb, 0), done(zero);
       std::vector<double> true_ratio(nb, 0.0);
+      // keep_direction: the previous inner solve's residual scale (true ratio)
+      // and whether it left a direction
+      std::vector<double> prev_scale(nb, 0.0);
+      bool have_dir = false;
       double prev_worst = INFINITY;
       bool learned_step = false;
       for (int outer = 0; outer <= s->opt.max_outer && rc == VX_OK; ++outer) {
@@ -1896,13 +1904,31 @@ static int solve_impl(vx_solver* s, int nb, const double* F, double* U, const do
         }
         prev_worst = worst;
         if (!open || outer == s->opt.max_outer || most >= max_iter) break;
-        // mark finished right-hand sides (tmp2) so their inner input is 0
-        for (int q = 0; q < nb; ++q) s->hscal[q].tmp2 = done[q] ? 1.0 : 0.0;
+        // mark finished right-hand sides (tmp2) so their inner input is 0.  A
+        // continued direction p (keep_direction, FP32 inner solves without the
+        // corrector) is in the previous inner system's units: scaled by
+        // old over new residual norm through beta (pscale; 0 starts afresh)
+        const bool keep = s->opt.keep_direction && have_dir && !s->corr_on;
+        for (int q = 0; q < nb; ++q) {
+          s->hscal[q].tmp2 = done[q] ? 1.0 : 0.0;
+          s->hscal[q].drift_max = s->opt.keep_drift;
+          s->hscal[q].pscale = (keep && !done[q] && prev_scale[q] > 0.0 && true_ratio[q] > 0.0)
+                                   ? prev_scale[q] / true_ratio[q]
+                                   : 0.0;
+        }
         HIPCHK(hipMemcpyAsync(s->dscal, s->hscal, sizeof(Scalars) * nb, hipMemcpyHostToDevice,
                               s->stream));
-        hipLaunchKernelGGL(k_scale_to_f32, dim3(EW_BLOCKS, nb), dim3(EW_THREADS), 0, s->stream, n,
-                           s->dscal, s->r64, s->r32);
-        LAUNCHCHK();
+        if (keep) {
+          hipLaunchKernelGGL(k_scale_to_f32_drift, dim3(EW_BLOCKS, nb), dim3(EW_THREADS), 0,
+                             s->stream, n, s->dscal, s->r64, s->r32, s->partial);
+          LAUNCHCHK();
+          rc = finalize(s, EW_BLOCKS, F_DRIFT);
+          if (rc != VX_OK) break;
+        } else {
+          hipLaunchKernelGGL(k_scale_to_f32, dim3(EW_BLOCKS, nb), dim3(EW_THREADS), 0, s->stream,
+                             n, s->dscal, s->r64, s->r32);
+          LAUNCHCHK();
+        }
         HIPCHK(hipMemsetAsync(s->d32, 0, sizeof(float) * n * nb, s->stream));
         // each inner right-hand side has unit norm
         hipLaunchKernelGGL(k_set_fnorm, dim3(nb), dim3(1), 0, s->stream, s->dscal, 1.0);
@@ -1916,7 +1942,8 @@ static int solve_impl(vx_solver* s, int nb, const double* F, double* U, const do
         int cap = max_iter - most;
         if (learned_step && s->corr_budget > 0) cap = std::min(cap, s->corr_budget);
         rc = pcg<float>(s, s->d32, s->r32, s->z32, s->p32, s->Ap32, L0.t32, want, cap,
-                        false, true_ratio, tol, total, cb, user, hist, hist_cap, pr, done);
+                        false, true_ratio, tol, total, cb, user, hist, hist_cap, pr, done, 0,
+                        keep);
         if (rc != VX_OK) break;
         if (learned_step && s->corr_budget > 0) {
           // over budget without reaching the inner tolerance: keep the (energy-
@@ -1940,8 +1967,19 @@ static int solve_impl(vx_solver* s, int nb, const double* F, double* U, const do
           corr_state = 2;
           learned_step = false;
           prev_worst = INFINITY;
+          have_dir = false;
           continue;
         }
+        // a direction is continued only from a plain inner solve, for the
+        // right-hand sides that iterated to its end (one frozen earlier keeps
+        // updating p with alpha = 0, which no longer is a CG direction)
+        int last = 0;
+        for (int q = 0; q < nb; ++q)
+          if (!done[q]) last = std::max(last, pr[q].iters);
+        for (int q = 0; q < nb; ++q)
+          prev_scale[q] = (!done[q] && pr[q].iters > 0 && pr[q].iters == last) ? true_ratio[q]
+                                                                                : 0.0;
+        have_dir = !learned_step;
         bool progressed = false;
         for (int q = 0; q < nb; ++q) {
           if (done[q]) continue;
