@@ -100,7 +100,8 @@ typedef struct {
                                  iterative refinement continues the previous one's search
                                  direction (rescaled to the new residual) instead of restarting
                                  CG, a "reliable update" of the recurrence residual by the FP64
-                                 true residual; 0 (default) = restart */
+                                 true residual; 0 (default) = restart.  vx_solve and
+                                 vx_solve_batch; vx_solve_parts always restarts */
   double keep_drift;          /* keep_direction: a refresh that moves the residual by more than
                                  this (relative to the new residual's norm) restarts the inner
                                  solve all the same; 0: no limit */
