@@ -1835,19 +1835,27 @@ __device__ __forceinline__ void p_update_u_run(long long n, T alpha, T beta,
 }
 
 template <typename T>
-__global__ void k_p_update_u(long long n, const Scalars* __restrict__ s, const T* __restrict__ z,
-                             T* __restrict__ p, T* __restrict__ u) {
-  const long long off = (long long)blockIdx.y * n;
-  p_update_u_run<T>(n, (T)s[blockIdx.y].alpha, (T)s[blockIdx.y].beta, z + off, p + off, u + off);
-}
-
-template <typename T>
 __device__ __forceinline__ void u_flush_run(long long n, T alpha, const T* __restrict__ p,
                                             T* __restrict__ u) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
     u[i] = u[i] + alpha * p[i];
 }
+
+// a frozen right-hand side takes only its pending u += alpha p, and nothing
+// once its alpha is 0 (its p is no longer used)
+template <typename T>
+__global__ void k_p_update_u(long long n, const Scalars* __restrict__ s, const T* __restrict__ z,
+                             T* __restrict__ p, T* __restrict__ u) {
+  const Scalars& q = s[blockIdx.y];
+  if (q.frozen && q.alpha == 0.0) return;
+  const long long off = (long long)blockIdx.y * n;
+  if (q.frozen)
+    u_flush_run<T>(n, (T)q.alpha, p + off, u + off);
+  else
+    p_update_u_run<T>(n, (T)q.alpha, (T)q.beta, z + off, p + off, u + off);
+}
+
 
 // the last deferred update: u += alpha p per right-hand side (blockIdx.y)
 template <typename T>
@@ -2629,12 +2637,14 @@ __global__ void k_cg_update_x1_parts(const PartGrid* __restrict__ pg,
 }
 
 // u += alpha p (the deferred update), then p = z + beta p; a frozen part
-// takes only its pending update u += alpha p (its later alphas are 0)
+// takes only its pending update u += alpha p, and nothing once its alpha is 0
+// (every step after the one that froze it)
 template <typename T>
 __global__ void k_p_update_u_parts(const PartGrid* __restrict__ pg,
                                    const Scalars* __restrict__ s, const T* __restrict__ z,
                                    T* __restrict__ p, T* __restrict__ u) {
   const int q = blockIdx.y;
+  if (s[q].frozen && s[q].alpha == 0.0) return;
   const PartGrid g = pg[q];
   const long long n = 3 * part_nodes(g);
   if (s[q].frozen)

@@ -498,18 +498,32 @@ int parts_build(vx_parts* P) {
     V.hr.resize(V.m + 1);
     V.hp.resize(V.m + 1);
     int br = 0, bp = 0;
+    // the level's work-group targets are shared among its parts by size (a
+    // part of its own would get the whole target): one launch then dispatches
+    // about as many work-groups as a single grid of the same total size
+    double cn = 0.0, fn = 0.0;
+    for (int q = 0; q < V.m; ++q) {
+      const PartGrid& f = V.hg[q];
+      const PartGrid& c = C.hg[q];
+      cn += (double)(c.nx + 1) * (c.ny + 1) * (c.nz + 1);
+      fn += (double)(f.nx + 1) * (f.ny + 1) * (f.nz + 1);
+    }
     for (int q = 0; q < V.m; ++q) {
       const PartGrid& f = V.hg[q];
       const PartGrid& c = C.hg[q];
       PartTr& r = V.hr[q];
       r.Lcx = c.nx + 1; r.Lcy = c.ny + 1; r.Lcz = c.nz + 1;
-      r.g = tr_grid((long long)r.Lcx * r.Lcy, r.Lcz, false);
+      const double cshare = (double)r.Lcx * r.Lcy * r.Lcz / cn;
+      r.g = tr_grid((long long)r.Lcx * r.Lcy, r.Lcz, false,
+                    std::max(1, (int)(TR_TARGET_GROUPS * cshare + 0.5)));
       r.begin = br;
       r.src = f.voff; r.dst = c.voff; r.node = c.noff;
       br += r.g.gxy * r.g.nchunk;
       PartTr& p = V.hp[q];
       p.Lcx = r.Lcx; p.Lcy = r.Lcy; p.Lcz = r.Lcz;
-      p.g = tr_grid(3LL * (f.nx + 1) * (f.ny + 1), f.nz + 1, true, PROLONG_TARGET_GROUPS);
+      const double fshare = (double)(f.nx + 1) * (f.ny + 1) * (f.nz + 1) / fn;
+      p.g = tr_grid(3LL * (f.nx + 1) * (f.ny + 1), f.nz + 1, true,
+                    std::max(1, (int)(PROLONG_TARGET_GROUPS * fshare + 0.5)));
       p.begin = bp;
       p.src = c.voff; p.dst = f.voff; p.node = f.noff;
       bp += p.g.gxy * p.g.nchunk;
