@@ -218,6 +218,15 @@ int vx_parts_set_fixed(vx_parts_handle h, const int64_t* fixed_dofs_host,
  * the active-DOF system) */
 int vx_parts_set_density(vx_parts_handle h, const double* E_dev, const uint8_t* active_dev,
                          double e_min);
+/* Tile z-chunks of the parts, level by level (the vx_autotune of a parts
+ * handle; no counterpart in the reference, which solves parts one by one):
+ * each candidate, applied to all the parts of a level, is timed on reps
+ * smoothing steps over them and the fastest kept.  Needs set_fixed and
+ * set_density; drops captured graphs; a no-op when options.z_chunk > 0. */
+int vx_parts_autotune(vx_parts_handle h, int32_t reps);
+/* the z-chunk of one part's tile kernels at one of its levels (> 0), or
+ * VX_ERR_ARG */
+int vx_parts_zchunk(vx_parts_handle h, int32_t part, int32_t level);
 /* Every part solved at once, each to its own tol with the recurrence of its
  * own vx_solve (FP64) or iterative refinement (VX_MIXED).  hist_host (may be
  * NULL) is [nparts][hist_cap]; infos (may be NULL) one vx_info per part.

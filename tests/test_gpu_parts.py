@@ -169,3 +169,37 @@ def test_parts_graph_replay_equals_eager_launches(precision):
     for (ug, ig), (ue, ie) in zip(out[True], out[False]):
         assert ig == ie
         assert all(np.array_equal(a, b) for a, b in zip(ug, ue))
+
+
+def test_parts_autotune_keeps_the_solution():
+    """vx_parts_autotune picks each level's tile z-chunk by timing.  The
+    z-chunk only regroups the FP64 dot-product partials, so tuned solves agree
+    with fixed-z-chunk ones to rounding, in FP64 and mixed precision."""
+    import torch
+    from core.multigrid import PartsSolver
+    print("case parts autotune", flush=True)
+    parts = [_part(s, seed, z) for s, seed, z in PARTS]
+    kw = dict(smoother="chebyshev", cheb_ratio=0.05, system="active", max_levels=5, pad=True,
+              compute_mode="GPU", nu=0.3)
+    F = [torch.tensor(p["f"], device="cuda") for p in parts]
+    for precision, bar in (("fp64", 1e-9), ("mixed", 1e-6)):
+        out = {}
+        for name, extra in (("tuned", dict(autotune=True)), ("fixed", dict(z_chunk=4))):
+            ps = PartsSolver([p["shape"] for p in parts], [p["fixed"] for p in parts],
+                             precision=precision, **kw, **extra)
+            ps.set_density([p["E"] for p in parts], [p["active"] for p in parts])
+            assert ps.tuned == (name == "tuned")
+            zc = ps.zchunks()
+            assert all(len(zc[i]) == n and min(zc[i]) >= 1 for i, n in enumerate(ps.levels))
+            print(f"  {precision} {name} z-chunks {zc}", flush=True)
+            U, _ = ps.solve_device(F, tol=1e-8)
+            out[name] = ([u.cpu().numpy() for u in U], [i["iters"] for i in ps.last_parts],
+                         [i["converged"] for i in ps.last_parts])
+            ps.close()
+        (ut, it_t, ct), (uf, it_f, cf) = out["tuned"], out["fixed"]
+        print(f"  {precision} iterations tuned {it_t} fixed {it_f}", flush=True)
+        assert all(ct) and all(cf)
+        assert all(abs(a - b) <= 1 for a, b in zip(it_t, it_f))
+        for a, b in zip(ut, uf):
+            scale = np.abs(b).max()
+            assert scale == 0 or np.abs(a - b).max() <= bar * scale

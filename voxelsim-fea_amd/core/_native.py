@@ -111,6 +111,8 @@ SIGNATURES = {
     "vx_parts_num_levels": (ctypes.c_int, [_P, _I32]),
     "vx_parts_set_fixed": (ctypes.c_int, [_P, _P, _P]),
     "vx_parts_set_density": (ctypes.c_int, [_P, _P, _P, _D]),
+    "vx_parts_autotune": (ctypes.c_int, [_P, _I32]),
+    "vx_parts_zchunk": (ctypes.c_int, [_P, _I32, _I32]),
     "vx_solve_parts": (ctypes.c_int, [_P, _P, _P, _D, _I32, PROGRESS_CB, _P, _P, _I32,
                                       ctypes.POINTER(vx_info)]),
     "vx_matvec": (ctypes.c_int, [_P, _I32, _I32, _P, _P]),

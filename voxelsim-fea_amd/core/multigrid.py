@@ -577,8 +577,9 @@ class PartsSolver:
     DOF ids.  Options as ``MGSolver`` (``pad=True`` grows each part with void
     elements to coarsenable dims, which needs ``system="active"``; at most
     ``pad_overhead`` times a part's volume).  The tile kernels' z-chunks follow
-    one rule over the whole group (``z_chunk`` fixes them); ``autotune`` is
-    accepted for ``MGSolver``'s option sets and has no effect here."""
+    one rule over the whole group (``z_chunk`` fixes them); ``autotune=True``
+    times candidate z-chunks level by level after the first ``set_density``
+    (``zchunks()`` reports them)."""
 
     def __init__(self, parts, fixed, nu=0.3, n_smooth=2, omega=0.6, min_elems=4, max_levels=6,
                  compute_mode="AUTO", cpu_threads=0, verbose=False, precision="fp64",
@@ -622,6 +623,9 @@ class PartsSolver:
                                                  counts.ctypes.data_as(ctypes.c_void_p)))
         self.last_parts = []
         self.last_solve_ms = float("nan")
+        self._autotune = bool(autotune)
+        self.tuned = False
+        self.autotune_s = 0.0
 
     def _stream_bind(self):
         torch = _torch()
@@ -661,6 +665,23 @@ class PartsSolver:
         em = -1.0 if self.system == "active" else float(e_min)
         self._check(self._lib.vx_parts_set_density(self._h, _native.ptr(self._E),
                                                    _native.ptr(self._A), em))
+        if self._autotune and not self.tuned:
+            import time
+            t0 = time.perf_counter()
+            self.autotune()
+            self.autotune_s = time.perf_counter() - t0
+
+    def autotune(self, reps=5):
+        """Time the tile kernels' z-chunk candidates level by level (needs
+        set_density) and keep the fastest; drops captured graphs."""
+        self._stream_bind()
+        self._check(self._lib.vx_parts_autotune(self._h, int(reps)))
+        self.tuned = True
+
+    def zchunks(self):
+        """{part: [z-chunk per level]} of the tile kernels."""
+        return {i: [int(self._lib.vx_parts_zchunk(self._h, i, l)) for l in range(n)]
+                for i, n in enumerate(self.levels)}
 
     def solve_device(self, F_list, tol=1e-4, max_cg=None, progress_cb=None, hist_cap=2000):
         """Solve every part; F_list: device float64 vectors, one per part.

@@ -397,6 +397,25 @@ void part_zchunks(const vx_options& o, const std::vector<std::array<int, 3>>& d,
   }
 }
 
+// a level's tile table (host) for per-part z-chunks zc; returns the total
+// work-group count
+int part_tiles(PLevel& V, const std::vector<int>& zc) {
+  V.ht.resize(V.np + 1);
+  int begin = 0;
+  for (int q = 0; q < V.np; ++q) {
+    const PartGrid& g = V.hg[q];
+    PartTiles& t = V.ht[q];
+    t.zc = zc[q];
+    t.gx = (g.nx + 1 + OWN - 1) / OWN;
+    t.gy = (g.ny + 1 + OWN - 1) / OWN;
+    t.gz = (g.nz + 1 + zc[q] - 1) / zc[q];
+    t.begin = begin;
+    begin += t.gx * t.gy * t.gz;
+  }
+  V.ht[V.np] = PartTiles{0, 0, 0, 0, begin};
+  return begin;
+}
+
 template <typename T>
 int upload(vx_solver* s, T** dst, const std::vector<T>& v) {
   CHK(dalloc(s, dst, v.size()));
@@ -477,19 +496,7 @@ int parts_build(vx_parts* P) {
     }
     std::vector<int> zc;
     part_zchunks(s->opt, d, zc);
-    V.ht.resize(V.np + 1);
-    int begin = 0;
-    for (int q = 0; q < V.np; ++q) {
-      PartTiles& t = V.ht[q];
-      t.zc = zc[q];
-      t.gx = (d[q][0] + 1 + OWN - 1) / OWN;
-      t.gy = (d[q][1] + 1 + OWN - 1) / OWN;
-      t.gz = (d[q][2] + 1 + zc[q] - 1) / zc[q];
-      t.begin = begin;
-      begin += t.gx * t.gy * t.gz;
-    }
-    V.ht[V.np] = PartTiles{0, 0, 0, 0, begin};
-    max_tiles = std::max<long long>(max_tiles, begin);
+    max_tiles = std::max<long long>(max_tiles, part_tiles(V, zc));
   }
   // grid transfers of the parts that go coarser
   for (int l = 0; l + 1 < L; ++l) {
@@ -751,6 +758,115 @@ int vx_parts_set_density(vx_parts_handle P, const double* E, const uint8_t* acti
     }
   P->density_set = true;
   return VX_OK;
+}
+
+// Tile z-chunks of the parts, level by level.  Each candidate z-chunk (applied
+// to every part of the level, at most the part's node planes; the rule's
+// choice competes too) is timed on a smoothing step over all the parts of
+// the level; the first within TUNE_TOL of the fastest wins (the rule's
+// choice first, then ascending), so near-ties resolve the same way each run.
+int vx_parts_autotune(vx_parts_handle P, int32_t reps) {
+  if (!P) return VX_ERR_ARG;
+  vx_solver* s = &P->s;
+  if (!P->fixed_set || !P->density_set)
+    return fail(s, VX_ERR_ARG, "vx_parts_autotune before vx_parts_set_fixed / set_density");
+  if (s->opt.z_chunk > 0) return VX_OK;   // fixed by the options
+  if (reps < 1) reps = 5;
+  drop_graphs(s);   // the launch shapes change
+  const bool f32 = s->opt.precision == VX_MIXED;
+  const int cands[] = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32};
+  // candidate z-chunk sets per level, and a partial buffer for the largest
+  // work-group count among them (the dot-product variants write one each)
+  std::vector<std::vector<std::vector<int>>> cs(P->lv.size());
+  long long need = s->n_partial;
+  for (size_t l = 0; l < P->lv.size(); ++l) {
+    PLevel& V = P->lv[l];
+    std::vector<int> rule(V.np);
+    int maxz = 0;
+    for (int q = 0; q < V.np; ++q) {
+      rule[q] = V.ht[q].zc;
+      maxz = std::max(maxz, V.hg[q].nz + 1);
+    }
+    cs[l].push_back(rule);
+    for (int c : cands) {
+      if (c >= maxz) break;
+      std::vector<int> z(V.np);
+      for (int q = 0; q < V.np; ++q) z[q] = std::min(c, V.hg[q].nz + 1);
+      if (std::find(cs[l].begin(), cs[l].end(), z) == cs[l].end()) cs[l].push_back(z);
+    }
+    std::vector<int> keep = rule;
+    for (const auto& z : cs[l]) need = std::max<long long>(need, part_tiles(V, z));
+    part_tiles(V, keep);
+  }
+  if (need > s->n_partial) {
+    s->partial = nullptr;   // the smaller buffer stays owned by s->allocs
+    CHK(dalloc(s, &s->partial, (size_t)need));
+    s->n_partial = (int)need;
+  }
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  int rc = VX_OK;
+  auto put = [&](PLevel& V) -> int {
+    HIPCHK(hipMemcpyAsync(V.dt, V.ht.data(), sizeof(PartTiles) * V.ht.size(),
+                          hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));   // V.ht is rewritten next
+    return VX_OK;
+  };
+  for (size_t l = 0; l < P->lv.size() && rc == VX_OK; ++l) {
+    PLevel& V = P->lv[l];
+    const PartsTarget t{P, (int)l, 0, V.np, nullptr};
+    auto launch = [&]() -> int {
+      SmoothCoef sc;
+      sc.omega = s->opt.omega;
+      sc.cu = 0.5;
+      sc.cr = 0.1;
+      if (f32) {
+        sc.up = V.w32;
+        return ttile<float, M_CHEB, false>(t, V.x32, l ? V.b32 : P->r32, V.t32, nullptr, &sc);
+      }
+      sc.up = V.w64;
+      return ttile<double, M_CHEB, false>(t, V.x64, l ? V.b64 : P->r64, V.t64, nullptr, &sc);
+    };
+    std::vector<std::pair<size_t, float>> times;
+    float best = INFINITY;
+    for (size_t k = 0; k < cs[l].size() && rc == VX_OK; ++k) {
+      part_tiles(V, cs[l][k]);
+      rc = put(V);
+      if (rc == VX_OK) rc = launch();   // warm-up
+      if (rc != VX_OK) break;
+      if (hipEventRecord(a, s->stream) != hipSuccess) { rc = fail(s, VX_ERR_HIP, "event"); break; }
+      for (int r = 0; r < reps && rc == VX_OK; ++r) rc = launch();
+      if (rc != VX_OK) break;
+      float ms = 0.f;
+      if (hipEventRecord(b, s->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+          hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+        rc = fail(s, VX_ERR_HIP, "event timing");
+        break;
+      }
+      times.push_back({k, ms});
+      best = std::min(best, ms);
+    }
+    size_t pick = 0;
+    for (const auto& kt : times)
+      if (kt.second <= best * (1.0f + TUNE_TOL)) {
+        pick = kt.first;
+        break;
+      }
+    part_tiles(V, cs[l][pick]);
+    if (rc == VX_OK) rc = put(V);
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return rc;
+}
+
+// tuned (or rule) z-chunk of one part's tile kernels at one level
+int vx_parts_zchunk(vx_parts_handle P, int32_t part, int32_t level) {
+  if (!P || part < 0 || part >= P->np || level < 0 || level >= P->nlev[part]) return VX_ERR_ARG;
+  int q = 0;
+  while (P->order[q] != part) ++q;
+  return P->lv[level].ht[q].zc;
 }
 
 // Every part solved at once: F / U are the parts' full-grid float64 vectors
