@@ -853,8 +853,11 @@ int vx_parts_autotune(vx_parts_handle P, int32_t reps) {
         pick = kt.first;
         break;
       }
+    // the launch sizes come from the host table, the kernels' part lookup
+    // from the device one: leave them equal, also after a failure
     part_tiles(V, cs[l][pick]);
-    if (rc == VX_OK) rc = put(V);
+    const int rp = put(V);
+    if (rc == VX_OK) rc = rp;
   }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
