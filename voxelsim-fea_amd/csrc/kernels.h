@@ -164,11 +164,19 @@ __device__ inline double block_sum(double v, double* sred) {
 // The tile work of one work-group: tile vt of the grid described by a, on the
 // vectors au / ab / aup / aout of one right-hand side (or part); its DOT
 // partial goes to a.partial[slot].
+// A staged node plane lives in LDS in its global order (row-major, the three
+// components of a node together), so a thread's staging slot is its linear
+// index and needs no table.  Four plane buffers rotate: layer j reads its
+// bottom plane (buffer j % 4) and top plane, and stages the plane after the
+// top into buffer (j + 2) % 4; the bottom plane is overwritten only by layer
+// j + 2, after the barrier that ends layer j + 1, so the epilogue of layer j
+// reads the owner's own node value there instead of carrying it in registers.
+constexpr int NBUF = 4;
 template <typename T, int MODE, bool DOT>
 __device__ __forceinline__ void tile_run(const TileArgs<T>& a, int vt, int slot,
                                          const T* __restrict__ au, const T* __restrict__ ab,
                                          const T* __restrict__ aup, T* __restrict__ aout) {
-  __shared__ T sP[2][3 * FOOT * FOOT];
+  __shared__ T sP[NBUF][FOOTVALS];
   __shared__ T sQ[2][4][TILE][TILE + 1];
   __shared__ double sred[NT / 64];
   const int bx = vt % a.gx;
@@ -191,20 +199,19 @@ __device__ __forceinline__ void tile_run(const TileArgs<T>& a, int vt, int slot,
   const int plane3 = 3 * plane;
 
   // staging descriptors: byte offset inside a plane (BUF_OOB: outside the
-  // grid) and LDS slot of the (up to) PRE values this thread moves per plane
-  int goff[PRE], loff[PRE];
+  // grid) of the (up to) PRE values this thread moves per plane; value i of
+  // the footprint goes to LDS slot i
+  int goff[PRE];
 #pragma unroll
   for (int k = 0; k < PRE; ++k) {
     const int i = tid + k * NT;
     goff[k] = BUF_OOB;
-    loff[k] = -1;
     if (i < FOOTVALS) {
       const int row = i / (FOOT * 3);
       const int col = i - row * (FOOT * 3);
       const int xn = col / 3;
       const int c = col - xn * 3;
       const int x = X0 - 1 + xn, y = Y0 - 1 + row;
-      loff[k] = (c * FOOT + row) * FOOT + xn;
       if (x >= 0 && x < a.Lx && y >= 0 && y < a.Ly)
         goff[k] = ((y * a.Lx + x) * 3 + c) * (int)sizeof(T);
     }
@@ -226,18 +233,19 @@ __device__ __forceinline__ void tile_run(const TileArgs<T>& a, int vt, int slot,
   auto store = [&](int buf) {
 #pragma unroll
     for (int k = 0; k < PRE; ++k)
-      if (loff[k] >= 0) sP[buf][loff[k]] = pre[k];
+      if (k * NT + NT <= FOOTVALS || tid + k * NT < FOOTVALS) sP[buf][tid + k * NT] = pre[k];
   };
+  // this thread's element column: node (ty, tx) of the footprint, component c
+  const int cbase = (ty * FOOT + tx) * 3;
   auto corners = [&](int buf, int c, T& u00, T& u10, T& u01, T& u11) {
-    const T* P = &sP[buf][(c * FOOT + ty) * FOOT + tx];
+    const T* P = &sP[buf][cbase + c];
     u00 = P[0];
-    u10 = P[1];
-    u01 = P[FOOT];
-    u11 = P[FOOT + 1];
+    u10 = P[3];
+    u01 = P[3 * FOOT];
+    u11 = P[3 * FOOT + 3];
   };
 
   T Pb[3][4], acc[3][4];
-  T ub[3];
   T Eprev = T(0);
   double dacc = 0.0;
 
@@ -251,7 +259,6 @@ __device__ __forceinline__ void tile_run(const TileArgs<T>& a, int vt, int slot,
     T u00, u10, u01, u11;
     corners(0, c, u00, u10, u01, u11);
     had2(u00, u10, u01, u11, Pb[c]);
-    ub[c] = u00;
 #pragma unroll
     for (int m = 0; m < 4; ++m) acc[c][m] = T(0);
   }
@@ -294,14 +301,14 @@ __device__ __forceinline__ void tile_run(const TileArgs<T>& a, int vt, int slot,
   };
   T E_n = loadE(Z0 - 1);
   loadEpi(Z0);
-  // one element layer: the bottom-plane modes / force halves / u come in
-  // (Pb, acc, ub), the top plane's go out (Pt, Yt, ut).  The layer loop is
-  // unrolled by two with the two state sets swapped, so that no register
-  // copies carry the state from one layer to the next.
-  auto layer = [&](int j, const T (&Pb)[3][4], const T (&acc)[3][4], const T (&ub)[3],
-                   T (&Pt)[3][4], T (&Yt)[3][4], T (&ut)[3]) {
+  // one element layer: the bottom-plane modes and force halves come in
+  // (Pb, acc), the top plane's go out (Pt, Yt).  The layer loop is unrolled
+  // by two with the two state sets swapped, so that no register copies carry
+  // the state from one layer to the next.
+  auto layer = [&](int j, const T (&Pb)[3][4], const T (&acc)[3][4], T (&Pt)[3][4],
+                   T (&Yt)[3][4]) {
     const int iz = Z0 - 1 + j;
-    const int top = (j + 1) & 1;
+    const int bot = j & (NBUF - 1), top = (j + 1) & (NBUF - 1);
     const T Ee = E_n;
     E_n = loadE(iz + 1 < Z1 ? iz + 1 : -1);   // the last layer is Z1 - 1
 #pragma unroll
@@ -309,7 +316,6 @@ __device__ __forceinline__ void tile_run(const TileArgs<T>& a, int vt, int slot,
       T u00, u10, u01, u11;
       corners(top, c, u00, u10, u01, u11);
       had2(u00, u10, u01, u11, Pt[c]);
-      ut[c] = u00;
     }
     T Yb[3][4];
     elem_modes(Pb, Pt, Ee, a.K, Yb, Yt);
@@ -330,7 +336,7 @@ __device__ __forceinline__ void tile_run(const TileArgs<T>& a, int vt, int slot,
     const T EX = Es + dpp_shr1(Es);
     sQ[q][3][ty][tx] = EX;
     // stage the plane after the current top; prefetch the next one
-    store(j & 1);
+    store((j + 2) & (NBUF - 1));
     fetch(iz + 3);
     // this plane's epilogue operands (loaded one step ago)
     const unsigned mk = mk_n;
@@ -346,9 +352,12 @@ __device__ __forceinline__ void tile_run(const TileArgs<T>& a, int vt, int slot,
     if (iz >= Z0 && owner) {
       // output plane base (uniform, 64-bit) + this node's 32-bit offset
       T* const op = aout + (long long)iz * plane3 + 3 * n0;
-      T y[3];
+      T y[3], ub[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) y[c] = A0[c] + sQ[q][c][ty - 1][tx];
+      for (int c = 0; c < 3; ++c) {
+        y[c] = A0[c] + sQ[q][c][ty - 1][tx];
+        ub[c] = sP[bot][cbase + c];   // u at this node (the bottom plane's corner 00)
+      }
       if (MODE == M_MATVEC) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -384,13 +393,13 @@ __device__ __forceinline__ void tile_run(const TileArgs<T>& a, int vt, int slot,
       }
     }
   };
-  T Pb2[3][4], acc2[3][4], ub2[3];
+  T Pb2[3][4], acc2[3][4];
   int j = 0;
   for (; j + 1 < J; j += 2) {
-    layer(j, Pb, acc, ub, Pb2, acc2, ub2);
-    layer(j + 1, Pb2, acc2, ub2, Pb, acc, ub);
+    layer(j, Pb, acc, Pb2, acc2);
+    layer(j + 1, Pb2, acc2, Pb, acc);
   }
-  if (j < J) layer(j, Pb, acc, ub, Pb2, acc2, ub2);
+  if (j < J) layer(j, Pb, acc, Pb2, acc2);
 
   if (DOT) {
     const double s = block_sum(dacc, sred);
