@@ -91,6 +91,22 @@ __device__ __forceinline__ float buf_ld(__amdgpu_buffer_rsrc_t r, int off, float
 __device__ __forceinline__ double buf_ld(__amdgpu_buffer_rsrc_t r, int off, double) {
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
 }
+// Three consecutive values (one node's components) in one 96-bit global
+// load; the backend splits a 96-bit raw buffer load into dwords.
+struct V3f {
+  float x, y, z;
+};
+__device__ __forceinline__ void ld3(const float* p, float (&v)[3]) {
+  const V3f w = *reinterpret_cast<const V3f*>(p);
+  v[0] = w.x;
+  v[1] = w.y;
+  v[2] = w.z;
+}
+__device__ __forceinline__ void ld3(const double* p, double (&v)[3]) {
+  v[0] = p[0];
+  v[1] = p[1];
+  v[2] = p[2];
+}
 __device__ __forceinline__ unsigned buf_ld_u8(__amdgpu_buffer_rsrc_t r, int off) {
   return (unsigned)__builtin_amdgcn_raw_buffer_load_b8(r, off, 0, 0);
 }
@@ -278,9 +294,12 @@ __device__ __forceinline__ void tile_run(const TileArgs<T>& a, int vt, int slot,
     const auto r = buf_rsrc(a.E + (long long)(zin ? iz : 0) * nxy, zin ? nxy * (int)sizeof(T) : 0);
     return buf_ld(r, eoff, T(0));
   };
-  // epilogue operands of the owners (node n0 of plane pz; 0 for others)
+  // epilogue operands of the owners (node n0 of plane pz).  b and up are
+  // read as one 96-bit load per node; a non-owner, or a plane outside the
+  // chunk, reads node 0 of plane Z0 (a valid address) and never uses it:
+  // only the next layer's owners of planes Z0 .. Z1-1 do.
   const int moff = owner ? n0 : BUF_OOB;
-  const int voffb = owner ? 3 * n0 * (int)sizeof(T) : BUF_OOB;
+  const int vnode = 3 * (owner ? n0 : 0);
   unsigned mk_n = 0;
   T b_n[3] = {T(0), T(0), T(0)}, up_n[3] = {T(0), T(0), T(0)};
   auto loadEpi = [&](int pz) {
@@ -288,16 +307,9 @@ __device__ __forceinline__ void tile_run(const TileArgs<T>& a, int vt, int slot,
     const int pzc = zin ? pz : 0;
     const auto rm = buf_rsrc(a.mask + (long long)pzc * plane, zin ? plane : 0);
     mk_n = buf_ld_u8(rm, moff);
-    if (NEEDB) {
-      const auto rb = plane_rsrc(ab, zin ? pz : -1);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) b_n[c] = buf_ld(rb, voffb + c * (int)sizeof(T), T(0));
-    }
-    if (NEEDUP && has_up) {
-      const auto ru = plane_rsrc(aup, zin ? pz : -1);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) up_n[c] = buf_ld(ru, voffb + c * (int)sizeof(T), T(0));
-    }
+    const long long voff = (long long)(zin ? pz : Z0) * plane3 + vnode;
+    if (NEEDB) ld3(ab + voff, b_n);
+    if (NEEDUP && has_up) ld3(aup + voff, up_n);
   };
   T E_n = loadE(Z0 - 1);
   loadEpi(Z0);
