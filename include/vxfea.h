@@ -297,8 +297,9 @@ int vx_drop_islands_noalloc(vx_handle h, const uint8_t* active_dev,
 
 /* Timing helper: average duration (ms) of `reps` back-to-back launches of the
  * level-l operator kernel (kind 0 = masked matvec, 1 = damped-Jacobi sweep,
- * 2 = Chebyshev smoothing step) in the given precision, measured with HIP
- * events on the handle's stream. */
+ * 2 = Chebyshev smoothing step, 3 = the first two Chebyshev steps from zero in
+ * one launch, FP32 only) in the given precision, measured with HIP events on
+ * the handle's stream. */
 int vx_time_kernel(vx_handle h, int32_t kind, int32_t level, int32_t precision,
                    int32_t reps, double* avg_ms);
 
@@ -430,7 +431,9 @@ const char* vx_inside_last_error(void);
  * opt.z_chunk > 0 overrides the z-chunks, opt.block_steps / opt.block_edge
  * > 0 the blocked form. */
 int vx_autotune(vx_handle h, int32_t precision, int32_t reps);
-/* The z-chunk the handle uses for a level, precision and class. */
+/* The z-chunk the handle uses for a level, precision and class (0 the PCG
+ * operator, 1 a smoothing step, 2 the last smoothing step with its dot
+ * partials, 3 two Chebyshev steps from zero in one launch). */
 int vx_tuned_zchunk(vx_handle h, int32_t level, int32_t precision, int32_t cls);
 /* The blocked smoother's plan for an FP32 pass of `steps` steps on a level
  * (one right-hand side): owned box edge and steps per launch, or 0 and 0 when

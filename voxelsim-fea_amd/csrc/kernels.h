@@ -27,7 +27,7 @@ constexpr int NT = TILE * TILE;       // threads per tile work-group
 constexpr int FOOTVALS = FOOT * FOOT * 3;
 constexpr int PRE = (FOOTVALS + NT - 1) / NT;
 
-enum { M_MATVEC = 0, M_RESID = 1, M_JACOBI = 2, M_CHEB = 3 };
+enum { M_MATVEC = 0, M_RESID = 1, M_JACOBI = 2, M_CHEB = 3, M_CHEB_Z2 = 4 };
 
 // PCG scalars of one right-hand side, in device memory.
 struct Scalars {
@@ -69,6 +69,9 @@ struct TileArgs {
   T kd;                  // unit-E diagonal of KE
   ElemConst<T> K;
   double* __restrict__ partial;   // one double per work-group (DOT variants)
+  // M_CHEB_Z2: D^-1 per node and the weight of the first step from zero
+  const T* __restrict__ minv = nullptr;
+  T c1 = T(0);
 };
 
 __device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
@@ -171,6 +174,11 @@ __device__ inline double block_sum(double v, double* sred) {
 //                                                     (:367-402, :300-304)
 //       M_CHEB  : out = u + mask ? cu (u - up) + cr (b-Ku)/d : 0, one step of
 //                 the Chebyshev iteration on D^-1 A (three-term recurrence)
+//       M_CHEB_Z2: the first two steps of a Chebyshev pass from zero in one
+//                 launch.  u holds b; the first step x1 = mask c1 D^-1 b is
+//                 formed pointwise from each staged value (D^-1 from minv),
+//                 and the second, out = x1 + mask (cu x1 + cr (b - K x1)/d),
+//                 takes b at the node from the same staged plane
 // DOT   adds the work-group partial of  sum out*u (MATVEC) or sum out*b
 //       (the smoothers) to partial[block]; RESID: sum out*out.
 // ---------------------------------------------------------------------------
@@ -192,9 +200,15 @@ template <typename T, int MODE, bool DOT>
 __device__ __forceinline__ void tile_run(const TileArgs<T>& a, int vt, int slot,
                                          const T* __restrict__ au, const T* __restrict__ ab,
                                          const T* __restrict__ aup, T* __restrict__ aout) {
+  constexpr bool RUP = MODE == M_CHEB_Z2;
+  constexpr int FN = FOOT * FOOT;               // footprint nodes of a plane
+  constexpr int NPN = (FN + NT - 1) / NT;       // of them per thread
   __shared__ T sP[NBUF][FOOTVALS];
   __shared__ T sQ[2][4][TILE][TILE + 1];
   __shared__ double sred[NT / 64];
+  // M_CHEB_Z2: c1 D^-1 and the mask byte of each staged node
+  __shared__ T sW[RUP ? NBUF : 1][RUP ? FN : 1];
+  __shared__ uint8_t sM[RUP ? NBUF : 1][RUP ? FN : 1];
   const int bx = vt % a.gx;
   const int byz = vt / a.gx;
   const int by = byz % a.gy;
@@ -238,27 +252,85 @@ __device__ __forceinline__ void tile_run(const TileArgs<T>& a, int vt, int slot,
     const bool zin = pz >= 0 && pz < a.Lz;
     return buf_rsrc(v + (long long)(zin ? pz : 0) * plane3, zin ? plane3 * (int)sizeof(T) : 0);
   };
+  // M_CHEB_Z2: node nd = tid + k NT of the footprint, its offset in a plane
+  // (BUF_OOB: outside the grid or the footprint)
+  int noff[RUP ? NPN : 1];
+  if (RUP) {
+#pragma unroll
+    for (int k = 0; k < NPN; ++k) {
+      const int nd = tid + k * NT;
+      const int row = nd / FOOT, xn = nd - row * FOOT;
+      const int x = X0 - 1 + xn, y = Y0 - 1 + row;
+      noff[k] = nd < FN && x >= 0 && x < a.Lx && y >= 0 && y < a.Ly ? y * a.Lx + x : BUF_OOB;
+    }
+  }
   T pre[PRE];
+  // M_CHEB_Z2: D^-1 and the mask byte of each staged node, prefetched with
+  // the plane
+  T pw[RUP ? NPN : 1];
+  unsigned pm[RUP ? NPN : 1];
   // planes above Z1 (the last layer's top) are never used: a zero-size
   // resource makes those prefetches of the last two layers cost no traffic
   auto fetch = [&](int pz) {
-    const auto r = plane_rsrc(au, pz <= Z1 ? pz : -1);
+    const int pzu = pz <= Z1 ? pz : -1;
+    const auto r = plane_rsrc(au, pzu);
 #pragma unroll
     for (int k = 0; k < PRE; ++k) pre[k] = buf_ld(r, goff[k], T(0));
+    if (RUP) {
+      const bool zin = pzu >= 0 && pzu < a.Lz;
+      const long long pb = (long long)(zin ? pzu : 0) * plane;
+      const auto rw = buf_rsrc(a.minv + pb, zin ? plane * (int)sizeof(T) : 0);
+      const auto rm = buf_rsrc(a.mask + pb, zin ? plane : 0);
+#pragma unroll
+      for (int k = 0; k < NPN; ++k) {
+        pw[k] = buf_ld(rw, noff[k] == BUF_OOB ? BUF_OOB : noff[k] * (int)sizeof(T), T(0));
+        pm[k] = buf_ld_u8(rm, noff[k]);
+      }
+    }
   };
   auto store = [&](int buf) {
 #pragma unroll
     for (int k = 0; k < PRE; ++k)
-      if (k * NT + NT <= FOOTVALS || tid + k * NT < FOOTVALS) sP[buf][tid + k * NT] = pre[k];
+      if (k * NT + NT <= FOOTVALS || tid + k * NT < FOOTVALS)
+        sP[buf][tid + k * NT] = pre[k];
+    if (RUP) {
+#pragma unroll
+      for (int k = 0; k < NPN; ++k)
+        if (k * NT + NT <= FN || tid + k * NT < FN) {
+          sW[buf][tid + k * NT] = a.c1 * pw[k];
+          sM[buf][tid + k * NT] = (uint8_t)pm[k];
+        }
+    }
   };
   // this thread's element column: node (ty, tx) of the footprint, component c
-  const int cbase = (ty * FOOT + tx) * 3;
+  const int nbase = ty * FOOT + tx;
+  const int cbase = 3 * nbase;
+  // M_CHEB_Z2: the four corner nodes' c1 D^-1 and mask of a plane, read once
+  // per layer; the element sees the first step x1 = mask c1 D^-1 b of its corners
+  T cw[RUP ? 4 : 1];
+  unsigned cm[RUP ? 4 : 1];
+  auto corner_scales = [&](int buf) {
+    if (RUP) {
+      const int nd[4] = {nbase, nbase + 1, nbase + FOOT, nbase + FOOT + 1};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        cw[q] = sW[buf][nd[q]];
+        cm[q] = sM[buf][nd[q]];
+      }
+    }
+  };
   auto corners = [&](int buf, int c, T& u00, T& u10, T& u01, T& u11) {
     const T* P = &sP[buf][cbase + c];
     u00 = P[0];
     u10 = P[3];
     u01 = P[3 * FOOT];
     u11 = P[3 * FOOT + 3];
+    if (RUP) {
+      u00 = ((cm[0] >> c) & 1u) ? cw[0] * u00 : T(0);
+      u10 = ((cm[1] >> c) & 1u) ? cw[1] * u10 : T(0);
+      u01 = ((cm[2] >> c) & 1u) ? cw[2] * u01 : T(0);
+      u11 = ((cm[3] >> c) & 1u) ? cw[3] * u11 : T(0);
+    }
   };
 
   T Pb[3][4], acc[3][4];
@@ -270,6 +342,7 @@ __device__ __forceinline__ void tile_run(const TileArgs<T>& a, int vt, int slot,
   store(0);
   fetch(Z0);
   __syncthreads();
+  corner_scales(0);
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     T u00, u10, u01, u11;
@@ -285,7 +358,7 @@ __device__ __forceinline__ void tile_run(const TileArgs<T>& a, int vt, int slot,
   const int J = Z1 - Z0 + 1;   // layers Z0-1 .. Z1-1
   // software pipeline: E of the next layer and the owner's b / up / mask of
   // the next output plane are loaded one step ahead
-  constexpr bool NEEDB = MODE != M_MATVEC;
+  constexpr bool NEEDB = MODE != M_MATVEC && !RUP;   // M_CHEB_Z2: b from LDS
   constexpr bool NEEDUP = MODE == M_CHEB;
   const bool has_up = NEEDUP && aup != nullptr;
   const int eoff = col_in ? e0 * (int)sizeof(T) : BUF_OOB;
@@ -303,6 +376,7 @@ __device__ __forceinline__ void tile_run(const TileArgs<T>& a, int vt, int slot,
   unsigned mk_n = 0;
   T b_n[3] = {T(0), T(0), T(0)}, up_n[3] = {T(0), T(0), T(0)};
   auto loadEpi = [&](int pz) {
+    if (RUP) return;   // the node's mask, r' and x1 come from LDS
     const bool zin = pz >= Z0 && pz < Z1;
     const int pzc = zin ? pz : 0;
     const auto rm = buf_rsrc(a.mask + (long long)pzc * plane, zin ? plane : 0);
@@ -323,6 +397,7 @@ __device__ __forceinline__ void tile_run(const TileArgs<T>& a, int vt, int slot,
     const int bot = j & (NBUF - 1), top = (j + 1) & (NBUF - 1);
     const T Ee = E_n;
     E_n = loadE(iz + 1 < Z1 ? iz + 1 : -1);   // the last layer is Z1 - 1
+    corner_scales(top);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       T u00, u10, u01, u11;
@@ -365,22 +440,35 @@ __device__ __forceinline__ void tile_run(const TileArgs<T>& a, int vt, int slot,
       // output plane base (uniform, 64-bit) + this node's 32-bit offset
       T* const op = aout + (long long)iz * plane3 + 3 * n0;
       T y[3], ub[3];
+      unsigned mke = mk;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         y[c] = A0[c] + sQ[q][c][ty - 1][tx];
         ub[c] = sP[bot][cbase + c];   // u at this node (the bottom plane's corner 00)
       }
+      if (RUP) {
+        // b at the node is the staged value (sP); the node's own u is x1,
+        // the step before it 0
+        mke = sM[bot][nbase];
+        const T w = sW[bot][nbase];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          bb[c] = ub[c];
+          ub[c] = ((mke >> c) & 1u) ? w * bb[c] : T(0);
+          upv[c] = T(0);
+        }
+      }
       if (MODE == M_MATVEC) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          const T o = ((mk >> c) & 1u) ? y[c] : T(0);
+          const T o = ((mke >> c) & 1u) ? y[c] : T(0);
           op[c] = o;
           if (DOT) dacc = fmaT((double)o, (double)ub[c], dacc);
         }
       } else if (MODE == M_RESID) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          const T o = bb[c] - (((mk >> c) & 1u) ? y[c] : T(0));
+          const T o = bb[c] - (((mke >> c) & 1u) ? y[c] : T(0));
           op[c] = o;
           if (DOT) dacc = fmaT((double)o, (double)o, dacc);
         }
@@ -392,7 +480,7 @@ __device__ __forceinline__ void tile_run(const TileArgs<T>& a, int vt, int slot,
           // explicit fused multiply-adds and no other contractible mul+add:
           // the two unrolled copies of the layer body must round alike
           T o;
-          const bool fr = (mk >> c) & 1u;
+          const bool fr = (mke >> c) & 1u;
           if (MODE == M_JACOBI) {
             o = fr ? fmaT(a.omega, minv * (bb[c] - y[c]), ub[c]) : ub[c];
           } else {
@@ -1793,6 +1881,40 @@ __global__ void k_cg_update_x1(long long Nn, const Scalars* __restrict__ s, floa
   const long long off = (long long)blockIdx.y * 3 * Nn;
   double acc = cg_update_x1_run(Nn, (float)s[blockIdx.y].alpha, r + off, Ap + off, mask, minv, c,
                                 x1 + off);
+  acc = block_sum(acc, sred);
+  if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = acc;
+}
+
+// r -= alpha Ap and the r.r partials (FP32, 16-byte aligned rows): the PCG
+// update when the next V-cycle's level-0 pre-smoothing runs as M_CHEB_Z2
+// (u += alpha p is deferred, as with k_cg_update_x1)
+__global__ void k_cg_update_r(long long n, const Scalars* __restrict__ s, float* __restrict__ r,
+                              const float* __restrict__ Ap, double* __restrict__ partial) {
+  __shared__ double sred[4];
+  const long long off = (long long)blockIdx.y * n;
+  const float a = (float)s[blockIdx.y].alpha;
+  float* rr = r + off;
+  const float* aa = Ap + off;
+  double acc = 0.0;
+  const long long n4 = n >> 2;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n4;
+       i += (long long)gridDim.x * blockDim.x) {
+    float v[4], w[4];
+    ld4(rr + 4 * i, v);
+    ld4(aa + 4 * i, w);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      v[k] = v[k] - a * w[k];
+      acc += (double)v[k] * (double)v[k];
+    }
+    st4(rr + 4 * i, v);
+  }
+  if (blockIdx.x == 0 && threadIdx.x < (n & 3)) {
+    const long long i = 4 * n4 + threadIdx.x;
+    const float v = rr[i] - a * aa[i];
+    rr[i] = v;
+    acc += (double)v * (double)v;
+  }
   acc = block_sum(acc, sred);
   if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = acc;
 }

@@ -49,7 +49,7 @@ struct Level {
   // autotuned node planes per work-group: [precision 0 FP64 / 1 FP32][class
   // 0 operator (matvec forms), 1 smoother / residual forms, 2 smoother forms
   // with the dot-product partials]; 0 = rule
-  int zc_tuned[2][3] = {{0, 0, 0}, {0, 0, 0}};
+  int zc_tuned[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
 };
 
 template <typename T> struct Sel;
@@ -504,6 +504,55 @@ static double first_step_coef(const vx_solver* s, int k) {
   return s->opt.omega;
 }
 
+// coefficients (cu, cr) of step j >= 1 of a k-step Chebyshev pass, the
+// recurrence of tcheb_n
+static SmoothCoef cheb_step_coef(const vx_solver* s, int k, int j) {
+  const double beta = s->lam_max, alpha = cheb_ratio_for(s, k) * beta;
+  const double theta = 0.5 * (beta + alpha), delta = 0.5 * (beta - alpha);
+  const double sigma = theta / delta;
+  double rho_prev = 1.0 / sigma, rho = rho_prev;
+  SmoothCoef sc;
+  for (int i = 1; i <= j; ++i) {
+    rho = 1.0 / (2.0 * sigma - rho_prev);
+    sc.cu = rho * rho_prev;
+    sc.cr = 2.0 * rho / delta;
+    rho_prev = rho;
+  }
+  return sc;
+}
+
+// The first two steps of a Chebyshev pass from zero on level li in one launch
+// (M_CHEB_Z2): x1 = c1 D^-1 b formed pointwise while b is staged, then the
+// second step into x; the pass's own coefficients (the recurrence of tcheb_n)
+int launch_cheb_z2(vx_solver* s, int li, int k, const float* b, float* x) {
+  Level& L = s->lv[li];
+  const SmoothCoef sc = cheb_step_coef(s, k, 1);
+  TileArgs<float> a;
+  a.nx = L.nx; a.ny = L.ny; a.nz = L.nz;
+  a.Lx = L.Lx; a.Ly = L.Ly; a.Lz = L.Lz;
+  a.zc = zchunk(s, L, 1, 3);
+  a.nb = s->nb;
+  a.vstride = 3 * L.Nn;
+  a.u = b; a.E = L.E32; a.mask = L.mask; a.b = nullptr; a.out = x;
+  a.omega = 0.f;
+  a.up = nullptr;
+  a.cu = (float)sc.cu;
+  a.cr = (float)sc.cr;
+  a.kd = (float)s->kd;
+  a.K = kconst<float>(s);
+  a.partial = nullptr;
+  a.minv = L.mi32;
+  a.c1 = (float)first_step_coef(s, k);
+  dim3 g = tile_grid(s, L, a.zc);
+  a.gx = (int)g.x; a.gy = (int)g.y; a.gz = (int)g.z;
+  const long long G = (long long)g.x * g.y * g.z;
+  const long long launch = ((G * s->nb + 7) / 8) * 8;   // XCD-ordered 1D grid
+  hipLaunchKernelGGL((k_tile<float, M_CHEB_Z2, false>), dim3((unsigned)launch), dim3(NT), 0,
+                     s->stream, a);
+  LAUNCHCHK();
+  return VX_OK;
+}
+
 // smoothing step of the configured kind; result in dst (tmp, tmp2 scratch)
 template <typename T, typename Tgt>
 int tsmooth_step(const Tgt& tg, const T* u_in, const T* b, int k, T* dst, T* tmp, T* tmp2,
@@ -858,9 +907,12 @@ int coarse_direct(vx_solver* s, const T* b, T* x) {
   return VX_OK;
 }
 
+// first_done: 0 none, 1 the first level-l0 smoothing step from zero is in t0,
+// 2 the level-l0 pre-smoothing pass (two Chebyshev steps from zero, FP32) runs
+// as one M_CHEB_Z2 launch
 template <typename T>
 int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, T* w0, bool dot, int* n_part, int l0,
-           bool first_done = false);
+           int first_done = 0);
 
 // The plain V-cycle's coarse-grid correction of level li for residual t:
 // restrict, V-cycle from level li+1, prolong (v = 0 + P x); at the coarsest
@@ -960,7 +1012,7 @@ int capture_residual(vx_solver* s, int li, const T* t) {
 // the network's correction between its pre- and post-smoothing.
 template <typename T>
 int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, T* w0, bool dot, int* n_part, int l0,
-           bool first_done) {
+           int first_done) {
   const int nl = (int)s->lv.size();
   const int nb = s->nb;
   const bool learned = std::is_same<T, float>::value && s->corr_on && s->corr_level >= l0 &&
@@ -978,10 +1030,14 @@ int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, T* w0, bool dot, int* n_part
     T* x = li == l0 ? z0 : Sel<T>::x(L);
     T* t = li == l0 ? t0 : Sel<T>::t(L);
     T* w = li == l0 ? w0 : Sel<T>::w(L);
-    const bool pre_done = li == l0 ? first_done : restricted_x1;
+    const bool pre_all = std::is_same<T, float>::value && li == l0 && first_done == 2;
+    const bool pre_done = li == l0 ? first_done == 1 : restricted_x1;
     const int ns = level_ns(s, li);
     if (li == bottom && learned) {
-      CHK(smooth_step<T>(s, li, nullptr, b, ns, x, t, w, false, nullptr, pre_done));
+      if (pre_all)
+        CHK(launch_cheb_z2(s, li, ns, reinterpret_cast<const float*>(b), reinterpret_cast<float*>(x)));
+      else
+        CHK(smooth_step<T>(s, li, nullptr, b, ns, x, t, w, false, nullptr, pre_done));
       CHK((launch_tile<T, M_RESID, false>(s, li, x, b, t, nullptr, nullptr)));
       if (l0 == 0) CHK(capture_residual<T>(s, li, t));
       CHK(corrector_apply(s, li, reinterpret_cast<const float*>(t), reinterpret_cast<float*>(x)));
@@ -1004,7 +1060,10 @@ int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, T* w0, bool dot, int* n_part
       }
       break;
     }
-    CHK(smooth_step<T>(s, li, nullptr, b, ns, x, t, w, false, nullptr, pre_done));
+    if (pre_all)
+      CHK(launch_cheb_z2(s, li, ns, reinterpret_cast<const float*>(b), reinterpret_cast<float*>(x)));
+    else
+      CHK(smooth_step<T>(s, li, nullptr, b, ns, x, t, w, false, nullptr, pre_done));
     // residual into t, then restrict into b of level li+1
     CHK((launch_tile<T, M_RESID, false>(s, li, x, b, t, nullptr, nullptr)));
     if (l0 == 0) CHK(capture_residual<T>(s, li, t));
@@ -1179,6 +1238,11 @@ int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* Ap, T* t0, const std::vector<do
   // direction update (which streams p anyway) and a flush at the end
   const bool lazy_u = fuse_x1;
   const double x1_coef = first_step_coef(s, level_ns(s, l0));
+  // a two-step Chebyshev pre-smoothing pass on level l0 runs as one launch
+  // that forms its first step from r itself (M_CHEB_Z2): the update then
+  // writes only r (k_cg_update_r), not the first step
+  const bool fuse_z2 = fuse_x1 && s->opt.smoother == VX_SMOOTH_CHEBYSHEV &&
+                       level_ns(s, l0) == 2 && !(s->corr_on && s->corr_level == l0);
   // p starts at 0 so that the first p = z + 0 * p is exact
   if (!keep_p && hipMemsetAsync(p, 0, sizeof(T) * n * nb, s->stream) != hipSuccess) {
     cleanup();
@@ -1199,7 +1263,12 @@ int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* Ap, T* t0, const std::vector<do
     rc = launch_tile_L<T, M_MATVEC, true>(s, L0, p, nullptr, Ap, s->partial, &np);
     if (rc == VX_OK) rc = finalize(s, np, F_ALPHA);
     if (rc != VX_OK) break;
-    if (fuse_x1) {
+    if (fuse_z2) {
+      // r -= alpha Ap (u += alpha p waits for the next direction update or the flush)
+      hipLaunchKernelGGL(k_cg_update_r, dim3(EW_BLOCKS, nb), dim3(EW_THREADS), 0, s->stream, n,
+                         s->dscal, reinterpret_cast<float*>(r),
+                         reinterpret_cast<const float*>(Ap), s->partial);
+    } else if (fuse_x1) {
       // also the next V-cycle's first smoothing step from zero, into t0
       // (u += alpha p waits for the next direction update or the flush)
       hipLaunchKernelGGL(k_cg_update_x1, dim3(EW_BLOCKS, nb), dim3(EW_THREADS), 0, s->stream,
@@ -1229,7 +1298,7 @@ int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* Ap, T* t0, const std::vector<do
     // a captured graph where the cycle allows it
     auto cycle_and_beta = [&]() -> int {
       int np2 = 0;
-      CHK(vcycle<T>(s, r, z, t0, Sel<T>::w(L0), true, &np2, l0, fuse_x1));
+      CHK(vcycle<T>(s, r, z, t0, Sel<T>::w(L0), true, &np2, l0, fuse_z2 ? 2 : (fuse_x1 ? 1 : 0)));
       if (!flexible) return finalize(s, np2, F_BETA);
       CHK(finalize(s, np2, F_RZNEW));
       CHK(dot_partials<T>(s, n, z, Ap, &np2));
@@ -1237,8 +1306,8 @@ int pcg(vx_solver* s, T* u, T* r, T* z, T* p, T* Ap, T* t0, const std::vector<do
     };
     auto queue_next = [&]() -> int {
       const void* const key[4] = {r, z, t0, Ap};
-      return run_graphed(s, key, std::is_same<T, float>::value ? 1 : 0, l0, fuse_x1 ? 1 : 0,
-                         flexible ? 1 : 0, cycle_and_beta);
+      return run_graphed(s, key, std::is_same<T, float>::value ? 1 : 0, l0,
+                         fuse_z2 ? 2 : (fuse_x1 ? 1 : 0), flexible ? 1 : 0, cycle_and_beta);
     };
     bool queued = false;
     if (!predicted_done) {
@@ -2593,7 +2662,7 @@ int vx_drop_islands_noalloc(vx_handle s, const uint8_t* active, const int64_t* f
 int vx_time_kernel(vx_handle s, int32_t kind, int32_t l, int32_t prec, int32_t reps,
                    double* avg_ms) {
   CHK(check_level(s, l));
-  if (reps < 1 || !avg_ms || kind < 0 || kind > 2)
+  if (reps < 1 || !avg_ms || kind < 0 || kind > 3 || (kind == 3 && prec == VX_FP64))
     return fail(s, VX_ERR_ARG, "bad timing arguments");
   Level& L = s->lv[l];
   hipEvent_t a, b;
@@ -2609,6 +2678,8 @@ int vx_time_kernel(vx_handle s, int32_t kind, int32_t l, int32_t prec, int32_t r
     sc.cr = 0.1;
     if (kind == 0) return launch_tile<T, M_MATVEC, false>(s, l, x, nullptr, y, nullptr, nullptr);
     if (kind == 1) return launch_tile<T, M_JACOBI, false>(s, l, x, bb, y, nullptr, nullptr, &sc);
+    if (kind == 3)   // two Chebyshev steps from zero, b -> y
+      return launch_cheb_z2(s, l, 2, reinterpret_cast<const float*>(bb), reinterpret_cast<float*>(y));
     // as inside the smoother, the new iterate overwrites the previous one
     return launch_tile<T, M_CHEB, false>(s, l, x, bb, up, nullptr, nullptr, &sc);
   };
@@ -2686,8 +2757,10 @@ extern "C" int vx_autotune(vx_handle s, int32_t precision, int32_t reps) {
   s->nb = s->opt.max_batch;
   for (size_t li = 0; li < s->lv.size() && rc == VX_OK; ++li) {
     Level& L = s->lv[li];
-    for (int cls = 0; cls < 3 && rc == VX_OK; ++cls) {
+    for (int cls = 0; cls < 4 && rc == VX_OK; ++cls) {
       L.zc_tuned[pi][cls] = 0;
+      // class 3 (the two Chebyshev steps from zero in one launch) is FP32 only
+      if (cls == 3 && (pi == 0 || s->opt.smoother != VX_SMOOTH_CHEBYSHEV)) continue;
       std::vector<int> cs;
       cs.push_back(zchunk(s, L));   // the rule's choice competes too
       for (int c : cands)
@@ -2702,7 +2775,8 @@ extern "C" int vx_autotune(vx_handle s, int32_t precision, int32_t reps) {
         L.zc_tuned[pi][cls] = zc;
         // the variants the solve launches: class 0 the PCG operator (with the
         // p.Ap partials), 1 a smoothing step, 2 the last smoothing step of the
-        // V-cycle (with its dot-product partials)
+        // V-cycle (with its dot-product partials), 3 the first two steps of a
+        // Chebyshev pass from zero (M_CHEB_Z2)
         auto variant = [&](auto* x, auto* y, auto* bb, auto* w) -> int {
           using T = std::remove_pointer_t<decltype(x)>;
           SmoothCoef sc;
@@ -2714,6 +2788,9 @@ extern "C" int vx_autotune(vx_handle s, int32_t precision, int32_t reps) {
           if (cls == 0) return launch_tile_L<T, M_MATVEC, true>(s, L, x, nullptr, y, s->partial, &np);
           if (cls == 1)
             return launch_tile<T, M_CHEB, false>(s, (int)li, x, bb, w, nullptr, nullptr, &sc);
+          if (cls == 3)
+            return launch_cheb_z2(s, (int)li, 2, reinterpret_cast<const float*>(bb),
+                                  reinterpret_cast<float*>(w));
           return launch_tile<T, M_CHEB, true>(s, (int)li, x, bb, w, s->partial, &np, &sc);
         };
         auto launch = [&]() -> int {
@@ -2801,7 +2878,7 @@ extern "C" int vx_autotune(vx_handle s, int32_t precision, int32_t reps) {
 }
 
 extern "C" int vx_tuned_zchunk(vx_handle s, int32_t level, int32_t precision, int32_t cls) {
-  if (!s || level < 0 || level >= (int)s->lv.size() || cls < 0 || cls > 2) return VX_ERR_ARG;
+  if (!s || level < 0 || level >= (int)s->lv.size() || cls < 0 || cls > 3) return VX_ERR_ARG;
   const Level& L = s->lv[level];
   return zchunk(s, L, precision == VX_FP64 ? 0 : 1, cls);
 }
