@@ -1023,14 +1023,14 @@ int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, T* w0, bool dot, int* n_part
   // descend.  The restriction into a level that smooths from zero (not the
   // coarsest) also writes that level's first step into its t vector, where
   // the smoother expects it (first_in_tmp).
-  bool restricted_x1 = false;
+  bool restricted_x1 = false, z2_next = false;
   for (int li = l0; li <= bottom; ++li) {
     Level& L = s->lv[li];
     const T* b = li == l0 ? b0 : Sel<T>::b(L);
     T* x = li == l0 ? z0 : Sel<T>::x(L);
     T* t = li == l0 ? t0 : Sel<T>::t(L);
     T* w = li == l0 ? w0 : Sel<T>::w(L);
-    const bool pre_all = std::is_same<T, float>::value && li == l0 && first_done == 2;
+    const bool pre_all = std::is_same<T, float>::value && (li == l0 ? first_done == 2 : z2_next);
     const bool pre_done = li == l0 ? first_done == 1 : restricted_x1;
     const int ns = level_ns(s, li);
     if (li == bottom && learned) {
@@ -1069,6 +1069,12 @@ int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, T* w0, bool dot, int* n_part
     if (l0 == 0) CHK(capture_residual<T>(s, li, t));
     Level& C = s->lv[li + 1];
     restricted_x1 = li + 1 < nl - 1 && li + 1 != fuse_li && first_in_tmp(s, level_ns(s, li + 1));
+    // a two-step Chebyshev pass from zero on a tile-smoothed level runs as one
+    // M_CHEB_Z2 launch, which forms the first step itself
+    z2_next = restricted_x1 && std::is_same<T, float>::value && li + 1 != bottom &&
+              s->opt.smoother == VX_SMOOTH_CHEBYSHEV && level_ns(s, li + 1) == 2 &&
+              !block_eligible(s, li + 1, 2, false);
+    if (z2_next) restricted_x1 = false;
     if (restricted_x1)
       CHK(restrict_to<T>(s, C, t, Sel<T>::b(C), nb, Sel<T>::t(C),
                          first_step_coef(s, level_ns(s, li + 1))));
