@@ -298,8 +298,9 @@ int vx_drop_islands_noalloc(vx_handle h, const uint8_t* active_dev,
 /* Timing helper: average duration (ms) of `reps` back-to-back launches of the
  * level-l operator kernel (kind 0 = masked matvec, 1 = damped-Jacobi sweep,
  * 2 = Chebyshev smoothing step, 3 = the first two Chebyshev steps from zero in
- * one launch, FP32 only) in the given precision, measured with HIP events on
- * the handle's stream. */
+ * one launch, 4 = the residual restricted to level l+1 in the same launch;
+ * 3 and 4 FP32 only) in the given precision, measured with HIP events on the
+ * handle's stream. */
 int vx_time_kernel(vx_handle h, int32_t kind, int32_t level, int32_t precision,
                    int32_t reps, double* avg_ms);
 
@@ -433,7 +434,9 @@ const char* vx_inside_last_error(void);
 int vx_autotune(vx_handle h, int32_t precision, int32_t reps);
 /* The z-chunk the handle uses for a level, precision and class (0 the PCG
  * operator, 1 a smoothing step, 2 the last smoothing step with its dot
- * partials, 3 two Chebyshev steps from zero in one launch). */
+ * partials, 3 two Chebyshev steps from zero in one launch, 4 the residual
+ * restricted to the next level in the same launch: (zc + 1) / 2 coarse planes
+ * per work-group). */
 int vx_tuned_zchunk(vx_handle h, int32_t level, int32_t precision, int32_t cls);
 /* The blocked smoother's plan for an FP32 pass of `steps` steps on a level
  * (one right-hand side): owned box edge and steps per launch, or 0 and 0 when

@@ -447,7 +447,7 @@ def test_autotuned_solver_same_solution():
         if tune:
             assert set(mg.tuned) == set(range(len(mg.levels)))
             for zcs in mg.tuned.values():
-                assert len(zcs) == 4 and all(1 <= z <= 65 for z in zcs)
+                assert len(zcs) == 5 and all(1 <= z <= 65 for z in zcs)
         mg.close()
     assert _rel(out[True], out[False]) < 1e-9
 

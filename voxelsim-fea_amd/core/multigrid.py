@@ -400,11 +400,11 @@ class MGSolver:
     def autotune(self, precision=None, reps=5):
         """Pick each level's tile z-chunk by timing (needs the density set);
         returns {level: (operator zc, smoother zc, smoother-with-dot zc,
-        two-Chebyshev-steps-from-zero zc)}."""
+        two-Chebyshev-steps-from-zero zc, residual-restriction zc)}."""
         prec = _PRECISION[precision or self.precision]
         self._stream_bind()
         self._check(self._lib.vx_autotune(self._h, prec, int(reps)))
-        return {li: tuple(self._lib.vx_tuned_zchunk(self._h, li, prec, c) for c in range(4))
+        return {li: tuple(self._lib.vx_tuned_zchunk(self._h, li, prec, c) for c in range(5))
                 for li in range(len(self.levels))}
 
     def _level_steps(self, li):
@@ -560,7 +560,8 @@ class MGSolver:
         self._unpadded_only('time_kernel')
         out = ctypes.c_double()
         self._stream_bind()
-        self._check(self._lib.vx_time_kernel(self._h, {"matvec": 0, "jacobi": 1, "chebyshev": 2, "chebyshev2_zero": 3}[kind], level,
+        self._check(self._lib.vx_time_kernel(self._h, {"matvec": 0, "jacobi": 1, "chebyshev": 2, "chebyshev2_zero": 3,
+                                              "residual_restrict": 4}[kind], level,
                                              _PRECISION[precision], reps, ctypes.byref(out)))
         return out.value
 

@@ -27,7 +27,7 @@ constexpr int NT = TILE * TILE;       // threads per tile work-group
 constexpr int FOOTVALS = FOOT * FOOT * 3;
 constexpr int PRE = (FOOTVALS + NT - 1) / NT;
 
-enum { M_MATVEC = 0, M_RESID = 1, M_JACOBI = 2, M_CHEB = 3, M_CHEB_Z2 = 4 };
+enum { M_MATVEC = 0, M_RESID = 1, M_JACOBI = 2, M_CHEB = 3, M_CHEB_Z2 = 4, M_RESR = 5 };
 
 // PCG scalars of one right-hand side, in device memory.
 struct Scalars {
@@ -72,6 +72,16 @@ struct TileArgs {
   // M_CHEB_Z2: D^-1 per node and the weight of the first step from zero
   const T* __restrict__ minv = nullptr;
   T c1 = T(0);
+  // M_RESR: the coarse level (nodes Lcx x Lcy x Lcz, zc counts its planes per
+  // work-group), its right-hand side rc, mask and D^-1, and (cx1 set) its
+  // first smoothing step from zero w0 D^-1 rc
+  int Lcx = 0, Lcy = 0, Lcz = 0;
+  long long cvstride = 0;
+  T* __restrict__ rc = nullptr;
+  T* __restrict__ cx1 = nullptr;
+  const uint8_t* __restrict__ cmask = nullptr;
+  const T* __restrict__ cminv = nullptr;
+  T w0 = T(0);
 };
 
 __device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
@@ -179,6 +189,13 @@ __device__ inline double block_sum(double v, double* sred) {
 //                 formed pointwise from each staged value (D^-1 from minv),
 //                 and the second, out = x1 + mask (cu x1 + cr (b - K x1)/d),
 //                 takes b at the node from the same staged plane
+//       M_RESR  : the residual b - (mask ? K u : 0) restricted at once to the
+//                 coarse level (R = P^T, the coarse mask, optionally the coarse
+//                 first step): tiles step by 14 fine nodes from -1, so a tile's
+//                 15 owned fine columns hold the full 3-wide stencils of 7
+//                 coarse columns (neighbouring tiles both compute the shared
+//                 column), and work-groups own whole coarse planes; the fine
+//                 residual goes through LDS, plane by plane, never to memory
 // DOT   adds the work-group partial of  sum out*u (MATVEC) or sum out*b
 //       (the smoothers) to partial[block]; RESID: sum out*out.
 // ---------------------------------------------------------------------------
@@ -199,8 +216,13 @@ constexpr int NBUF = 4;
 template <typename T, int MODE, bool DOT>
 __device__ __forceinline__ void tile_run(const TileArgs<T>& a, int vt, int slot,
                                          const T* __restrict__ au, const T* __restrict__ ab,
-                                         const T* __restrict__ aup, T* __restrict__ aout) {
+                                         const T* __restrict__ aup, T* __restrict__ aout,
+                                         T* __restrict__ arc = nullptr,
+                                         T* __restrict__ ax1 = nullptr) {
   constexpr bool RUP = MODE == M_CHEB_Z2;
+  constexpr bool RES = MODE == M_RESR;
+  constexpr int RSTEP = 2 * (OWN / 2);          // M_RESR: fine nodes per tile step (14)
+  constexpr int RCOL = RSTEP / 2;               // coarse columns per tile edge (7)
   constexpr int FN = FOOT * FOOT;               // footprint nodes of a plane
   constexpr int NPN = (FN + NT - 1) / NT;       // of them per thread
   __shared__ T sP[NBUF][FOOTVALS];
@@ -209,6 +231,8 @@ __device__ __forceinline__ void tile_run(const TileArgs<T>& a, int vt, int slot,
   // M_CHEB_Z2: c1 D^-1 and the mask byte of each staged node
   __shared__ T sW[RUP ? NBUF : 1][RUP ? FN : 1];
   __shared__ uint8_t sM[RUP ? NBUF : 1][RUP ? FN : 1];
+  // M_RESR: the residual of the owned nodes of one fine plane
+  __shared__ T sRes[RES ? OWN * OWN * 3 : 1];
   const int bx = vt % a.gx;
   const int byz = vt / a.gx;
   const int by = byz % a.gy;
@@ -216,14 +240,18 @@ __device__ __forceinline__ void tile_run(const TileArgs<T>& a, int vt, int slot,
 
   const int tid = threadIdx.x;
   const int tx = tid & (TILE - 1), ty = tid / TILE;
-  const int X0 = bx * OWN, Y0 = by * OWN;
-  const int Z0 = bz * a.zc;
-  const int Z1 = min(Z0 + a.zc, a.Lz);
+  // owned nodes X0 .. X0+14 (M_RESR: from -1 in steps of 14, see above) and
+  // output planes Z0 .. Z1-1 (M_RESR: fine planes 2 cz0 - 1 .. 2 (cz0 + zc) - 1
+  // for the coarse planes cz0 .. cz0 + zc - 1); planes below 0 are skipped
+  const int X0 = RES ? bx * RSTEP - 1 : bx * OWN, Y0 = RES ? by * RSTEP - 1 : by * OWN;
+  const int Z0 = RES ? 2 * bz * a.zc - 1 : bz * a.zc;
+  const int Z1 = RES ? min(2 * (bz * a.zc + a.zc), a.Lz) : min(Z0 + a.zc, a.Lz);
+  const int Zlo = max(Z0, 0);
   const int ex = X0 - 1 + tx, ey = Y0 - 1 + ty;
   const bool col_in = ex >= 0 && ex < a.nx && ey >= 0 && ey < a.ny;
   const int e0 = col_in ? ey * a.nx + ex : 0;
   const int nxy = a.nx * a.ny;
-  const bool owner = tx >= 1 && ty >= 1 && ex < a.Lx && ey < a.Ly;
+  const bool owner = tx >= 1 && ty >= 1 && ex >= 0 && ey >= 0 && ex < a.Lx && ey < a.Ly;
   const int n0 = ey * a.Lx + ex;
   const int plane = a.Lx * a.Ly;
   const int plane3 = 3 * plane;
@@ -336,6 +364,7 @@ __device__ __forceinline__ void tile_run(const TileArgs<T>& a, int vt, int slot,
   T Pb[3][4], acc[3][4];
   T Eprev = T(0);
   double dacc = 0.0;
+  T ra = T(0), rh = T(0);   // M_RESR: this thread's coarse value and odd-plane partial
 
   // prologue: plane Z0-1 (bottom of the first layer) and plane Z0
   fetch(Z0 - 1);
@@ -369,19 +398,19 @@ __device__ __forceinline__ void tile_run(const TileArgs<T>& a, int vt, int slot,
   };
   // epilogue operands of the owners (node n0 of plane pz).  b and up are
   // read as one 96-bit load per node; a non-owner, or a plane outside the
-  // chunk, reads node 0 of plane Z0 (a valid address) and never uses it:
-  // only the next layer's owners of planes Z0 .. Z1-1 do.
+  // chunk, reads node 0 of plane Zlo (a valid address) and never uses it:
+  // only the next layer's owners of planes Zlo .. Z1-1 do.
   const int moff = owner ? n0 : BUF_OOB;
   const int vnode = 3 * (owner ? n0 : 0);
   unsigned mk_n = 0;
   T b_n[3] = {T(0), T(0), T(0)}, up_n[3] = {T(0), T(0), T(0)};
   auto loadEpi = [&](int pz) {
-    if (RUP) return;   // the node's mask, r' and x1 come from LDS
-    const bool zin = pz >= Z0 && pz < Z1;
+    if (RUP) return;   // the node's mask, b and x1 come from LDS
+    const bool zin = pz >= Zlo && pz < Z1;
     const int pzc = zin ? pz : 0;
     const auto rm = buf_rsrc(a.mask + (long long)pzc * plane, zin ? plane : 0);
     mk_n = buf_ld_u8(rm, moff);
-    const long long voff = (long long)(zin ? pz : Z0) * plane3 + vnode;
+    const long long voff = (long long)(zin ? pz : Zlo) * plane3 + vnode;
     if (NEEDB) ld3(ab + voff, b_n);
     if (NEEDUP && has_up) ld3(aup + voff, up_n);
   };
@@ -436,9 +465,11 @@ __device__ __forceinline__ void tile_run(const TileArgs<T>& a, int vt, int slot,
     if (iz >= Z0) loadEpi(iz + 1);
     __syncthreads();
 
-    if (iz >= Z0 && owner) {
+    if (iz >= Zlo && owner) {
       // output plane base (uniform, 64-bit) + this node's 32-bit offset
-      T* const op = aout + (long long)iz * plane3 + 3 * n0;
+      // (M_RESR: the node's slot in sRes)
+      T* const op = RES ? &sRes[((ty - 1) * OWN + (tx - 1)) * 3]
+                        : aout + (long long)iz * plane3 + 3 * n0;
       T y[3], ub[3];
       unsigned mke = mk;
 #pragma unroll
@@ -465,7 +496,7 @@ __device__ __forceinline__ void tile_run(const TileArgs<T>& a, int vt, int slot,
           op[c] = o;
           if (DOT) dacc = fmaT((double)o, (double)ub[c], dacc);
         }
-      } else if (MODE == M_RESID) {
+      } else if (MODE == M_RESID || RES) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           const T o = bb[c] - (((mke >> c) & 1u) ? y[c] : T(0));
@@ -489,6 +520,47 @@ __device__ __forceinline__ void tile_run(const TileArgs<T>& a, int vt, int slot,
           }
           op[c] = o;
           if (DOT) dacc = fmaT((double)o, (double)bb[c], dacc);
+        }
+      }
+    }
+    if (RES && iz >= Zlo) {
+      // restriction of this fine plane's residual into the coarse columns of
+      // the tile, in the tap order of k_restrict_z: an even plane 2 jz adds
+      // its weighted taps to h (the odd plane below, weight 1/2) and, if it is
+      // the grid's last plane, completes coarse plane jz; an odd plane 2 jz + 1
+      // completes jz with its half-weighted taps (unless it is the chunk's
+      // first plane) and starts h of jz + 1
+      __syncthreads();
+      if (tid < RCOL * RCOL * 3) {
+        const int col = tid / 3, c = tid - 3 * col;
+        const int cjy = col / RCOL, cjx = col - cjy * RCOL;
+        const int jx = bx * RCOL + cjx, jy = by * RCOL + cjy;
+        const bool odd = iz & 1;
+        const T so = odd ? T(0.5) : T(1);
+        T acc_a = odd ? ra : rh;   // the chain this plane continues
+        T acc_h = T(0);
+#pragma unroll
+        for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+          for (int dx = -1; dx <= 1; ++dx) {
+            // a tap outside the grid reads the column's own node with weight 0
+            const int fx = 2 * jx + dx, fy = 2 * jy + dy;
+            const bool in = fx >= 0 && fx < a.Lx && fy >= 0 && fy < a.Ly;
+            const int lx = (in ? fx : 2 * jx) - X0, ly = (in ? fy : 2 * jy) - Y0;
+            const T wk = so * (in ? (dx == 0 ? T(1) : T(0.5)) * (dy == 0 ? T(1) : T(0.5)) : T(0));
+            const T qv = sRes[(ly * OWN + lx) * 3 + c];
+            acc_a = fmaT(wk, qv, acc_a);
+            acc_h = fmaT(wk, qv, acc_h);
+          }
+        const int jz = iz >> 1;
+        const bool done = odd ? iz > Z0 : iz + 1 >= a.Lz;
+        if (odd) rh = acc_h;
+        else ra = acc_a;
+        if (done && jx < a.Lcx && jy < a.Lcy) {
+          const long long J = ((long long)jz * a.Lcy + jy) * a.Lcx + jx;
+          const bool fr = (a.cmask[J] >> c) & 1u;
+          arc[3 * J + c] = fr ? acc_a : T(0);
+          if (ax1) ax1[3 * J + c] = fr ? (a.w0 * a.cminv[J]) * acc_a : T(0);
         }
       }
     }
@@ -519,8 +591,12 @@ void k_tile(TileArgs<T> a) {
   const int rhs = v / G;
   const int vt = v - rhs * G;
   const long long voff = (long long)rhs * a.vstride;
-  tile_run<T, MODE, DOT>(a, vt, v, a.u + voff, a.b ? a.b + voff : nullptr,
-                         a.up ? a.up + voff : nullptr, a.out + voff);
+  if (MODE == M_RESR)
+    tile_run<T, MODE, DOT>(a, vt, v, a.u + voff, a.b + voff, nullptr, nullptr,
+                           a.rc + rhs * a.cvstride, a.cx1 ? a.cx1 + rhs * a.cvstride : nullptr);
+  else
+    tile_run<T, MODE, DOT>(a, vt, v, a.u + voff, a.b ? a.b + voff : nullptr,
+                           a.up ? a.up + voff : nullptr, a.out + voff);
 }
 
 // Inverse Jacobi diagonal per node of a level: 1/(kd * sum of the <= 8

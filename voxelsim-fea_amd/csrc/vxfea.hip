@@ -49,7 +49,7 @@ struct Level {
   // autotuned node planes per work-group: [precision 0 FP64 / 1 FP32][class
   // 0 operator (matvec forms), 1 smoother / residual forms, 2 smoother forms
   // with the dot-product partials]; 0 = rule
-  int zc_tuned[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+  int zc_tuned[2][5] = {{0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}};
 };
 
 template <typename T> struct Sel;
@@ -305,6 +305,45 @@ int restrict_to(vx_solver* s, Level& C, const T* rf, T* rc, int nb, T* x1 = null
   const TrGrid g = tr_grid((long long)C.Lx * C.Ly, C.Lz, false);
   hipLaunchKernelGGL(k_restrict_z<T>, dim3(tr_blocks(g, nb)), dim3(TR_THREADS), 0, s->stream,
                      C.Lx, C.Ly, C.Lz, g, nb, rf, C.mask, rc, (const T*)Sel<T>::mi(C), (T)w0, x1);
+  LAUNCHCHK();
+  return VX_OK;
+}
+
+// The residual b - K x of level li restricted straight into level li+1
+// (M_RESR): rc = the coarse right-hand side, x1 (if set) the coarse level's
+// first smoothing step w0 D^-1 rc.  FP32.  The fine residual never goes to
+// memory.
+int launch_resr(vx_solver* s, int li, const float* x, const float* b, float* rc, float* x1,
+                double w0) {
+  Level& L = s->lv[li];
+  const Level& C = s->lv[li + 1];
+  TileArgs<float> a;
+  a.nx = L.nx; a.ny = L.ny; a.nz = L.nz;
+  a.Lx = L.Lx; a.Ly = L.Ly; a.Lz = L.Lz;
+  a.zc = std::max(1, (zchunk(s, L, 1, 4) + 1) / 2);   // coarse planes per work-group
+  a.nb = s->nb;
+  a.vstride = 3 * L.Nn;
+  a.u = x; a.E = L.E32; a.mask = L.mask; a.b = b; a.out = nullptr;
+  a.omega = 0.f;
+  a.up = nullptr;
+  a.kd = (float)s->kd;
+  a.K = kconst<float>(s);
+  a.partial = nullptr;
+  a.Lcx = C.Lx; a.Lcy = C.Ly; a.Lcz = C.Lz;
+  a.cvstride = 3 * C.Nn;
+  a.rc = rc;
+  a.cx1 = x1;
+  a.cmask = C.mask;
+  a.cminv = C.mi32;
+  a.w0 = (float)w0;
+  constexpr int RCOL = 2 * (OWN / 2) / 2;
+  a.gx = (C.Lx + RCOL - 1) / RCOL;
+  a.gy = (C.Ly + RCOL - 1) / RCOL;
+  a.gz = (C.Lz + a.zc - 1) / a.zc;
+  const long long G = (long long)a.gx * a.gy * a.gz;
+  const long long launch = ((G * s->nb + 7) / 8) * 8;   // XCD-ordered 1D grid
+  hipLaunchKernelGGL((k_tile<float, M_RESR, false>), dim3((unsigned)launch), dim3(NT), 0,
+                     s->stream, a);
   LAUNCHCHK();
   return VX_OK;
 }
@@ -1064,9 +1103,8 @@ int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, T* w0, bool dot, int* n_part
       CHK(launch_cheb_z2(s, li, ns, reinterpret_cast<const float*>(b), reinterpret_cast<float*>(x)));
     else
       CHK(smooth_step<T>(s, li, nullptr, b, ns, x, t, w, false, nullptr, pre_done));
-    // residual into t, then restrict into b of level li+1
-    CHK((launch_tile<T, M_RESID, false>(s, li, x, b, t, nullptr, nullptr)));
-    if (l0 == 0) CHK(capture_residual<T>(s, li, t));
+    // residual into t, then restrict into b of level li+1 (with, for a level
+    // that smooths from zero, its first step into its t vector)
     Level& C = s->lv[li + 1];
     restricted_x1 = li + 1 < nl - 1 && li + 1 != fuse_li && first_in_tmp(s, level_ns(s, li + 1));
     // a two-step Chebyshev pass from zero on a tile-smoothed level runs as one
@@ -1075,11 +1113,20 @@ int vcycle(vx_solver* s, const T* b0, T* z0, T* t0, T* w0, bool dot, int* n_part
               s->opt.smoother == VX_SMOOTH_CHEBYSHEV && level_ns(s, li + 1) == 2 &&
               !block_eligible(s, li + 1, 2, false);
     if (z2_next) restricted_x1 = false;
-    if (restricted_x1)
-      CHK(restrict_to<T>(s, C, t, Sel<T>::b(C), nb, Sel<T>::t(C),
-                         first_step_coef(s, level_ns(s, li + 1))));
-    else
-      CHK(restrict_to<T>(s, C, t, Sel<T>::b(C), nb));
+    const double x1w = restricted_x1 ? first_step_coef(s, level_ns(s, li + 1)) : 0.0;
+    T* const cx1 = restricted_x1 ? Sel<T>::t(C) : nullptr;
+    if (std::is_same<T, float>::value && !(l0 == 0 && s->cap_buf && s->cap_level == li)) {
+      // the residual goes from the tiles straight into the restriction
+      CHK(launch_resr(s, li, reinterpret_cast<const float*>(x), reinterpret_cast<const float*>(b),
+                      reinterpret_cast<float*>(Sel<T>::b(C)), reinterpret_cast<float*>(cx1), x1w));
+    } else {
+      CHK((launch_tile<T, M_RESID, false>(s, li, x, b, t, nullptr, nullptr)));
+      if (l0 == 0) CHK(capture_residual<T>(s, li, t));
+      if (restricted_x1)
+        CHK(restrict_to<T>(s, C, t, Sel<T>::b(C), nb, cx1, x1w));
+      else
+        CHK(restrict_to<T>(s, C, t, Sel<T>::b(C), nb));
+    }
   }
   // ascend (from below the fused levels' result when k_bottom ran)
   for (int li = (fuse_li >= 0 ? fuse_li : bottom) - 1; li >= l0; --li) {
@@ -2668,7 +2715,8 @@ int vx_drop_islands_noalloc(vx_handle s, const uint8_t* active, const int64_t* f
 int vx_time_kernel(vx_handle s, int32_t kind, int32_t l, int32_t prec, int32_t reps,
                    double* avg_ms) {
   CHK(check_level(s, l));
-  if (reps < 1 || !avg_ms || kind < 0 || kind > 3 || (kind == 3 && prec == VX_FP64))
+  if (reps < 1 || !avg_ms || kind < 0 || kind > 4 || (kind >= 3 && prec == VX_FP64) ||
+      (kind == 4 && l + 1 >= (int)s->lv.size()))
     return fail(s, VX_ERR_ARG, "bad timing arguments");
   Level& L = s->lv[l];
   hipEvent_t a, b;
@@ -2686,6 +2734,9 @@ int vx_time_kernel(vx_handle s, int32_t kind, int32_t l, int32_t prec, int32_t r
     if (kind == 1) return launch_tile<T, M_JACOBI, false>(s, l, x, bb, y, nullptr, nullptr, &sc);
     if (kind == 3)   // two Chebyshev steps from zero, b -> y
       return launch_cheb_z2(s, l, 2, reinterpret_cast<const float*>(bb), reinterpret_cast<float*>(y));
+    if (kind == 4)   // the residual b - K x restricted into level l + 1
+      return launch_resr(s, l, reinterpret_cast<const float*>(x), reinterpret_cast<const float*>(bb),
+                         s->lv[l + 1].b32, nullptr, 0.0);
     // as inside the smoother, the new iterate overwrites the previous one
     return launch_tile<T, M_CHEB, false>(s, l, x, bb, up, nullptr, nullptr, &sc);
   };
@@ -2763,10 +2814,12 @@ extern "C" int vx_autotune(vx_handle s, int32_t precision, int32_t reps) {
   s->nb = s->opt.max_batch;
   for (size_t li = 0; li < s->lv.size() && rc == VX_OK; ++li) {
     Level& L = s->lv[li];
-    for (int cls = 0; cls < 4 && rc == VX_OK; ++cls) {
+    for (int cls = 0; cls < 5 && rc == VX_OK; ++cls) {
       L.zc_tuned[pi][cls] = 0;
-      // class 3 (the two Chebyshev steps from zero in one launch) is FP32 only
+      // class 3 (the two Chebyshev steps from zero in one launch) and class 4
+      // (the residual restricted in the tiles, a level with a coarser one) are FP32 only
       if (cls == 3 && (pi == 0 || s->opt.smoother != VX_SMOOTH_CHEBYSHEV)) continue;
+      if (cls == 4 && (pi == 0 || li + 1 >= s->lv.size())) continue;
       std::vector<int> cs;
       cs.push_back(zchunk(s, L));   // the rule's choice competes too
       for (int c : cands)
@@ -2782,7 +2835,8 @@ extern "C" int vx_autotune(vx_handle s, int32_t precision, int32_t reps) {
         // the variants the solve launches: class 0 the PCG operator (with the
         // p.Ap partials), 1 a smoothing step, 2 the last smoothing step of the
         // V-cycle (with its dot-product partials), 3 the first two steps of a
-        // Chebyshev pass from zero (M_CHEB_Z2)
+        // Chebyshev pass from zero (M_CHEB_Z2), 4 the residual restricted in
+        // the tiles (M_RESR; its work-groups own (zc + 1) / 2 coarse planes)
         auto variant = [&](auto* x, auto* y, auto* bb, auto* w) -> int {
           using T = std::remove_pointer_t<decltype(x)>;
           SmoothCoef sc;
@@ -2797,6 +2851,9 @@ extern "C" int vx_autotune(vx_handle s, int32_t precision, int32_t reps) {
           if (cls == 3)
             return launch_cheb_z2(s, (int)li, 2, reinterpret_cast<const float*>(bb),
                                   reinterpret_cast<float*>(w));
+          if (cls == 4)
+            return launch_resr(s, (int)li, reinterpret_cast<const float*>(x),
+                               reinterpret_cast<const float*>(bb), s->lv[li + 1].b32, nullptr, 0.0);
           return launch_tile<T, M_CHEB, true>(s, (int)li, x, bb, w, s->partial, &np, &sc);
         };
         auto launch = [&]() -> int {
@@ -2884,7 +2941,7 @@ extern "C" int vx_autotune(vx_handle s, int32_t precision, int32_t reps) {
 }
 
 extern "C" int vx_tuned_zchunk(vx_handle s, int32_t level, int32_t precision, int32_t cls) {
-  if (!s || level < 0 || level >= (int)s->lv.size() || cls < 0 || cls > 3) return VX_ERR_ARG;
+  if (!s || level < 0 || level >= (int)s->lv.size() || cls < 0 || cls > 4) return VX_ERR_ARG;
   const Level& L = s->lv[level];
   return zchunk(s, L, precision == VX_FP64 ? 0 : 1, cls);
 }
