@@ -2846,6 +2846,11 @@ extern "C" int vx_autotune(vx_handle s, int32_t precision, int32_t reps) {
           sc.up = w;
           int np = 0;
           if (cls == 0) return launch_tile_L<T, M_MATVEC, true>(s, L, x, nullptr, y, s->partial, &np);
+          // class 1 on an FP32 level with two-step passes is only the first
+          // post-smoothing step (the pre-smoothing runs as class 3): time that
+          if (cls == 1 && pi == 1 && s->opt.smoother == VX_SMOOTH_CHEBYSHEV &&
+              level_ns(s, (int)li) == 2)
+            return launch_tile<T, M_JACOBI, false>(s, (int)li, x, bb, w, nullptr, nullptr, &sc);
           if (cls == 1)
             return launch_tile<T, M_CHEB, false>(s, (int)li, x, bb, w, nullptr, nullptr, &sc);
           if (cls == 3)
