@@ -1210,34 +1210,67 @@ __global__ __launch_bounds__(BLK_THREADS) void k_smooth_block(BlockArgs a) {
     if (k * nt + tid < NER) sE[k * nt + tid] = ve[k];
   __syncthreads();
 
+  // Element slots are indexed by the element's place in the region (the same
+  // in every step), so each thread's elements and each node's eight slots are
+  // found once per launch: thread t applies region elements t and t + nt (those
+  // inside a step's element box), a node reads slot 3 q NER + its element q.
+  int el[BLK_NPT][3], en0[BLK_NPT];
+#pragma unroll
+  for (int k = 0; k < BLK_NPT; ++k) {
+    const int e = tid + k * nt;
+    const int row = qdiv(e, rqx), lz = qdiv(row, rqy);
+    el[k][0] = e < NER ? e - row * QX : -1;   // -1: no element (never inside a box)
+    el[k][1] = row - lz * QY;
+    el[k][2] = lz;
+    en0[k] = (lz * RY + el[k][1]) * RX + el[k][0];
+  }
+  int nsl[BLK_NPT];        // the slot of a node's element 0 (q = 0), other q at fixed offsets
+  unsigned nok[BLK_NPT];   // bit q: the node's element q lies in the region
+#pragma unroll
+  for (int k = 0; k < BLK_NPT; ++k) {
+    const int lx = gxn[k] - rx0, ly = gyn[k] - ry0, lz = gzn[k] - rz0;
+    nsl[k] = (lz * QY + ly) * QX + lx;
+    unsigned okb = 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int ex = lx - (q & 1), ey = ly - ((q >> 1) & 1), ez = lz - (q >> 2);
+      const bool ok = ((unsigned)ex < (unsigned)QX) & ((unsigned)ey < (unsigned)QY) &
+                      ((unsigned)ez < (unsigned)(RZ - 1));
+      okb |= ok ? 1u << q : 0u;
+    }
+    nok[k] = okb;
+  }
+  const int QXY = QX * QY;
   for (int j = 0; j < S; ++j) {
     const int h = S - 1 - j;   // the step updates the owned box grown by h
     const int ux0 = max(0, o0x - h), ux1 = min(a.Lx, o1x + h);
     const int uy0 = max(0, o0y - h), uy1 = min(a.Ly, o1y + h);
     const int uz0 = max(0, o0z - h), uz1 = min(a.Lz, o1z + h);
     const int form = a.form[j];
-    // elements touching the updated nodes
-    const int ex0 = max(0, ux0 - 1), ey0 = max(0, uy0 - 1), ez0 = max(0, uz0 - 1);
-    const int EX = min(a.nx, ux1) - ex0, EY = min(a.ny, uy1) - ey0, EZ = min(a.nz, uz1) - ez0;
-    const int NE = EX * EY * EZ;
+    // elements touching the updated nodes (in region coordinates)
+    const int bx0 = max(0, ux0 - 1) - rx0, by0 = max(0, uy0 - 1) - ry0, bz0 = max(0, uz0 - 1) - rz0;
+    const int EX = min(a.nx, ux1) - rx0 - bx0, EY = min(a.ny, uy1) - ry0 - by0,
+              EZ = min(a.nz, uz1) - rz0 - bz0;
     if (form != BLK_ZERO) {
-      const float rex = 1.f / (float)EX, rey = 1.f / (float)EY;
-      for (int e = tid; e < NE; e += nt) {
-        const int row = qdiv(e, rex), ez = qdiv(row, rey);
-        const int lx = ex0 - rx0 + e - row * EX, ly = ey0 - ry0 + row - ez * EY,
-                  lz = ez0 - rz0 + ez;
+#pragma unroll
+      for (int k = 0; k < BLK_NPT; ++k) {
+        if (!(((unsigned)(el[k][0] - bx0) < (unsigned)EX) & ((unsigned)(el[k][1] - by0) < (unsigned)EY) &
+              ((unsigned)(el[k][2] - bz0) < (unsigned)EZ)))
+          continue;
+        const int e = tid + k * nt;
+        const int n0 = en0[k];
         float u[8][3], y[8][3];
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-          const int n = ((lz + (q >> 2)) * RY + (ly + ((q >> 1) & 1))) * RX + (lx + (q & 1));
+          const int n = n0 + (q >> 2) * RX * RY + ((q >> 1) & 1) * RX + (q & 1);
 #pragma unroll
           for (int c = 0; c < 3; ++c) u[q][c] = sx[3 * n + c];
         }
-        elem_apply<float>(u, sE[(lz * QY + ly) * QX + lx], a.K, y);
+        elem_apply<float>(u, sE[e], a.K, y);
 #pragma unroll
         for (int q = 0; q < 8; ++q)
 #pragma unroll
-          for (int c = 0; c < 3; ++c) sye[(3 * q + c) * NE + e] = y[q][c];
+          for (int c = 0; c < 3; ++c) sye[(3 * q + c) * NER + e] = y[q][c];
       }
       __syncthreads();
     }
@@ -1249,28 +1282,22 @@ __global__ __launch_bounds__(BLK_THREADS) void k_smooth_block(BlockArgs a) {
             ((unsigned)(Z - uz0) < (unsigned)(uz1 - uz0))))
         continue;
       const int n = tid + k * nt;
-      // the node's <= 8 element slots, read without branches (a missing
-      // element reads slot 0 and adds +0)
-      int sl[8];
-      bool ok[8];
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const int ex = X - (q & 1) - ex0, ey = Y - ((q >> 1) & 1) - ey0, ez = Z - (q >> 2) - ez0;
-        ok[q] = ((unsigned)ex < (unsigned)EX) & ((unsigned)ey < (unsigned)EY) &
-                ((unsigned)ez < (unsigned)EZ);
-        sl[q] = ok[q] ? 3 * q * NE + (ez * EY + ey) * EX + ex : 0;
-      }
       float kx[3] = {0.f, 0.f, 0.f};
       if (form != BLK_ZERO) {
+        // the node's <= 8 element slots, read without branches (a missing
+        // element reads the node's q = 0 slot and adds +0)
         float v[8][3];
 #pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const bool ok = (nok[k] >> q) & 1u;
+          const int sl = 3 * q * NER + (ok ? nsl[k] - (q & 1) - ((q >> 1) & 1) * QX - (q >> 2) * QXY : 0);
+#pragma unroll
+          for (int c = 0; c < 3; ++c) v[q][c] = sye[sl + c * NER];
+        }
+#pragma unroll
         for (int q = 0; q < 8; ++q)
 #pragma unroll
-          for (int c = 0; c < 3; ++c) v[q][c] = sye[sl[q] + c * NE];
-#pragma unroll
-        for (int q = 0; q < 8; ++q)
-#pragma unroll
-          for (int c = 0; c < 3; ++c) kx[c] += ok[q] ? v[q][c] : 0.f;
+          for (int c = 0; c < 3; ++c) kx[c] += ((nok[k] >> q) & 1u) ? v[q][c] : 0.f;
       }
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
