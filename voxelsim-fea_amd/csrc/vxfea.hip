@@ -2715,7 +2715,7 @@ int vx_drop_islands_noalloc(vx_handle s, const uint8_t* active, const int64_t* f
 int vx_time_kernel(vx_handle s, int32_t kind, int32_t l, int32_t prec, int32_t reps,
                    double* avg_ms) {
   CHK(check_level(s, l));
-  if (reps < 1 || !avg_ms || kind < 0 || kind > 4 || (kind >= 3 && prec == VX_FP64) ||
+  if (reps < 1 || !avg_ms || kind < 0 || kind > 5 || ((kind == 3 || kind == 4) && prec == VX_FP64) ||
       (kind == 4 && l + 1 >= (int)s->lv.size()))
     return fail(s, VX_ERR_ARG, "bad timing arguments");
   Level& L = s->lv[l];
@@ -2737,6 +2737,10 @@ int vx_time_kernel(vx_handle s, int32_t kind, int32_t l, int32_t prec, int32_t r
     if (kind == 4)   // the residual b - K x restricted into level l + 1
       return launch_resr(s, l, reinterpret_cast<const float*>(x), reinterpret_cast<const float*>(bb),
                          s->lv[l + 1].b32, nullptr, 0.0);
+    if (kind == 5) {   // the last smoothing step of a pass, with its dot partials
+      int np = 0;
+      return launch_tile<T, M_CHEB, true>(s, l, x, bb, up, s->partial, &np, &sc);
+    }
     // as inside the smoother, the new iterate overwrites the previous one
     return launch_tile<T, M_CHEB, false>(s, l, x, bb, up, nullptr, nullptr, &sc);
   };
