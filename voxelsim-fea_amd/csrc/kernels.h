@@ -366,6 +366,9 @@ __device__ __forceinline__ void tile_run(const TileArgs<T>& a, int vt, int slot,
   double dacc = 0.0;
   T ra = T(0), rh = T(0);   // M_RESR: this thread's coarse value and odd-plane partial
 
+  if (RES) {   // the slots of nodes outside the grid are never written
+    for (int i = tid; i < OWN * OWN * 3; i += NT) sRes[i] = T(0);
+  }
   // prologue: plane Z0-1 (bottom of the first layer) and plane Z0
   fetch(Z0 - 1);
   store(0);
@@ -539,16 +542,16 @@ __device__ __forceinline__ void tile_run(const TileArgs<T>& a, int vt, int slot,
         const T so = odd ? T(0.5) : T(1);
         T acc_a = odd ? ra : rh;   // the chain this plane continues
         T acc_h = T(0);
+        // fine node 2 jx + dx sits at slot 2 cjx + 1 + dx of sRes (0 .. 14); a
+        // tap outside the grid reads a slot that no owner writes, 0 since the
+        // kernel's start, so it adds 0 as its weight 0 did
+        const T* const q0 = &sRes[((2 * cjy + 1) * OWN + 2 * cjx + 1) * 3 + c];
 #pragma unroll
         for (int dy = -1; dy <= 1; ++dy)
 #pragma unroll
           for (int dx = -1; dx <= 1; ++dx) {
-            // a tap outside the grid reads the column's own node with weight 0
-            const int fx = 2 * jx + dx, fy = 2 * jy + dy;
-            const bool in = fx >= 0 && fx < a.Lx && fy >= 0 && fy < a.Ly;
-            const int lx = (in ? fx : 2 * jx) - X0, ly = (in ? fy : 2 * jy) - Y0;
-            const T wk = so * (in ? (dx == 0 ? T(1) : T(0.5)) * (dy == 0 ? T(1) : T(0.5)) : T(0));
-            const T qv = sRes[(ly * OWN + lx) * 3 + c];
+            const T wk = so * ((dx == 0 ? T(1) : T(0.5)) * (dy == 0 ? T(1) : T(0.5)));
+            const T qv = q0[(dy * OWN + dx) * 3];
             acc_a = fmaT(wk, qv, acc_a);
             acc_h = fmaT(wk, qv, acc_h);
           }
@@ -579,8 +582,19 @@ __device__ __forceinline__ void tile_run(const TileArgs<T>& a, int vt, int slot,
   }
 }
 
+// Waves per SIMD a tile kernel is compiled for at least.  FP32: 4, which
+// the compiler raises to 5 or 6 where 96 or 80 registers suffice; the
+// pre-smoothing pass (97 registers unconstrained) is held to 5, which it
+// reaches without spilling (170 against 174-178 us per level-0 launch at
+// 256^3).  Holding the operator with its p.Ap partials to 6 spills its FP64
+// accumulator and gains nothing.  FP64: 1.
 template <typename T, int MODE, bool DOT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
+constexpr int tile_waves() {
+  return sizeof(T) != 4 ? 1 : MODE == M_CHEB_Z2 ? 5 : 4;
+}
+
+template <typename T, int MODE, bool DOT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(tile_waves<T, MODE, DOT>())))
 void k_tile(TileArgs<T> a) {
   // XCD-aware tile numbering over (right-hand side, tile)
   const int G = a.gx * a.gy * a.gz;
