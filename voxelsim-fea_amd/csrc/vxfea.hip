@@ -2784,6 +2784,10 @@ int vx_time_kernel(vx_handle s, int32_t kind, int32_t l, int32_t prec, int32_t r
 
 // autotuner: candidates within this fraction of the fastest count as ties
 constexpr float TUNE_TOL = 0.02f;
+// a candidate's timed launches cover about this many node updates at least
+// (reps grow up to 16x on the small levels, whose launches take a few us and
+// would otherwise be timed over a span comparable to the timer's jitter)
+constexpr long long TUNE_MIN_NODES = 2000000;
 
 // ===========================================================================
 // autotuner of the tile kernels' z-chunk
@@ -2818,6 +2822,9 @@ extern "C" int vx_autotune(vx_handle s, int32_t precision, int32_t reps) {
   s->nb = s->opt.max_batch;
   for (size_t li = 0; li < s->lv.size() && rc == VX_OK; ++li) {
     Level& L = s->lv[li];
+    const long long per = (long long)L.Nn * s->nb;
+    const int lreps = reps * (int)std::min<long long>(16, std::max<long long>(
+                                                             1, TUNE_MIN_NODES / std::max(1LL, per)));
     for (int cls = 0; cls < 5 && rc == VX_OK; ++cls) {
       L.zc_tuned[pi][cls] = 0;
       // class 3 (the two Chebyshev steps from zero in one launch) and class 4
@@ -2875,7 +2882,7 @@ extern "C" int vx_autotune(vx_handle s, int32_t precision, int32_t reps) {
         rc = launch();   // warm-up
         if (rc != VX_OK) break;
         if (hipEventRecord(a, s->stream) != hipSuccess) { rc = fail(s, VX_ERR_HIP, "event"); break; }
-        for (int r = 0; r < reps && rc == VX_OK; ++r) rc = launch();
+        for (int r = 0; r < lreps && rc == VX_OK; ++r) rc = launch();
         if (rc != VX_OK) break;
         float ms = 0.f;
         if (hipEventRecord(b, s->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
