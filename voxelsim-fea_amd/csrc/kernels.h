@@ -438,8 +438,11 @@ __device__ __forceinline__ void tile_run(const TileArgs<T>& a, int vt, int slot,
     }
     T Yb[3][4];
     elem_modes(Pb, Pt, Ee, a.K, Yb, Yt);
+    // the diagonal D = kd * (sum of the node's <= 8 element E) of the
+    // smoothing epilogues: z pair in registers, x pair by DPP, y pair by LDS
+    constexpr bool NEEDD = MODE == M_JACOBI || MODE == M_CHEB || RUP;
     const T Es = Eprev + Ee;
-    Eprev = Ee;
+    if (NEEDD) Eprev = Ee;
     T A0[3];
     const int q = j & 1;
 #pragma unroll
@@ -452,8 +455,8 @@ __device__ __forceinline__ void tile_run(const TileArgs<T>& a, int vt, int slot,
       A0[c] = y00 + dpp_shr1(y10);
       sQ[q][c][ty][tx] = y01 + dpp_shr1(y11);
     }
-    const T EX = Es + dpp_shr1(Es);
-    sQ[q][3][ty][tx] = EX;
+    const T EX = NEEDD ? Es + dpp_shr1(Es) : T(0);
+    if (NEEDD) sQ[q][3][ty][tx] = EX;
     // stage the plane after the current top; prefetch the next one
     store((j + 2) & (NBUF - 1));
     fetch(iz + 3);
