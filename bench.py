@@ -41,6 +41,15 @@ from core.multigrid import PLAIN_CYCLE as OPT_PLAIN, VARIABLE_CYCLE as OPT  # no
 # iterations and save 9 % of the time against 30)
 OPT_PARTS = dict(OPT_PLAIN, coarse_sweeps=16)
 
+# the roofline kernel: the longest level-0 launch of a mixed PCG step at 256^3
+# (the last post-smoothing step, a damped-Jacobi step with its r.z partials:
+# u and b in, x out, 3 x FP32 each; the mask byte; E per element; the inverse
+# diagonal is formed from E in the kernel), and its counter summary
+ROOF_KERNEL = "jacobi_dot_fp32"
+ROOF_KERNEL_NAME = ("k_tile<float, M_JACOBI, DOT> (the last post-smoothing step with its r.z "
+                    "partials), level 0")
+TRAFFIC_PROFILE = "profiles/jacobi_dot_fp32_256_traffic.md"
+
 
 def log(msg):
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
@@ -167,7 +176,7 @@ def measured_traffic():
     counter summary (two rocprofv3 --pmc passes, FETCH_SIZE and WRITE_SIZE,
     scaled by a 1 GiB copy); None when the summary is absent."""
     import re
-    path = os.path.join(ROOT, "profiles", "chebyshev_fp32_256_traffic.md")
+    path = os.path.join(ROOT, TRAFFIC_PROFILE)
     try:
         m = re.search(r"= \*\*([0-9.]+) MB\*\*", open(path).read())
         return float(m.group(1)) * 1e6 if m else None
@@ -193,27 +202,29 @@ def copy_bandwidth(nbytes=2 << 30, reps=10):
 
 
 def kernel_only(p, E_dev, act_dev, MGSolver, z_chunk):
-    """The level-0 FP32 Chebyshev tile kernel with its dot partials (the last
-    post-smoothing step of a PCG step) 20 times, the level-0 FP32 operator
-    (masked K u) 20 times, then a plain 1 GiB FP32 copy 5 times (the reference
-    dispatch for counter scaling).  The z-chunk is fixed so that every counter
-    pass sees the same launches."""
+    """The level-0 FP32 tile kernels that take longest in a PCG step, 2 + 20
+    launches each at one fixed z-chunk: the last post-smoothing step (a
+    damped-Jacobi step with its r.z partials), the pre-smoothing pass (two
+    Chebyshev steps from zero in one launch) and the operator (masked K u);
+    then a plain 1 GiB FP32 copy 5 times (the reference dispatch for counter
+    scaling).  The z-chunk is fixed so that every counter pass sees the same
+    launches."""
     import torch
     opt = MGSolver(p.nx, p.ny, p.nz, p.fixed_dofs, nu=p.nu, compute_mode="GPU",
                    **dict(OPT, autotune=False, z_chunk=z_chunk))
     opt.set_density(E_dev, active=act_dev)
-    ms = opt.time_kernel("chebyshev_dot", 0, "mixed", 20)
-    ms_mv = opt.time_kernel("matvec", 0, "mixed", 20)
+    nn = (p.nx + 1) * (p.ny + 1) * (p.nz + 1)
+    rec = {}
+    for kind, nbytes in (("jacobi_dot", 36 * nn + nn + 4 * p.nelem),
+                         ("chebyshev2_zero", 24 * nn + 4 * nn + nn + 4 * p.nelem),
+                         ("matvec", 24 * nn + nn + 4 * p.nelem)):
+        rec[kind] = {"ms": opt.time_kernel(kind, 0, "mixed", 20), "algorithmic_bytes": nbytes}
     a = torch.ones((1 << 30) // 4, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
     for _ in range(5):
         b.copy_(a)
     torch.cuda.synchronize()
-    nn = (p.nx + 1) * (p.ny + 1) * (p.nz + 1)
-    print(json.dumps({"kernel": "chebyshev_dot_fp32_level0", "ms": ms,
-                      "algorithmic_bytes": 48 * nn + nn + 4 * p.nelem,
-                      "matvec_fp32_level0_ms": ms_mv,
-                      "matvec_algorithmic_bytes": 24 * nn + nn + 4 * p.nelem,
+    print(json.dumps({"level0_fp32": rec, "z_chunk": z_chunk,
                       "copy_bytes_read": 1 << 30, "copy_bytes_written": 1 << 30}), flush=True)
 
 
@@ -735,17 +746,15 @@ def main():
         opt.detach_corrector()
         net.close()
 
-    # --- the level-0 FP32 kernels of a PCG step, each at its tuned z-chunk --
-    # The roofline kernel is the most expensive one: the last post-smoothing
-    # step, a Chebyshev step with its r.z partials (u, previous iterate and b
-    # in, x out, 3 x FP32 each; the mask byte; E per element; the inverse
-    # diagonal is formed from E in the kernel)
+    # --- the level-0 FP32 tile kernels of a PCG step, each at its tuned
+    # z-chunk.  The roofline kernel is ROOF_KERNEL, the longest of them in the
+    # solve (profiles/bench_two_material_256_kernel_trace.md)
     reps = 50
     nc = (p.nx // 2 + 1) * (p.ny // 2 + 1) * (p.nz // 2 + 1)
     level0 = {
         # kind: (time_kernel kind, algorithmic bytes per launch)
-        "chebyshev_dot_fp32": ("chebyshev_dot", 48 * nn + nn + 4 * ne),
         "chebyshev2_zero_fp32": ("chebyshev2_zero", 24 * nn + 4 * nn + nn + 4 * ne),
+        "jacobi_dot_fp32": ("jacobi_dot", 36 * nn + nn + 4 * ne),
         "jacobi_fp32": ("jacobi", 36 * nn + nn + 4 * ne),
         "residual_restrict_fp32": ("residual_restrict", 24 * nn + nn + 4 * ne + 12 * nc),
         "matvec_fp32": ("matvec", 24 * nn + nn + 4 * ne),
@@ -755,8 +764,8 @@ def main():
         ms_k = opt.time_kernel(kind, 0, "mixed", reps)
         kernels[name] = {"ms": ms_k, "GBs": nbytes / (ms_k * 1e-3) / 1e9,
                          "algorithmic_bytes": nbytes}
-    ms_jac32 = kernels["chebyshev_dot_fp32"]["ms"]
-    bytes_sweep = kernels["chebyshev_dot_fp32"]["algorithmic_bytes"]
+    ms_jac32 = kernels[ROOF_KERNEL]["ms"]
+    bytes_sweep = kernels[ROOF_KERNEL]["algorithmic_bytes"]
     bytes_mv64 = 48 * nn + nn + 8 * ne
     gbs = bytes_sweep / (ms_jac32 * 1e-3) / 1e9
     opt.close()
@@ -813,9 +822,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gbs / HBM_PEAK_GBS,
                      "traffic": measured_traffic() if args.n == 256 else None,
-                     "traffic_source": "profiles/chebyshev_fp32_256_traffic.md",
-                     "kernel": "k_tile<float, M_CHEB, DOT> (the last post-smoothing step "
-                               "with its r.z partials), level 0",
+                     "traffic_source": TRAFFIC_PROFILE,
+                     "kernel": ROOF_KERNEL_NAME,
                      "kernel_ms": ms_jac32, "algorithmic_bytes": bytes_sweep,
                      "copy_GBs": copy_gbs, "frac_of_copy": gbs / copy_gbs},
         "kernels": dict(kernels, matvec_fp64={"ms": ms_mv64,

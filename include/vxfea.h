@@ -300,8 +300,9 @@ int vx_drop_islands_noalloc(vx_handle h, const uint8_t* active_dev,
  * 2 = Chebyshev smoothing step, 3 = the first two Chebyshev steps from zero in
  * one launch, 4 = the residual restricted to level l+1 in the same launch,
  * 5 = a Chebyshev step with the dot-product partials of the last step of a
- * pass; 3 and 4 FP32 only) in the given precision, measured with HIP events
- * on the handle's stream. */
+ * pass, 6 = a damped-Jacobi step with those partials, the last step of an
+ * FP32 two-step pass from a nonzero start; 3 and 4 FP32 only) in the given
+ * precision, measured with HIP events on the handle's stream. */
 int vx_time_kernel(vx_handle h, int32_t kind, int32_t level, int32_t precision,
                    int32_t reps, double* avg_ms);
 

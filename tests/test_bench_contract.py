@@ -23,7 +23,8 @@ def test_more_than_one_gpu_is_refused():
 def test_traffic_figure_matches_the_counter_summary():
     sys.path.insert(0, ROOT)
     import bench
-    text = open(os.path.join(ROOT, "profiles", "chebyshev_fp32_256_traffic.md")).read()
+    # the summary of the kernel the bench line names as its roofline kernel
+    text = open(os.path.join(ROOT, bench.TRAFFIC_PROFILE)).read()
     mb = float(re.search(r"\*\*([0-9.]+) MB\*\*", text).group(1))
     assert bench.measured_traffic() == pytest.approx(mb * 1e6, rel=1e-9)
 

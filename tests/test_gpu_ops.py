@@ -178,9 +178,9 @@ def test_vcycle_direct_coarse_singular_keeps_sweeps():
 def test_chebyshev_smoother_closed_form(k, power, start):
     """k Chebyshev steps on D^-1 A over [ratio*lam, lam] leave the error
     P_k(D^-1 A) e0 with P_k(x) = T_k((theta - x)/delta) / T_k(theta/delta):
-    checked against that matrix polynomial built densely on a tiny grid.  With
-    cheb_ratio_power p the interval of a k-step pass starts at
-    ratio * (n_smooth / k)^p * lam."""
+    checked against that matrix polynomial built densely on a tiny grid, in
+    FP64 and FP32.  With cheb_ratio_power p the interval of a k-step pass
+    starts at ratio * (n_smooth / k)^p * lam."""
     from core.multigrid import MGSolver
     from oracle.fea_ref import hex8_KE
     from oracle.multigrid_ref import MGSolverRef
@@ -221,6 +221,10 @@ def test_chebyshev_smoother_closed_form(k, power, start):
     mg.set_density(E)
     got = mg._smooth(0, None if start == "zero" else u0, b, k, "fp64")
     assert np.abs(got - want).max() <= 1e-10 * np.abs(want).max()
+    # FP32 (two steps from u run as two damped-Jacobi steps weighted by the
+    # reciprocal roots of the same polynomial): FP32 rounding only
+    got32 = mg._smooth(0, None if start == "zero" else u0, b, k, "mixed")
+    assert np.abs(got32 - want).max() <= 1e-5 * np.abs(want).max()
     mg.close()
 
 

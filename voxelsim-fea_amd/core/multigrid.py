@@ -561,7 +561,8 @@ class MGSolver:
         out = ctypes.c_double()
         self._stream_bind()
         self._check(self._lib.vx_time_kernel(self._h, {"matvec": 0, "jacobi": 1, "chebyshev": 2, "chebyshev2_zero": 3,
-                                              "residual_restrict": 4, "chebyshev_dot": 5}[kind], level,
+                                              "residual_restrict": 4, "chebyshev_dot": 5,
+                                              "jacobi_dot": 6}[kind], level,
                                              _PRECISION[precision], reps, ctypes.byref(out)))
         return out.value
 

@@ -423,6 +423,20 @@ int tsweep(const Tgt& t, const T* u, const T* b, T* out, bool dot, int* n_part, 
   return ttile<T, M_JACOBI, false>(t, u, b, out, nullptr, &sc);
 }
 
+// weight of step i (0, 1) of a two-step Chebyshev pass over [theta - delta,
+// theta + delta] written as two damped-Jacobi steps: 1 / (theta + delta /
+// sqrt 2), then 1 / (theta - delta / sqrt 2)
+static double rich2_weight(double theta, double delta, int i) {
+  const double r = delta / std::sqrt(2.0);
+  return 1.0 / (i == 0 ? theta + r : theta - r);
+}
+
+// the FP32 smoother runs a two-step pass from a nonzero start as two
+// damped-Jacobi steps (rich2_weight): its last step is M_JACOBI
+static bool post_pass_jacobi(const vx_solver* s, int k) {
+  return s->opt.smoother == VX_SMOOTH_CHEBYSHEV && k == 2;
+}
+
 // lower end of the Chebyshev interval (over lambda_max) of a k-step pass
 static double cheb_ratio_for(const vx_solver* s, int k) {
   const int p = s->opt.cheb_ratio_power;
@@ -443,6 +457,16 @@ int tcheb_n(const Tgt& tg, const T* u_in, const T* b, int k, T* dst, T* tmp, T* 
   const double beta = s->lam_max, alpha = cheb_ratio_for(s, k) * beta;
   const double theta = 0.5 * (beta + alpha), delta = 0.5 * (beta - alpha);
   const double sigma = theta / delta;
+  if (std::is_same<T, float>::value && k == 2 && u_in && !first_done) {
+    // FP32, two steps from a nonzero start: the same degree-2 polynomial,
+    // T_2((theta - lambda) / delta) / T_2(theta / delta), as two damped-Jacobi
+    // steps weighted by the reciprocals of its roots theta +- delta / sqrt 2
+    // (the smaller weight first; the other order measured the same), so the
+    // second step reads no previous iterate: one vector less per pass
+    if (tmp == u_in || tmp == dst) return fail(s, VX_ERR_ARG, "chebyshev aliasing");
+    CHK(tsweep<T>(tg, u_in, b, tmp, false, nullptr, rich2_weight(theta, delta, 0)));
+    return tsweep<T>(tg, tmp, b, dst, dot, n_part, rich2_weight(theta, delta, 1));
+  }
   auto target = [&](int j) { return j == k - 1 ? dst : (j % 2 == 0 ? tmp : tmp2); };
   const T* xprev = u_in;
   const T* xcur = u_in;
@@ -2715,7 +2739,7 @@ int vx_drop_islands_noalloc(vx_handle s, const uint8_t* active, const int64_t* f
 int vx_time_kernel(vx_handle s, int32_t kind, int32_t l, int32_t prec, int32_t reps,
                    double* avg_ms) {
   CHK(check_level(s, l));
-  if (reps < 1 || !avg_ms || kind < 0 || kind > 5 || ((kind == 3 || kind == 4) && prec == VX_FP64) ||
+  if (reps < 1 || !avg_ms || kind < 0 || kind > 6 || ((kind == 3 || kind == 4) && prec == VX_FP64) ||
       (kind == 4 && l + 1 >= (int)s->lv.size()))
     return fail(s, VX_ERR_ARG, "bad timing arguments");
   Level& L = s->lv[l];
@@ -2740,6 +2764,10 @@ int vx_time_kernel(vx_handle s, int32_t kind, int32_t l, int32_t prec, int32_t r
     if (kind == 5) {   // the last smoothing step of a pass, with its dot partials
       int np = 0;
       return launch_tile<T, M_CHEB, true>(s, l, x, bb, up, s->partial, &np, &sc);
+    }
+    if (kind == 6) {   // the same as a damped-Jacobi step (FP32 two-step passes)
+      int np = 0;
+      return launch_tile<T, M_JACOBI, true>(s, l, x, bb, y, s->partial, &np, &sc);
     }
     // as inside the smoother, the new iterate overwrites the previous one
     return launch_tile<T, M_CHEB, false>(s, l, x, bb, up, nullptr, nullptr, &sc);
@@ -2870,6 +2898,10 @@ extern "C" int vx_autotune(vx_handle s, int32_t precision, int32_t reps) {
           if (cls == 4)
             return launch_resr(s, (int)li, reinterpret_cast<const float*>(x),
                                reinterpret_cast<const float*>(bb), s->lv[li + 1].b32, nullptr, 0.0);
+          // class 2 on such a level: the last post-smoothing step is a
+          // damped-Jacobi step with the dot partials
+          if (pi == 1 && post_pass_jacobi(s, level_ns(s, (int)li)))
+            return launch_tile<T, M_JACOBI, true>(s, (int)li, x, bb, y, s->partial, &np, &sc);
           return launch_tile<T, M_CHEB, true>(s, (int)li, x, bb, w, s->partial, &np, &sc);
         };
         auto launch = [&]() -> int {
