@@ -42,13 +42,14 @@ from core.multigrid import PLAIN_CYCLE as OPT_PLAIN, VARIABLE_CYCLE as OPT  # no
 OPT_PARTS = dict(OPT_PLAIN, coarse_sweeps=16)
 
 # the roofline kernel: the longest level-0 launch of a mixed PCG step at 256^3
-# (the last post-smoothing step, a damped-Jacobi step with its r.z partials:
-# u and b in, x out, 3 x FP32 each; the mask byte; E per element; the inverse
-# diagonal is formed from E in the kernel), and its counter summary
-ROOF_KERNEL = "jacobi_dot_fp32"
-ROOF_KERNEL_NAME = ("k_tile<float, M_JACOBI, DOT> (the last post-smoothing step with its r.z "
-                    "partials), level 0")
-TRAFFIC_PROFILE = "profiles/jacobi_dot_fp32_256_traffic.md"
+# in the kernel trace (profiles/bench_two_material_256_kernel_trace.md), the
+# pre-smoothing pass: two Chebyshev steps from zero in one launch (b 3 x FP32,
+# D^-1 FP32 and the mask byte per node and E per element in, x 3 x FP32 out),
+# and its counter summary
+ROOF_KERNEL = "chebyshev2_zero_fp32"
+ROOF_KERNEL_NAME = ("k_tile<float, M_CHEB_Z2> (the pre-smoothing pass: two Chebyshev steps from "
+                    "zero in one launch), level 0")
+TRAFFIC_PROFILE = "profiles/level0_fp32_256_traffic.md"
 
 
 def log(msg):
