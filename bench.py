@@ -711,6 +711,7 @@ def main():
     rel_err = float(((u_dev - u64).abs().max() / u64.abs().max()).item())
     mixed_iters, mixed_outer = opt.last_iters, opt.last_outer_iters
     tuned, tune_s = opt.tuned, opt.autotune_s
+    plans = {str(k): list(v) for k, v in opt.block_plans().items()}
     mixed_true, mixed_conv = opt.last_true_resid_ratio, opt.last_converged
 
     # the same solve from host numpy arrays (the reference's calling form):
@@ -816,7 +817,8 @@ def main():
                   "true_resid": mixed_true, "converged": mixed_conv,
                   "rel_disp_diff_vs_fp64": rel_err,
                   "autotuned_zchunk": {str(k): v for k, v in (tuned or {}).items()},
-                  "autotune_s": tune_s},
+                  "autotune_s": tune_s,
+                  "blocked_smoothing_plans": plans},
         "learned": learned,
         "pcie_inclusive_s": t_pcie,
         "device_mem_gb": dev_mem_gb,

@@ -2959,7 +2959,9 @@ extern "C" int vx_autotune(vx_handle s, int32_t precision, int32_t reps) {
         rc = pass();   // warm-up
         if (rc != VX_OK) break;
         if (hipEventRecord(a, s->stream) != hipSuccess) { rc = fail(s, VX_ERR_HIP, "event"); break; }
-        for (int r = 0; r < reps && rc == VX_OK; ++r) rc = pass();
+        // passes of 20-200 us: up to 8x the reps on the smallest levels, so
+        // that plans a tenth apart are told apart run to run
+        for (int r = 0; r < std::min(lreps, 8 * reps) && rc == VX_OK; ++r) rc = pass();
         if (rc != VX_OK) break;
         float ms = 0.f;
         if (hipEventRecord(b, s->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
