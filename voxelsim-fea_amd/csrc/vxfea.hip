@@ -449,7 +449,8 @@ static double cheb_ratio_for(const vx_solver* s, int k) {
 // + (2 rho_j / delta) D^-1 r_j.  u_in == nullptr starts from zero.  The
 // iterates alternate between tmp and tmp2 (a step may overwrite the iterate
 // before the one it reads) and the k-th lands in dst; the first is in tmp
-// for every k >= 2.
+// for every k >= 2.  FP32 with k = 2 from u_in: the same polynomial as two
+// damped-Jacobi steps (rich2_weight), the first in tmp.
 template <typename T, typename Tgt>
 int tcheb_n(const Tgt& tg, const T* u_in, const T* b, int k, T* dst, T* tmp, T* tmp2, bool dot,
             int* n_part, bool first_done = false) {
