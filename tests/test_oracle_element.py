@@ -377,3 +377,29 @@ def test_blocked_smoother_small_division():
         rd = np.float32(1.0) / np.float32(d)
         q = (nf * rd).astype(np.int64)
         assert np.array_equal(q, n // d), d
+
+
+@pytest.mark.parametrize("ratio", [0.05, 0.3])
+def test_two_root_weights_give_the_degree2_chebyshev_polynomial(ratio):
+    """The FP32 smoother runs a two-step Chebyshev pass from a nonzero start
+    as two damped-Jacobi steps weighted 1/(theta + delta/sqrt 2), then
+    1/(theta - delta/sqrt 2).  The error polynomial of that pair equals the
+    pass's T_2((theta - x)/delta) / T_2(theta/delta) and the smoother's
+    three-term recurrence (x1 = x0 + D^-1 r0 / theta, then rho_1 rho_0 and
+    2 rho_1 / delta) on the eigenvalues x of D^-1 A; weights from any other
+    pair of points (here delta/2 in place of delta/sqrt 2) do not."""
+    x = np.linspace(0.0, 1.2, 241)
+    a, b = ratio, 1.0
+    theta, delta = 0.5 * (b + a), 0.5 * (b - a)
+    sigma = theta / delta
+    cheb = (2 * ((theta - x) / delta) ** 2 - 1) / (2 * sigma ** 2 - 1)
+    rho0 = 1.0 / sigma
+    rho1 = 1.0 / (2 * sigma - rho0)
+    e1 = 1 - x / theta
+    three_term = e1 + rho1 * rho0 * (e1 - 1) - (2 * rho1 / delta) * x * e1
+    r = delta / np.sqrt(2.0)
+    two_root = (1 - x / (theta + r)) * (1 - x / (theta - r))
+    assert np.abs(three_term - cheb).max() < 1e-12
+    assert np.abs(two_root - cheb).max() < 1e-12
+    wrong = (1 - x / (theta + delta / 2)) * (1 - x / (theta - delta / 2))
+    assert np.abs(wrong - cheb).max() > 1e-2
