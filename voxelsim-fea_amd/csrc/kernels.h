@@ -333,18 +333,17 @@ __device__ __forceinline__ void tile_run(const TileArgs<T>& a, int vt, int slot,
   // this thread's element column: node (ty, tx) of the footprint, component c
   const int nbase = ty * FOOT + tx;
   const int cbase = 3 * nbase;
-  // M_CHEB_Z2: the four corner nodes' c1 D^-1 and mask of a plane, read once
-  // per layer; the element sees the first step x1 = mask c1 D^-1 b of its corners
+  // M_CHEB_Z2: the four corner nodes' c1 D^-1 of a plane, read once per
+  // layer; the element sees the first step x1 = mask c1 D^-1 b of its
+  // corners.  b is zero at every fixed DOF (the PCG residual, the masked
+  // restriction and the masked operator keep it so), so x1 = c1 D^-1 b needs
+  // no mask test; the node's own mask still gates its output.
   T cw[RUP ? 4 : 1];
-  unsigned cm[RUP ? 4 : 1];
   auto corner_scales = [&](int buf) {
     if (RUP) {
       const int nd[4] = {nbase, nbase + 1, nbase + FOOT, nbase + FOOT + 1};
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        cw[q] = sW[buf][nd[q]];
-        cm[q] = sM[buf][nd[q]];
-      }
+      for (int q = 0; q < 4; ++q) cw[q] = sW[buf][nd[q]];
     }
   };
   auto corners = [&](int buf, int c, T& u00, T& u10, T& u01, T& u11) {
@@ -354,10 +353,10 @@ __device__ __forceinline__ void tile_run(const TileArgs<T>& a, int vt, int slot,
     u01 = P[3 * FOOT];
     u11 = P[3 * FOOT + 3];
     if (RUP) {
-      u00 = ((cm[0] >> c) & 1u) ? cw[0] * u00 : T(0);
-      u10 = ((cm[1] >> c) & 1u) ? cw[1] * u10 : T(0);
-      u01 = ((cm[2] >> c) & 1u) ? cw[2] * u01 : T(0);
-      u11 = ((cm[3] >> c) & 1u) ? cw[3] * u11 : T(0);
+      u00 = cw[0] * u00;
+      u10 = cw[1] * u10;
+      u01 = cw[2] * u01;
+      u11 = cw[3] * u11;
     }
   };
 
